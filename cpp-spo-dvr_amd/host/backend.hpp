@@ -1,0 +1,77 @@
+// The propagation seam: the backend owns the wave packet and the split-operator stages that act on it; the
+// time loop above talks to it once per analysis step. The first seven methods are the reference's
+// PropagationBackend (src/propagation/propagation_backend.hpp:34-59); analyse() and s_matrix_sums() move the
+// Balint-Kurti projection and its FP64 time-to-energy accumulation below the seam, so that only the
+// coefficients (not the analysis plane) cross it.
+#pragma once
+
+#include <functional>
+#include <memory>
+#include <vector>
+
+#include "analysis.hpp"
+#include "config.hpp"
+#include "gas_phase.hpp"
+#include "grids.hpp"
+#include "potential.hpp"
+#include "tables.hpp"
+
+namespace spo {
+
+// Scattering wave in the host layout (irot, ixy, izr), irot fastest, ixy = iy + ny * ix, izr = ir + nr * iz;
+// specular wave (ir, iz) (src/propagation/wave_function.hpp:17-59).
+struct WavePacket {
+  int n_rot = 0, n_xy = 0, n_zr = 0, nr = 0, nz_specular = 0;
+  std::vector<cplx> scattering;
+  std::vector<cplx> specular;
+};
+
+// What a backend is built from; every reference outlives the backend.
+struct BackendInput {
+  const Config& config;
+  const SpatialGrids& grids;
+  const AngularGrid& angular;
+  const RotationalBasis& basis;
+  const PropagationTables& tables;
+  const PotentialGrid& potential;  // V in hartree; the backend premultiplies by -dt itself
+  const GasPhase& gas;
+  const AnalysisBasis& analysis;
+  const AnalysisEnergies& energies;
+  const std::vector<cplx>& initial_specular;
+};
+
+class PropagationBackend {
+ public:
+  virtual ~PropagationBackend() = default;
+  virtual void read_wave_packet(const std::function<void(WavePacket&)>& read) = 0;
+  virtual const WavePacket& wave_packet_on_host() = 0;
+  // T/2 V, (T V) x (n-1), T/2 (src/propagation/split_operator_schedule.cpp:7-13).
+  virtual void propagate_analysis_step() = 0;
+  // One T block of that schedule: specular grid, T_Zr, T_xy, then V unless kinetic_only
+  // (src/propagation/split_operator.cpp:17-21); for stage-level checks.
+  virtual void kinetic_and_potential(bool half_step, bool kinetic_only) = 0;
+  virtual const std::vector<cplx>& specular() const = 0;
+  // The nr blocks of Z index iz in layout (irot, ixy, ir); valid until the next call.
+  virtual const std::vector<cplx>& z_plane(int iz) const = 0;
+  // sum |psi|^2 over (irot, ixy) per izr.
+  virtual std::vector<double> block_densities() const = 0;
+  // n_zr * n_rot values of one (x, y) point, layout (izr, irot).
+  virtual void site_column(int ixy, cplx* out) const = 0;
+  // Balint-Kurti coefficients (is, v) at the analysis line (the initial channel from the specular grid once
+  // the surface is fixed), added to the S-matrix sums with exp(i E t) dtAna (src/analysis/balint_kurti.cpp:42-84,
+  // src/analysis/s_matrix.cpp:49-56). coefficients may be null.
+  virtual void analyse(const AnalysisSurface& surface, double time, std::vector<cplx>* coefficients) = 0;
+  // The FP64 sums (is, v, ien).
+  virtual std::vector<cplx> s_matrix_sums() = 0;
+};
+
+using BackendFactory = std::function<std::unique_ptr<PropagationBackend>(const BackendInput&)>;
+
+// exp(i A) with A = -V dt, (cos A, sin A), over the whole potential grid (src/potential/potential_grid.cpp:164-167,
+// src/propagation/potential_operator.cpp:53-64).
+std::vector<cplx> exp_potential_table(const PotentialGrid& grid, double dt);
+
+// The backend on the GPU (hip_backend.cpp); throws DeviceError / NotSupportedError when no device is usable.
+std::unique_ptr<PropagationBackend> make_hip_backend(const BackendInput& input);
+
+}  // namespace spo

@@ -1,0 +1,80 @@
+#include "driver.hpp"
+
+#include <cmath>
+
+namespace spo {
+
+std::vector<cplx> exp_potential_table(const PotentialGrid& grid, double dt) {
+  std::vector<cplx> table(grid.values.size());
+#pragma omp parallel for schedule(static)
+  for (std::size_t i = 0; i < grid.values.size(); ++i) {
+    const double a = -grid.values[i] * dt;
+    table[i] = cplx(std::cos(a), std::sin(a));
+  }
+  return table;
+}
+
+std::shared_ptr<RunSetup> make_run_setup(const InputParameters& input) {
+  auto s = std::make_shared<RunSetup>();
+  s->config = build_config(input);
+  s->grids = make_spatial_grids(s->config);
+  s->angular = make_angular_grid(s->config);
+  s->basis = make_rotational_basis(s->angular, s->config.input.j_in, s->config.input.mj_in);
+  const auto pes = make_pes(s->config.input.system_id);
+  s->potential = compute_potential_grid(*pes, s->config, s->grids, s->angular);
+  s->gas = compute_gas_phase(s->config, s->angular, s->potential.asymptotic);
+  s->tables = make_propagation_tables(s->config, s->grids, s->basis, s->potential.asymptotic);
+  s->analysis = make_analysis_basis(s->config, s->angular, s->basis, s->gas, s->tables.xy_kinetic_energies);
+  s->energies = make_analysis_energies(s->config, s->gas);
+  s->initial_specular = make_initial_specular_wave(s->config, s->grids, s->gas);
+  return s;
+}
+
+Propagation::Propagation(std::shared_ptr<const RunSetup> setup, const BackendFactory& factory)
+    : setup_(std::move(setup)),
+      backend_(factory(setup_->backend_input())),
+      surface_(setup_->config, setup_->grids, setup_->initial_specular) {
+  const InputParameters& p = setup_->config.input;
+  record_averages(p.t_end - p.n_steps * p.time_step);
+}
+
+int Propagation::analysis_steps_total() const {
+  return setup_->config.input.n_steps / setup_->config.derived.steps_per_analysis;
+}
+
+double Propagation::time() const {
+  const InputParameters& p = setup_->config.input;
+  const int step = done_ * setup_->config.derived.steps_per_analysis;
+  return p.t_end + (step - p.n_steps) * p.time_step;
+}
+
+void Propagation::record_averages(double t) {
+  averages_.push_back(compute_averages(t, backend_->block_densities(), backend_->specular(), setup_->config,
+                                       setup_->grids, surface_.index()));
+}
+
+void Propagation::step() {
+  backend_->propagate_analysis_step();
+  done_ += 1;
+  const double t = time();
+  if (surface_.update(backend_->specular()) && !fixed_time_) fixed_time_ = t;
+  std::vector<cplx> bk;
+  backend_->analyse(surface_, t, keep_bk_ ? &bk : nullptr);
+  if (keep_bk_) bk_.push_back(std::move(bk));
+  if (done_ % setup_->config.derived.analyses_per_average == 0) record_averages(t);
+}
+
+void Propagation::run(int max_steps) {
+  for (int i = 0; i < max_steps && done_ < analysis_steps_total(); ++i) step();
+}
+
+std::vector<EnergyResult> Propagation::results(std::optional<double> specular_line) const {
+  const std::vector<cplx> sums = backend_->s_matrix_sums();
+  std::vector<EnergyResult> out;
+  for (int ien = 0; ien < setup_->energies.size(); ++ien)
+    out.push_back(s_matrix_at(setup_->config, setup_->analysis, setup_->gas, setup_->energies, sums, ien,
+                              specular_line));
+  return out;
+}
+
+}  // namespace spo

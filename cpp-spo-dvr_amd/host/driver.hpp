@@ -1,0 +1,69 @@
+// The set-up of a run from its input (grids, potential, gas phase, tables, analysis basis) and the time loop
+// that drives a backend through the analysis steps (src/propagation/propagation.cpp:9-95).
+#pragma once
+
+#include <memory>
+#include <optional>
+#include <vector>
+
+#include "backend.hpp"
+
+namespace spo {
+
+struct RunSetup {
+  Config config;
+  SpatialGrids grids;
+  AngularGrid angular;
+  RotationalBasis basis;
+  PotentialGrid potential;
+  GasPhase gas;
+  PropagationTables tables;
+  AnalysisBasis analysis;
+  AnalysisEnergies energies;
+  std::vector<cplx> initial_specular;
+
+  BackendInput backend_input() const {
+    return {config, grids, angular, basis, tables, potential, gas, analysis, energies, initial_specular};
+  }
+};
+
+std::shared_ptr<RunSetup> make_run_setup(const InputParameters& input);
+
+class Propagation {
+ public:
+  Propagation(std::shared_ptr<const RunSetup> setup, const BackendFactory& factory);
+
+  const RunSetup& setup() const { return *setup_; }
+  PropagationBackend& backend() { return *backend_; }
+  int analysis_steps_total() const;
+  int analysis_steps_done() const { return done_; }
+  double time() const;
+
+  // One analysis step: propagate, move the analysis surface, project and accumulate; records the coefficients
+  // and, every dtAve, the averages.
+  void step();
+  void run(int max_analysis_steps);
+
+  // Averages at tStart (before any step) and every dtAve.
+  const std::vector<Averages>& averages() const { return averages_; }
+  const std::vector<std::vector<cplx>>& bk_records() const { return bk_; }
+  void keep_bk_records(bool keep) { keep_bk_ = keep; }
+  const AnalysisSurface& surface() const { return surface_; }
+  std::optional<double> surface_fixed_time() const { return fixed_time_; }
+  std::vector<EnergyResult> results(std::optional<double> specular_line) const;
+  std::vector<EnergyResult> results() const { return results(surface_.fixed_line()); }
+
+ private:
+  void record_averages(double time);
+
+  std::shared_ptr<const RunSetup> setup_;
+  std::unique_ptr<PropagationBackend> backend_;
+  AnalysisSurface surface_;
+  int done_ = 0;
+  bool keep_bk_ = true;
+  std::vector<Averages> averages_;
+  std::vector<std::vector<cplx>> bk_;
+  std::optional<double> fixed_time_;
+};
+
+}  // namespace spo

@@ -1,0 +1,52 @@
+// Potential energy surfaces on the propagation path and the potential on the DVR grid of the interaction
+// region.
+#pragma once
+
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "config.hpp"
+#include "grids.hpp"
+#include "symmetry.hpp"
+
+namespace spo {
+
+// V(x, y, Z, r, theta, phi) in eV; (x, y) skewed coordinates along the lattice vectors
+// (src/potential/potential.hpp:25-70).
+class Pes {
+ public:
+  virtual ~Pes() = default;
+  virtual double value(double x, double y, double z, double r, double theta, double phi) const = 0;
+};
+
+// H2 Rydberg V(r) plus a 1/(Z^4 + 0.001) wall switched off by a cosine between Z = 0.1778 and 7
+// (src/potential/hard_wall_potential.cpp:19-51).
+class HardWallPes final : public Pes {
+ public:
+  double value(double x, double y, double z, double r, double theta, double phi) const override;
+  static double gas_phase(double r);
+};
+
+// Dai-Light LEPS surface for H2/Cu(111) (src/potential/dai_light_cu111_potential.cpp:71-118).
+class DaiLightPes final : public Pes {
+ public:
+  double value(double x, double y, double z, double r, double theta, double phi) const override;
+};
+
+std::unique_ptr<Pes> make_pes(const std::string& system_id);
+
+// V in hartree on (itp, irep, izr), itp = iphi + nphi * itheta, izr = ir + nr * iz < nz_potential * nr, and
+// the gas-phase V(r) (src/potential/potential_grid.hpp:14-31).
+struct PotentialGrid {
+  int n_tp = 0, n_rep = 0, n_zr = 0;
+  std::vector<double> values;
+  std::vector<double> asymptotic;
+  GridSymmetry symmetry;
+};
+
+// src/potential/potential_grid.cpp:65-159 (evaluation at the representatives, switching, limits).
+PotentialGrid compute_potential_grid(const Pes& pes, const Config& config, const SpatialGrids& grids,
+                                     const AngularGrid& angular);
+
+}  // namespace spo

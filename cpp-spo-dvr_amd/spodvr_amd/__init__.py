@@ -1,0 +1,263 @@
+"""Python bindings of the MI355X split-operator propagation backend.
+
+The product is two in-tree shared libraries:
+  lib/libspo_hip.so   HIP/CDNA4 kernels behind the C ABI of include/spo_hip.h (the propagation backend);
+  lib/libspo_host.so  the host driver behind include/spo_host.h (set-up from the JSON input, the time loop,
+                      the analysis results), which runs the HIP backend or a backend made by a caller's factory.
+
+This module only loads them with ctypes and converts arrays; it computes nothing itself. A missing library is an
+error: there is no CPU fallback.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import json
+import os
+from typing import Optional
+
+import numpy as np
+
+PACKAGE_DIR = os.path.dirname(os.path.abspath(os.path.dirname(__file__)))
+LIB_DIR = os.path.join(PACKAGE_DIR, "lib")
+
+SPO_DEVICE_HIP = 1
+SPO_DEVICE_EXTERNAL = 2
+
+_FACTORY = ctypes.CFUNCTYPE(ctypes.c_void_p, ctypes.c_void_p)
+
+
+class SpoError(RuntimeError):
+    """A failure reported by the C ABI; code is the status (2 input, 3 not supported, 1 other)."""
+
+    def __init__(self, code: int, message: str):
+        super().__init__(f"[status {code}] {message}")
+        self.code = code
+
+
+class RunInfo(ctypes.Structure):
+    _fields_ = [(name, ctypes.c_int) for name in (
+        "n_rot", "nx", "ny", "nr", "nz", "nz_specular", "nz_potential",
+        "n_theta", "n_phi", "n_representatives", "initial_channel",
+        "n_states", "n_v", "n_energies", "n_diffraction",
+        "steps_per_analysis", "analysis_steps_total", "analyses_per_average", "iz_analysis")] + [
+        ("time_step", ctypes.c_double)]
+
+
+_host = None
+
+
+def library_path(name: str) -> str:
+    return os.path.join(LIB_DIR, name)
+
+
+def host_library() -> ctypes.CDLL:
+    """libspo_host.so (loads libspo_hip.so with it). Raises if it has not been built."""
+    global _host
+    if _host is None:
+        path = library_path("libspo_host.so")
+        if not os.path.exists(path):
+            raise RuntimeError(f"{path} is missing: build the extension first (__graft_entry__.build())")
+        lib = ctypes.CDLL(path, mode=ctypes.RTLD_GLOBAL)
+        lib.spo_last_error.restype = ctypes.c_char_p
+        lib.spo_run_create.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int, ctypes.c_void_p,
+                                       ctypes.POINTER(ctypes.c_void_p)]
+        lib.spo_run_destroy.argtypes = [ctypes.c_void_p]
+        lib.spo_run_get_info.argtypes = [ctypes.c_void_p, ctypes.POINTER(RunInfo)]
+        lib.spo_run_backend.argtypes = [ctypes.c_void_p]
+        lib.spo_run_backend.restype = ctypes.c_void_p
+        for fn in ("spo_run_steps",):
+            getattr(lib, fn).argtypes = [ctypes.c_void_p, ctypes.c_int]
+        lib.spo_run_propagate.argtypes = [ctypes.c_void_p]
+        lib.spo_run_kinetic_and_potential.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]
+        lib.spo_run_analyse.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_void_p]
+        lib.spo_run_get_wave.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+        lib.spo_run_set_wave.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+        lib.spo_run_get_specular.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+        lib.spo_run_block_densities.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+        lib.spo_run_z_plane.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]
+        lib.spo_run_site_column.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]
+        lib.spo_run_bk_record.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]
+        lib.spo_run_bk_count.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int)]
+        lib.spo_run_averages.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
+        lib.spo_run_surface.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
+                                        ctypes.POINTER(ctypes.c_double)]
+        lib.spo_run_results.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
+                                        ctypes.c_void_p, ctypes.c_void_p]
+        lib.spo_run_s_sums.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+        lib.spo_run_array.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_void_p, ctypes.c_size_t,
+                                      ctypes.POINTER(ctypes.c_size_t)]
+        lib.spo_pes_value.argtypes = [ctypes.c_char_p] + [ctypes.c_double] * 6 + [ctypes.POINTER(ctypes.c_double)]
+        _host = lib
+    return _host
+
+
+def _check(status: int) -> None:
+    if status != 0:
+        raise SpoError(status, host_library().spo_last_error().decode(errors="replace"))
+
+
+def _ptr(a: Optional[np.ndarray]):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+def pes_value(system_id: str, x, y, z, r, theta, phi) -> float:
+    out = ctypes.c_double()
+    _check(host_library().spo_pes_value(system_id.encode(), x, y, z, r, theta, phi, ctypes.byref(out)))
+    return out.value
+
+
+class Run:
+    """A propagation run: the set-up from a JSON input (plus overrides) and a backend.
+
+    device="hip" uses the HIP backend; device="external" uses factory, a C function pointer
+    (void* (*)(const void*)) that makes a spo::PropagationBackend (e.g. the CPU oracle's, in tests)."""
+
+    def __init__(self, input_path: str, overrides: Optional[dict] = None, device: str = "hip", factory=None):
+        lib = host_library()
+        handle = ctypes.c_void_p()
+        over = json.dumps(overrides).encode() if overrides else None
+        if device == "hip":
+            code, fptr = SPO_DEVICE_HIP, None
+        elif device == "external":
+            if factory is None:
+                raise ValueError("device='external' needs a factory")
+            code, fptr = SPO_DEVICE_EXTERNAL, ctypes.cast(factory, ctypes.c_void_p)
+        else:
+            raise ValueError(f"unknown device {device!r}")
+        _check(lib.spo_run_create(input_path.encode(), over, code, fptr, ctypes.byref(handle)))
+        self._h = handle
+        info = RunInfo()
+        _check(lib.spo_run_get_info(self._h, ctypes.byref(info)))
+        self.info = info
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            host_library().spo_run_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # shapes
+    @property
+    def n_zr(self) -> int:
+        return self.info.nr * self.info.nz
+
+    @property
+    def n_xy(self) -> int:
+        return self.info.nx * self.info.ny
+
+    @property
+    def scattering_size(self) -> int:
+        return self.info.n_rot * self.n_xy * self.n_zr
+
+    @property
+    def specular_size(self) -> int:
+        return self.info.nr * self.info.nz_specular
+
+    @property
+    def backend_pointer(self) -> int:
+        return host_library().spo_run_backend(self._h)
+
+    # time loop
+    def steps(self, n: int) -> None:
+        _check(host_library().spo_run_steps(self._h, n))
+
+    def propagate(self) -> None:
+        _check(host_library().spo_run_propagate(self._h))
+
+    def kinetic_and_potential(self, half_step: bool, kinetic_only: bool) -> None:
+        _check(host_library().spo_run_kinetic_and_potential(self._h, int(half_step), int(kinetic_only)))
+
+    def analyse(self, t: float) -> np.ndarray:
+        bk = np.zeros(self.info.n_states * self.info.n_v, dtype=np.complex128)
+        _check(host_library().spo_run_analyse(self._h, t, _ptr(bk)))
+        return bk
+
+    # wave packet
+    def get_wave(self):
+        s = np.empty(self.scattering_size, dtype=np.complex128)
+        p = np.empty(self.specular_size, dtype=np.complex128)
+        _check(host_library().spo_run_get_wave(self._h, _ptr(s), _ptr(p)))
+        return s, p
+
+    def set_wave(self, scattering: np.ndarray, specular: np.ndarray) -> None:
+        s = np.ascontiguousarray(scattering, dtype=np.complex128)
+        p = np.ascontiguousarray(specular, dtype=np.complex128)
+        if s.size != self.scattering_size or p.size != self.specular_size:
+            raise ValueError("wave packet has the wrong size")
+        _check(host_library().spo_run_set_wave(self._h, _ptr(s), _ptr(p)))
+
+    def specular(self) -> np.ndarray:
+        p = np.empty(self.specular_size, dtype=np.complex128)
+        _check(host_library().spo_run_get_specular(self._h, _ptr(p)))
+        return p
+
+    def block_densities(self) -> np.ndarray:
+        d = np.empty(self.n_zr, dtype=np.float64)
+        _check(host_library().spo_run_block_densities(self._h, _ptr(d)))
+        return d
+
+    def z_plane(self, iz: int) -> np.ndarray:
+        out = np.empty(self.info.n_rot * self.n_xy * self.info.nr, dtype=np.complex128)
+        _check(host_library().spo_run_z_plane(self._h, iz, _ptr(out)))
+        return out
+
+    def site_column(self, ixy: int) -> np.ndarray:
+        out = np.empty(self.n_zr * self.info.n_rot, dtype=np.complex128)
+        _check(host_library().spo_run_site_column(self._h, ixy, _ptr(out)))
+        return out
+
+    # analysis
+    def bk_records(self) -> np.ndarray:
+        count = ctypes.c_int()
+        _check(host_library().spo_run_bk_count(self._h, ctypes.byref(count)))
+        out = np.zeros((count.value, self.info.n_states * self.info.n_v), dtype=np.complex128)
+        for k in range(count.value):
+            _check(host_library().spo_run_bk_record(self._h, k, _ptr(out[k])))
+        return out
+
+    def averages(self) -> np.ndarray:
+        count = ctypes.c_int()
+        _check(host_library().spo_run_averages(self._h, None, 0, ctypes.byref(count)))
+        rows = np.zeros((max(count.value, 1), 5), dtype=np.float64)
+        _check(host_library().spo_run_averages(self._h, _ptr(rows), count.value, ctypes.byref(count)))
+        return rows[:count.value]
+
+    def surface(self):
+        fixed, index, z = ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
+        _check(host_library().spo_run_surface(self._h, ctypes.byref(fixed), ctypes.byref(index), ctypes.byref(z)))
+        return bool(fixed.value), index.value, z.value
+
+    def results(self, zana_convention: bool = False) -> dict:
+        ne, nk = self.info.n_energies, self.info.n_states * self.info.n_v
+        s = np.zeros((ne, nk), dtype=np.complex128)
+        p = np.zeros((ne, nk), dtype=np.float64)
+        pr = np.zeros(ne, dtype=np.float64)
+        te = np.zeros(ne, dtype=np.float64)
+        _check(host_library().spo_run_results(self._h, int(zana_convention), _ptr(s), _ptr(p), _ptr(pr), _ptr(te)))
+        return {"s": s, "p": p, "p_reaction": pr, "total_energy": te}
+
+    def s_sums(self) -> np.ndarray:
+        out = np.zeros(self.info.n_energies * self.info.n_states * self.info.n_v, dtype=np.complex128)
+        _check(host_library().spo_run_s_sums(self._h, _ptr(out)))
+        return out
+
+    def array(self, name: str) -> np.ndarray:
+        lib = host_library()
+        count = ctypes.c_size_t()
+        _check(lib.spo_run_array(self._h, name.encode(), None, 0, ctypes.byref(count)))
+        is_complex = name == "initial.specular"
+        out = np.zeros(count.value * (2 if is_complex else 1), dtype=np.float64)
+        _check(lib.spo_run_array(self._h, name.encode(), _ptr(out), out.size, ctypes.byref(count)))
+        return out.view(np.complex128) if is_complex else out

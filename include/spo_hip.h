@@ -1,0 +1,125 @@
+/* C ABI of the MI355X split-operator propagation backend (libspo_hip.so).
+ *
+ * It replaces the reference's device backend below the PropagationBackend seam
+ * (src/propagation/propagation_backend.hpp:34-59; the CUDA implementation it stands in for is
+ * src/cuda/cuda_backend.cpp:51-199). Plain pointers and sizes only: complex arrays are interleaved (re, im)
+ * doubles, integer tables are int32. Host layouts are those of src/propagation/wave_function.hpp:12-59:
+ * scattering wave (irot, ixy, izr) with irot fastest, ixy = iy + ny * ix, izr = ir + nr * iz; specular wave
+ * (ir, iz). The device keeps its own layout (izr fastest) and converts at upload / download.
+ *
+ * Every function returns 0 on success or a status (SPO_HIP_E_*) with the message in spo_hip_last_error().
+ * Calls are synchronous: when one returns, the device work it started has finished. One backend is driven
+ * from one host thread at a time. */
+#ifndef SPO_HIP_H
+#define SPO_HIP_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  SPO_HIP_OK = 0,
+  SPO_HIP_E_DEVICE = 1,        /* HIP runtime error (reference: CudaError -> exit 1, src/cuda/cuda_error.hpp:10-13) */
+  SPO_HIP_E_ARGUMENT = 2,      /* inconsistent shapes or tables */
+  SPO_HIP_E_NOT_SUPPORTED = 3  /* no usable device, too little memory, or a shape this build cannot run
+                                  (reference: NotSupportedError -> exit 3, src/app/backend_factory.cpp:27, :52) */
+};
+
+enum { SPO_HIP_FP64 = 0 };
+
+typedef struct spo_hip_backend spo_hip_backend;
+
+/* Grid shape of one propagation (src/propagation/kinetic_operators.hpp:18-24 plus the angular and symmetry
+ * extents). */
+typedef struct spo_hip_shape {
+  int n_rot, nx, ny, nr, nz, nz_specular, nz_potential;
+  int n_theta, n_phi, n_representatives, n_operations;
+  int initial_channel; /* index of (JIn, MJIn) in the rotational basis */
+  int mj_max;          /* the mJ blocks are mJ = -mj_max .. mj_max */
+  int steps_per_analysis;
+  int r_absorber_first, r_absorber_last; /* r points inside the r absorber (specular grid) */
+  int needs_shift;     /* the momentum shift of T_xy is not the identity */
+  int precision;       /* SPO_HIP_FP64 */
+} spo_hip_shape;
+
+/* The factors of the split-operator step (src/propagation/propagation_tables.hpp:20-59), the potential
+ * step's merged factors (src/propagation/potential_operator.hpp:23-26), the angular transform
+ * (src/propagation/angular_transform.hpp:67-75) and exp(i A) with its symmetry maps
+ * (src/symmetry/grid_symmetry.hpp:60-70). */
+typedef struct spo_hip_tables {
+  const double* zr_full; const double* zr_half;             /* (ir, iz), nr*nz complex */
+  const double* xy_full; const double* xy_half;             /* ixy, complex */
+  const double* specular_full; const double* specular_half; /* (ir, iz), nr*nz_specular complex */
+  double fft_scale_z, fft_scale_r, fft_scale_x, fft_scale_y;
+  const double* rotation_half;      /* (irot, ir) complex: exp(-i E_rot dt/2) */
+  const double* before_interaction; /* (irot, ir) complex: exp(-V_opt,r dt) rot(dt/2) */
+  const double* gas_phase_step;     /* (irot, ir) complex: exp(-V_opt,r dt) rot exp(-i vasy dt) rot */
+  const double* specular_gas_phase; /* ir complex: rot exp(-i vasy dt) rot of the initial channel */
+  const double* z_optical;          /* nz real */
+  const double* r_optical;          /* nr real */
+  const double* z_specular_optical; /* nz_specular real */
+  const double* shift_forward; const double* shift_backward;                  /* ixy complex */
+  const double* specular_to_scattering; const double* scattering_to_specular; /* ixy complex */
+  const double* legendre;       /* sqrt(w_t / nphi) P_J^m(theta_t), ordered (mJ block, theta, J) */
+  const int* block_first;       /* per mJ block: first basis index */
+  const int* block_size;        /* per mJ block: number of J */
+  const double* exp_potential;  /* exp(i A), A = -V dt, (itp, irep, izr) complex, izr < nz_potential * nr */
+  const int* representative_of; /* per ixy */
+  const int* operation_of;      /* per ixy */
+  const int* phi_index;         /* (iphi, operation) */
+} spo_hip_tables;
+
+/* The Balint-Kurti analysis (src/analysis/balint_kurti.cpp:42-84) and the time-to-energy sums
+ * (src/analysis/s_matrix.cpp:49-56). Coefficients are (is, v) with is = irotana * n_diffraction + idif. */
+typedef struct spo_hip_analysis {
+  int n_states, n_v, n_rotation, n_diffraction, n_energies;
+  int iz_analysis, initial_state;
+  const int* rotation_index;    /* per analysed rotational state: index in the basis */
+  const int* diffraction_slot;  /* per analysed channel: iy + ny * ix of the (x, y) FFT output */
+  const double* radial_weight;  /* (ir, irotana, v): phi_{v, j(irotana)}(r) * dR */
+  const double* initial_radial; /* (ir, v): phi_{v, JIn}(r) */
+  double dr, xy_scale, analysis_time_step;
+  const double* total_energy_hartree; /* per energy */
+} spo_hip_analysis;
+
+const char* spo_hip_last_error(void);
+
+/* Number of HIP devices visible (0 on a machine without a GPU; never an error). */
+int spo_hip_device_count(int* count);
+
+/* Uploads the tables and the initial specular wave (ir, iz); the scattering wave starts at zero. */
+int spo_hip_create(const spo_hip_shape* shape, const spo_hip_tables* tables, const spo_hip_analysis* analysis,
+                   const double* initial_specular, spo_hip_backend** out);
+int spo_hip_destroy(spo_hip_backend* backend);
+
+/* Full wave packet, host layouts (replaces read_wave_packet / wave_packet_on_host). */
+int spo_hip_upload_wave(spo_hip_backend* backend, const double* scattering, const double* specular);
+int spo_hip_download_wave(spo_hip_backend* backend, double* scattering, double* specular);
+
+/* T(dt/2) V, (T V) x (n-1), T(dt/2) (src/propagation/split_operator_schedule.cpp:7-13). */
+int spo_hip_propagate_analysis_step(spo_hip_backend* backend);
+/* One T block: specular grid, T_Zr, T_xy, then V unless kinetic_only (src/propagation/split_operator.cpp:17-21). */
+int spo_hip_kinetic_and_potential(spo_hip_backend* backend, int half_step, int kinetic_only);
+
+int spo_hip_get_specular(spo_hip_backend* backend, double* specular);
+/* nr blocks of Z index iz, layout (irot, ixy, ir). */
+int spo_hip_get_z_plane(spo_hip_backend* backend, int iz, double* out);
+/* sum |psi|^2 over (irot, ixy) per izr, n_zr values. */
+int spo_hip_block_densities(spo_hip_backend* backend, double* out);
+/* One (x, y) point, n_zr * n_rot complex values, layout (izr, irot). */
+int spo_hip_site_column(spo_hip_backend* backend, int ixy, double* out);
+
+/* Balint-Kurti coefficients at the analysis line; with surface_fixed the initial state's from the specular wave
+ * at iz_specular. Then sums(is, v, ien) += c(is, v) exp(i E t) dtAna in FP64. coefficients (n_states * n_v
+ * complex) may be NULL. */
+int spo_hip_analyse(spo_hip_backend* backend, int surface_fixed, int iz_specular, double t, double* coefficients);
+/* The sums, (is, v, ien) with is fastest. */
+int spo_hip_get_s_sums(spo_hip_backend* backend, double* out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

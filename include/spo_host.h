@@ -1,0 +1,106 @@
+/* C ABI of the host driver (libspo_host.so): the set-up of a run from its JSON input, the time loop over a
+ * propagation backend and the analysis results. Complex arrays are interleaved (re, im) doubles; layouts are
+ * those of the reference's host side (src/propagation/wave_function.hpp:12-59).
+ *
+ * Every function returns 0 on success or a status (SPO_E_*), with the message in spo_last_error(). The status
+ * codes are the reference program's exit codes for the same failures (src/app/run.cpp:596-607). */
+#ifndef SPO_HOST_H
+#define SPO_HOST_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  SPO_OK = 0,
+  SPO_E_OTHER = 1,         /* device or other runtime error */
+  SPO_E_INPUT = 2,         /* invalid input */
+  SPO_E_NOT_SUPPORTED = 3  /* input or device not supported by this build */
+};
+
+enum {
+  SPO_DEVICE_HIP = 1,      /* the HIP backend (libspo_hip.so) */
+  SPO_DEVICE_EXTERNAL = 2  /* a backend made by the caller's factory */
+};
+
+typedef struct spo_run spo_run;
+
+/* A backend factory: input is a const spo::BackendInput*, the result a new spo::PropagationBackend* (owned by
+ * the run afterwards), or NULL on failure. Replaces the reference's BackendFactory
+ * (src/propagation/propagation_backend.hpp:73-74) at the C boundary. */
+typedef void* (*spo_backend_factory_fn)(const void* backend_input);
+
+/* Shapes and counts of a run. */
+typedef struct spo_run_info {
+  int n_rot, nx, ny, nr, nz, nz_specular, nz_potential;
+  int n_theta, n_phi, n_representatives, initial_channel;
+  int n_states, n_v, n_energies, n_diffraction;
+  int steps_per_analysis, analysis_steps_total, analyses_per_average;
+  int iz_analysis;
+  double time_step;
+} spo_run_info;
+
+const char* spo_last_error(void);
+
+/* Reads input_path (JSON, the reference's keys), merges overrides_json (a JSON object, may be NULL), builds the
+ * set-up and the backend. factory is used with SPO_DEVICE_EXTERNAL only. */
+int spo_run_create(const char* input_path, const char* overrides_json, int device, spo_backend_factory_fn factory,
+                   spo_run** out);
+int spo_run_destroy(spo_run* run);
+int spo_run_get_info(const spo_run* run, spo_run_info* info);
+/* The backend object (a spo::PropagationBackend*), for stage-level checks. */
+void* spo_run_backend(spo_run* run);
+
+/* Analysis steps of the time loop: propagate, move the analysis surface, project and accumulate. */
+int spo_run_steps(spo_run* run, int n_analysis_steps);
+/* The backend's propagate_analysis_step() alone (no analysis). */
+int spo_run_propagate(spo_run* run);
+/* One T block of the schedule (specular grid, T_Zr, T_xy, then V unless kinetic_only) on the backend. */
+int spo_run_kinetic_and_potential(spo_run* run, int half_step, int kinetic_only);
+/* Analysis at time t of the current wave packet (surface update, projection, accumulation); bk may be NULL,
+ * else n_states * n_v complex values (is, v). */
+int spo_run_analyse(spo_run* run, double t, double* bk);
+
+/* Wave packet in host layout: scattering (irot, ixy, izr) n_rot*nx*ny*nr*nz complex, specular (ir, iz). */
+int spo_run_get_wave(spo_run* run, double* scattering, double* specular);
+int spo_run_set_wave(spo_run* run, const double* scattering, const double* specular);
+int spo_run_get_specular(spo_run* run, double* specular);
+int spo_run_block_densities(spo_run* run, double* out);               /* n_zr values */
+int spo_run_z_plane(spo_run* run, int iz, double* out);                /* n_rot*nx*ny*nr complex, (irot, ixy, ir) */
+int spo_run_site_column(spo_run* run, int ixy, double* out);           /* n_zr*n_rot complex, (izr, irot) */
+
+/* Recorded BK coefficients of analysis step k (0-based), n_states*n_v complex. */
+int spo_run_bk_record(const spo_run* run, int k, double* out);
+int spo_run_bk_count(const spo_run* run, int* count);
+/* Averages rows so far: count, then rows of (time, norm, z, r, rest). */
+int spo_run_averages(const spo_run* run, double* rows, int capacity, int* count);
+/* The analysis surface: fixed flag, index and Z of the fixed line (or 0). */
+int spo_run_surface(const spo_run* run, int* fixed, int* index, double* z_line);
+
+/* S-matrix results at every energy from the sums so far. s: n_energies*n_states*n_v complex (ien, v, is) with is
+ * fastest; p likewise (doubles); p_reaction: n_energies; total_energy: n_energies (eV). zana_convention != 0
+ * refers the initial state's phase to Zana instead of the fixed specular line. Any pointer may be NULL. */
+int spo_run_results(const spo_run* run, int zana_convention, double* s, double* p, double* p_reaction,
+                    double* total_energy);
+/* The raw FP64 sums (ien, v, is), n_energies*n_states*n_v complex. */
+int spo_run_s_sums(const spo_run* run, double* out);
+
+/* Set-up arrays by name, for checks of the set-up: "gas.energies" (j + (jmax+1) v), "gas.wave_functions"
+ * (ir, j, v), "potential.values" (itp, irep, izr), "potential.asymptotic", "angular.thetas", "angular.weights",
+ * "angular.phis", "basis.j", "basis.mj", "state.j", "state.mj", "state.n", "state.m", "state.order",
+ * "level.energies" (E(v, j) in eV, j + (jmax+1) v), "symmetry.representatives", "tables.z_optical",
+ * "tables.r_optical", "initial.specular" (complex). count receives the element count (doubles, or complex
+ * values for complex arrays); out may be NULL to query the count. */
+int spo_run_array(const spo_run* run, const char* name, double* out, size_t capacity, size_t* count);
+
+/* V(x, y, Z, r, theta, phi) in eV of a potential by SystemID. */
+int spo_pes_value(const char* system_id, double x, double y, double z, double r, double theta, double phi,
+                  double* out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,89 @@
+"""Input files of the test and benchmark workloads (the reference's JSON keys).
+
+GOLDEN_DIR holds the inputs of the reference's Fortran-gated cases, copied as data together with the extracted
+Fortran outputs (tests/golden/make_fixtures.py). The dictionaries below are this repository's own workloads:
+  SMOKE_DAILIGHT  a small Dai-Light H2/Cu(111) grid with full angular coupling (seconds on the CPU oracle);
+  HARDWALL_6D     the HardWallPES workload: NZ 128 on Z in [0.5, 20], NR 32 on r in [0.6, 4.5], 8 x 8 (x, y),
+                  JMax 10, 2000 steps of 2 a.u.;
+  DZ_6D           the Dai-Light H2/Cu(111) LEPS workload: NZ 96 on Z in [-1, 12], NR 48 on r in [0.8, 6],
+                  16 x 16 (x, y), JMax 24 with MJMax 23 (13 theta x 48 phi), switched to the gas phase between
+                  Z = 6 and 8, C6v symmetry-reduced potential, specular grid of 512 Z points.
+"""
+
+from __future__ import annotations
+
+import json
+import os
+import tempfile
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+GOLDEN_DIR = os.path.join(HERE, "golden")
+
+_COMMON = {
+    "NewCalc": True,
+    "ZDoF": True, "RDoF": True, "XDoF": True, "YDoF": True, "ThetaDoF": True, "PhiDoF": True,
+    "Mass1": 1.0078, "Mass2": 1.0078, "HomoNuc": True,
+    "ProjOp": 2, "tStart": 0.0, "DynamZSpAna": True,
+    "VIn": 0, "JIn": 0, "MJIn": 0, "EParallel": 0.0, "PhiParallel": 0.0,
+    "TolSp": 1.0e-7,
+    "writeSMatrix": False, "writeProbs": True, "writeRDProbs": True,
+}
+
+SMOKE_DAILIGHT = dict(_COMMON, **{
+    "Title": "small Dai-Light H2/Cu(111) grid with angular coupling",
+    "SystemID": "H2Cu111DaiLight",
+    "ZStart": -1.0, "dZ": 0.16, "nZ": 64, "nZSpecular": 128,
+    "RStart": 0.8, "dR": 0.3, "nR": 16,
+    "nX": 4, "nY": 4, "JMax": 4, "MJMax": 4,
+    "tEnd": 80.0, "dtStep": 2.0, "nStep": 40,
+    "dtAna": 10.0, "dtAve": 20.0, "Zana": 7.5, "NrOfEs": 21,
+    "EMinWP": 0.2, "EMaxWP": 0.6, "Z0": 11.0,
+    "ZOptStart": 8.0, "ZOptEnd": 9.08, "ZOptEnergy": 0.3,
+    "ZSpOptStart": 15.0, "ZSpOptEnd": 19.32, "ZSpOptEnergy": 0.3,
+    "ROptStart": 4.0, "ROptEnd": 5.3, "ROptEnergy": 0.3,
+    "UseSwitching": True, "ZminSwitch": 5.0, "ZmaxSwitch": 7.0,
+})
+
+HARDWALL_6D = dict(_COMMON, **{
+    "Title": "HardWallPES 6D workload",
+    "SystemID": "HardWallPES",
+    "ZStart": 0.5, "dZ": 19.5 / 127, "nZ": 128, "nZSpecular": 192,
+    "RStart": 0.6, "dR": 3.9 / 31, "nR": 32,
+    "nX": 8, "nY": 8, "JMax": 10, "MJMax": 10,
+    "tEnd": 4000.0, "dtStep": 2.0, "nStep": 2000,
+    "dtAna": 10.0, "dtAve": 100.0, "Zana": 10.0, "NrOfEs": 200,
+    "EMinWP": 0.05, "EMaxWP": 1.0, "Z0": 14.0,
+    "ZOptStart": 16.1, "ZOptEnd": 20.0, "ZOptEnergy": 0.3,
+    "ZSpOptStart": 24.0, "ZSpOptEnd": 29.8, "ZSpOptEnergy": 0.3,
+    "ROptStart": 3.72, "ROptEnd": 4.5, "ROptEnergy": 0.3,
+})
+
+DZ_6D = dict(_COMMON, **{
+    "Title": "Dai-Light H2/Cu(111) 6D workload",
+    "SystemID": "H2Cu111DaiLight",
+    "ZStart": -1.0, "dZ": 13.0 / 95, "nZ": 96, "nZSpecular": 512,
+    "RStart": 0.8, "dR": 5.2 / 47, "nR": 48,
+    "nX": 16, "nY": 16, "JMax": 24, "MJMax": 23,
+    "tEnd": 40000.0, "dtStep": 2.0, "nStep": 20000,
+    "dtAna": 10.0, "dtAve": 100.0, "Zana": 8.5, "NrOfEs": 100,
+    "EMinWP": 0.2, "EMaxWP": 1.0, "Z0": 14.0,
+    "ZOptStart": 9.5, "ZOptEnd": 12.0, "ZOptEnergy": 0.5,
+    "ZSpOptStart": 40.0, "ZSpOptEnd": 68.9, "ZSpOptEnergy": 0.5,
+    "ROptStart": 4.5, "ROptEnd": 6.0, "ROptEnergy": 0.5,
+    "UseSwitching": True, "ZminSwitch": 6.0, "ZmaxSwitch": 8.0,
+    "UseSymmetry": True,
+})
+
+
+def write_case(name: str, params: dict, directory: str | None = None) -> str:
+    """Writes params as <directory>/<name>.json (a temporary directory by default) and returns the path."""
+    directory = directory or os.path.join(tempfile.gettempdir(), "spodvr_cases")
+    os.makedirs(directory, exist_ok=True)
+    path = os.path.join(directory, f"{name}.json")
+    with open(path, "w") as f:
+        json.dump(params, f, indent=1)
+    return path
+
+
+def golden_input(case: str) -> str:
+    return os.path.join(GOLDEN_DIR, case, "input.json")

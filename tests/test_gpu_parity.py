@@ -1,0 +1,76 @@
+"""HIP backend against the CPU oracle, from the same seeded pseudo-random wave packet (the reference's
+CPU-vs-GPU check, tests/cuda/test_cuda_backend.cpp:94-106: max |a - b| / max |a| per stage, 1e-13 there).
+
+Every test prints the case, the seed and the stage before touching the GPU."""
+
+import sys
+
+import numpy as np
+import pytest
+
+from tests import cases
+from tests.util import pseudo_random, relative_difference
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-12
+
+CASE_PARAMS = {
+    "smoke": lambda: cases.write_case("smoke_dailight", cases.SMOKE_DAILIGHT),
+    "hardwall_short": lambda: cases.golden_input("hardwall_short"),
+    "cu111_dailight_short": lambda: cases.golden_input("cu111_dailight_short"),
+    "cu111_dailight_short_sym": lambda: cases.golden_input("cu111_dailight_short_sym"),
+    "cu111_dailight_oblique": lambda: cases.golden_input("cu111_dailight_oblique"),
+}
+
+
+def _pair(case, overrides=None):
+    import spodvr_amd
+    from oracle import oracle
+    path = CASE_PARAMS[case]()
+    gpu = spodvr_amd.Run(path, overrides, device="hip")
+    cpu = oracle.run(path, overrides)
+    return gpu, cpu
+
+
+def _randomise(gpu, cpu, seed):
+    s = pseudo_random(gpu.scattering_size, seed)
+    p = pseudo_random(gpu.specular_size, seed + 1000)
+    gpu.set_wave(s, p)
+    cpu.set_wave(s, p)
+    return s, p
+
+
+@pytest.mark.parametrize("case", list(CASE_PARAMS))
+@pytest.mark.parametrize("half,kinetic_only", [(False, False), (True, False), (True, True)])
+def test_one_t_block_matches_oracle(case, half, kinetic_only):
+    seed = 1 + 2 * int(half) + int(kinetic_only)
+    print(f"case {case} seed {seed} half {half} kinetic_only {kinetic_only}", flush=True)
+    gpu, cpu = _pair(case)
+    s0, p0 = _randomise(gpu, cpu, seed)
+    gpu.kinetic_and_potential(half, kinetic_only)
+    cpu.kinetic_and_potential(half, kinetic_only)
+    sg, pg = gpu.get_wave()
+    sc, pc = cpu.get_wave()
+    ds, dp = relative_difference(sc, sg), relative_difference(pc, pg)
+    print(f"  scattering {ds:.3e} specular {dp:.3e}", flush=True)
+    assert ds < TOL and dp < TOL
+    assert relative_difference(s0, sc) > 1e-3 and relative_difference(p0, pc) > 1e-3
+
+
+@pytest.mark.parametrize("case", ["smoke", "cu111_dailight_short_sym", "cu111_dailight_oblique"])
+def test_analysis_step_matches_oracle(case):
+    print(f"case {case} seed 11 analysis step", flush=True)
+    gpu, cpu = _pair(case)
+    _randomise(gpu, cpu, 11)
+    gpu.propagate()
+    cpu.propagate()
+    bg, bc = gpu.analyse(10.0), cpu.analyse(10.0)
+    db = relative_difference(bc, bg)
+    dsum = relative_difference(cpu.s_sums(), gpu.s_sums())
+    dens = relative_difference(cpu.block_densities(), gpu.block_densities())
+    print(f"  bk {db:.3e} sums {dsum:.3e} densities {dens:.3e}", flush=True)
+    assert db < TOL and dsum < TOL and dens < TOL
+    iz = gpu.info.iz_analysis
+    assert relative_difference(cpu.z_plane(iz), gpu.z_plane(iz)) < TOL
+    assert relative_difference(cpu.site_column(3), gpu.site_column(3)) < TOL
