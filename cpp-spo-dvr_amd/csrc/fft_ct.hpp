@@ -1,0 +1,252 @@
+// Compile-time-shaped Stockham FFTs over sequences staged in LDS (the fast path of the hot kernels).
+//
+// The length N, its factorisation into radices {2, 3, 4, 5, 6, 7, 8, 10, 11, 12, 16} (fewest stages, larger
+// radices first), the sequence layout and the thread count are template parameters, so every index computation
+// divides by constants and every butterfly's values live in registers. A stage is one read-all / barrier /
+// write-all round over whole sequences (in place). Twiddles between stages come from the table
+// W[m] = exp(-2 pi i m / N) (host-built in long double); twiddles inside composite radices are compile-time
+// constants. The exponent sign is a run-time argument and only conjugates.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <utility>
+
+#include "fft_lds.hpp"
+
+namespace spo_dev {
+
+// ---- compile-time unit roots (cos, sin)(2 pi m / R), exact at the quarter turns --------------------------------
+struct CS {
+  double c, s;
+};
+
+constexpr double ct_sin_series(double x) {
+  double term = x, sum = x;
+  for (int k = 1; k < 14; ++k) {
+    term = -term * x * x / ((2 * k) * (2 * k + 1));
+    sum += term;
+  }
+  return sum;
+}
+constexpr double ct_cos_series(double x) {
+  double term = 1.0, sum = 1.0;
+  for (int k = 1; k < 14; ++k) {
+    term = -term * x * x / ((2 * k - 1) * (2 * k));
+    sum += term;
+  }
+  return sum;
+}
+constexpr CS unit_root(int m, int R) {
+  m %= R;
+  if (m < 0) m += R;
+  const int q = (4 * m) / R;  // quarter turns
+  const int r = 4 * m - q * R;  // angle within the quarter: (pi/2) r / R
+  constexpr double half_pi = 1.57079632679489661923;
+  double c = 1.0, s = 0.0;
+  if (r == 0) {
+    c = 1.0;
+    s = 0.0;
+  } else if (2 * r <= R) {
+    const double t = half_pi * r / R;
+    c = ct_cos_series(t);
+    s = ct_sin_series(t);
+  } else {
+    const double t = half_pi * (R - r) / R;
+    c = ct_sin_series(t);
+    s = ct_cos_series(t);
+  }
+  switch (q) {
+    case 1: return {-s, c};
+    case 2: return {-c, -s};
+    case 3: return {s, -c};
+    default: return {c, s};
+  }
+}
+
+// ---- compile-time factorisation -------------------------------------------------------------------------------
+struct Factorisation {
+  int count = 0;
+  int radix[8] = {};
+};
+
+constexpr bool is_radix(int r) {
+  return r == 2 || r == 3 || r == 4 || r == 5 || r == 6 || r == 7 || r == 8 || r == 10 || r == 11 || r == 12 ||
+         r == 16;
+}
+
+// Fewest stages; among those, the lexicographically largest radix sequence.
+constexpr Factorisation factorise(int n) {
+  Factorisation best;
+  best.count = 99;
+  if (n == 1) {
+    best.count = 0;
+    return best;
+  }
+  for (int r = 16; r >= 2; --r) {
+    if (!is_radix(r) || n % r != 0) continue;
+    const Factorisation rest = factorise(n / r);
+    if (rest.count + 1 < best.count) {
+      best.count = rest.count + 1;
+      best.radix[0] = r;
+      for (int i = 0; i < rest.count; ++i) best.radix[i + 1] = rest.radix[i];
+    }
+  }
+  return best;
+}
+
+template <int N>
+struct CtPlan {
+  static constexpr Factorisation f = factorise(N);
+  static_assert(f.count > 0 && f.count < 8, "FFT length not supported by the compile-time engine");
+  static constexpr int stages = f.count;
+  static constexpr int radix(int i) { return f.radix[i]; }
+  static constexpr int span(int i) {
+    int s = 1;
+    for (int k = 0; k < i; ++k) s *= f.radix[k];
+    return s;
+  }
+};
+
+// ---- register DFTs --------------------------------------------------------------------------------------------
+template <int... I, class F>
+__device__ __forceinline__ void static_for_impl(std::integer_sequence<int, I...>, F&& f) {
+  (f(std::integral_constant<int, I>{}), ...);
+}
+template <int N, class F>
+__device__ __forceinline__ void static_for(F&& f) {
+  static_for_impl(std::make_integer_sequence<int, N>{}, f);
+}
+
+template <int R, typename T>
+__device__ __forceinline__ void rdft(cx<T>* v, int sign);
+
+// y[k1 + R1 k2] = sum_{n2} w_R^{n2 k1} (sum_{n1} x[R2 n1 + n2] w_R1^{n1 k1}) w_R2^{n2 k2}, R = R1 R2.
+template <int R1, int R2, typename T>
+__device__ __forceinline__ void rdft_composite(cx<T>* v, int sign) {
+  constexpr int R = R1 * R2;
+  cx<T> a[R2][R1];
+#pragma unroll
+  for (int n2 = 0; n2 < R2; ++n2) {
+#pragma unroll
+    for (int n1 = 0; n1 < R1; ++n1) a[n2][n1] = v[R2 * n1 + n2];
+    rdft<R1, T>(a[n2], sign);
+  }
+  // Internal twiddles w_R^{n2 k1}, n2, k1 >= 1, as compile-time constants.
+  static_for<R2 * R1>([&](auto idx) {
+    constexpr int n2 = decltype(idx)::value / R1;
+    constexpr int k1 = decltype(idx)::value % R1;
+    if constexpr (n2 > 0 && k1 > 0) {
+      constexpr CS w = unit_root(n2 * k1, R);
+      const cx<T> t = {T(w.c), T(sign) * T(w.s)};
+      a[n2][k1] = a[n2][k1] * t;
+    }
+  });
+#pragma unroll
+  for (int k1 = 0; k1 < R1; ++k1) {
+    cx<T> b[R2];
+#pragma unroll
+    for (int n2 = 0; n2 < R2; ++n2) b[n2] = a[n2][k1];
+    rdft<R2, T>(b, sign);
+#pragma unroll
+    for (int k2 = 0; k2 < R2; ++k2) v[k1 + R1 * k2] = b[k2];
+  }
+}
+
+template <int R, typename T>
+__device__ __forceinline__ void rdft(cx<T>* v, int sign) {
+  if constexpr (R == 1) {
+  } else if constexpr (R == 2 || R == 3 || R == 4 || R == 5 || R == 7 || R == 8 || R == 11) {
+    dft<R, T>(v, sign);
+  } else if constexpr (R == 6) {
+    rdft_composite<3, 2, T>(v, sign);
+  } else if constexpr (R == 10) {
+    rdft_composite<5, 2, T>(v, sign);
+  } else if constexpr (R == 12) {
+    rdft_composite<4, 3, T>(v, sign);
+  } else if constexpr (R == 16) {
+    rdft_composite<4, 4, T>(v, sign);
+  }
+}
+
+// ---- one stage, whole sequences per round ---------------------------------------------------------------------
+// Sequence s starts at (s % INNER) * IS + (s / INNER) * OS (INNER = 0: at s * IS); elements ES apart.
+template <int INNER, int IS, int OS, int ES>
+struct CtLayout {
+  static constexpr int es = ES;
+  static __device__ __forceinline__ int base(int s) {
+    if constexpr (INNER == 0) {
+      return s * IS;
+    } else {
+      return (s % INNER) * IS + (s / INNER) * OS;
+    }
+  }
+};
+
+template <typename T>
+constexpr int ct_max_values() {
+  return sizeof(T) == 8 ? 16 : 32;
+}
+
+template <typename T, int N, int STAGE, int THREADS, class Layout>
+__device__ __forceinline__ void ct_stage(cx<T>* buf, int count, const cx<T>* table, int sign) {
+  using P = CtPlan<N>;
+  constexpr int R = P::radix(STAGE);
+  constexpr int SPAN = P::span(STAGE);
+  constexpr int M = N / R;
+  constexpr int TW = N / (SPAN * R);
+  constexpr int ES = Layout::es;
+  constexpr int NB_MAX = ct_max_values<T>() / R > 0 ? ct_max_values<T>() / R : 1;
+  constexpr int SEQ_PER_ROUND = (THREADS * NB_MAX) / M > 0 ? (THREADS * NB_MAX) / M : 1;
+  const int tid = threadIdx.x;
+  for (int s0 = 0; s0 < count; s0 += SEQ_PER_ROUND) {
+    const int n_seq = min(SEQ_PER_ROUND, count - s0);
+    const int total = n_seq * M;
+    constexpr int NB = (SEQ_PER_ROUND * M + THREADS - 1) / THREADS;
+    cx<T> v[NB][R];
+    int out[NB];
+#pragma unroll
+    for (int q = 0; q < NB; ++q) {
+      const int b = tid + q * THREADS;
+      out[q] = -1;
+      if (b < total) {
+        const int s = s0 + b / M;
+        const int j = b % M;
+        const int k = j % SPAN;
+        const int base = Layout::base(s);
+#pragma unroll
+        for (int r = 0; r < R; ++r) v[q][r] = buf[base + (j + r * M) * ES];
+        if constexpr (SPAN > 1) {
+          if (k > 0) {
+#pragma unroll
+            for (int r = 1; r < R; ++r) {
+              cx<T> w = table[r * k * TW];
+              if (sign > 0) w = conj(w);
+              v[q][r] = v[q][r] * w;
+            }
+          }
+        }
+        rdft<R, T>(v[q], sign);
+        out[q] = base + ((j / SPAN) * SPAN * R + k) * ES;
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < NB; ++q)
+      if (out[q] >= 0) {
+#pragma unroll
+        for (int r = 0; r < R; ++r) buf[out[q] + r * SPAN * ES] = v[q][r];
+      }
+    __syncthreads();
+  }
+}
+
+template <typename T, int N, int THREADS, class Layout, int STAGE = 0>
+__device__ __forceinline__ void ct_fft(cx<T>* buf, int count, const cx<T>* table, int sign) {
+  if constexpr (STAGE < CtPlan<N>::stages) {
+    ct_stage<T, N, STAGE, THREADS, Layout>(buf, count, table, sign);
+    ct_fft<T, N, THREADS, Layout, STAGE + 1>(buf, count, table, sign);
+  }
+}
+
+}  // namespace spo_dev

@@ -31,8 +31,8 @@ struct Lines {
 };
 
 template <typename T>
-__global__ void lines_fft_kernel(cx<T>* data, Lines L, FftPlan plan, int sign_a, int mode, const cx<T>* factor,
-                                 int f_line, int f_elem) {
+__global__ __launch_bounds__(256) void lines_fft_kernel(cx<T>* data, Lines L, FftPlan plan, int sign_a, int mode,
+                                                        const cx<T>* factor, int f_line, int f_elem) {
   extern __shared__ unsigned char smem_raw[];
   cx<T>* tile = reinterpret_cast<cx<T>*>(smem_raw);
   const int l0 = blockIdx.x * L.lines_per_block;
@@ -84,7 +84,8 @@ __global__ void specular_potential_kernel(cx<T>* spec, Shape s, const cx<T>* gas
 // T_Zr: one (irot, ixy) plane per workgroup, the whole (Z, r) plane in LDS: 2D FFT, x exp(-i (T_Z + T_r) dt) with
 // the FFT scales folded in, inverse 2D FFT. One read and one write of psi.
 template <typename T>
-__global__ void zr_plane_kernel(cx<T>* psi, Shape s, FftPlan plan_z, FftPlan plan_r, const cx<T>* factor) {
+__global__ __launch_bounds__(256) void zr_plane_kernel(cx<T>* psi, Shape s, FftPlan plan_z, FftPlan plan_r,
+                                                       const cx<T>* factor) {
   extern __shared__ unsigned char smem_raw[];
   cx<T>* tile = reinterpret_cast<cx<T>*>(smem_raw);
   cx<T>* plane = psi + (long)blockIdx.x * s.n_zr;
@@ -107,7 +108,8 @@ __global__ void zr_plane_kernel(cx<T>* psi, Shape s, FftPlan plan_z, FftPlan pla
 // kinetic_only, optZ (optR rot(dt/2) [exp(-i vasy dt) rot(dt/2) beyond zasymp]) per (irot, izr)
 // (src/propagation/kinetic_operators.cpp:134-172, src/propagation/potential_operator.cpp:100-120).
 template <typename T>
-__global__ void xy_kernel(cx<T>* psi, Shape s, int C, FftPlan plan_x, FftPlan plan_y, const cx<T>* factor,
+__global__ __launch_bounds__(256) void xy_kernel(cx<T>* psi, Shape s, int C, FftPlan plan_x, FftPlan plan_y,
+                                                 const cx<T>* factor,
                           const cx<T>* shift_f, const cx<T>* shift_b, int needs_shift, int apply_potential,
                           const T* z_opt, const cx<T>* before, const cx<T>* gas_step) {
   extern __shared__ unsigned char smem_raw[];
@@ -166,9 +168,9 @@ struct AngularTables {
 };
 
 template <typename T>
-__global__ void interaction_kernel(cx<T>* psi, const cx<T>* spec, Shape s, int C, FftPlan plan_phi,
-                                   AngularTables at, const cx<T>* exp_table, const cx<T>* rot_half,
-                                   const cx<T>* s2sc) {
+__global__ __launch_bounds__(256) void interaction_kernel(cx<T>* psi, const cx<double>* spec, Shape s, int C,
+                                                          FftPlan plan_phi, AngularTables at, const cx<T>* exp_table,
+                                                          const cx<T>* rot_half, const cx<T>* s2sc) {
   extern __shared__ unsigned char smem_raw[];
   cx<T>* fbr = reinterpret_cast<cx<T>*>(smem_raw);
   cx<T>* dvr = fbr + C * s.n_rot;
@@ -186,7 +188,8 @@ __global__ void interaction_kernel(cx<T>* psi, const cx<T>* spec, Shape s, int C
       if (irot == s.ichn) {
         const int izr = izr0 + c;
         const int ir = izr % s.nr, iz = izr / s.nr;
-        v = v + rot_half[s.ichn + s.n_rot * ir] * (s2sc[ixy] * spec[ir + s.nr * iz]);
+        const cx<double> in = spec[ir + s.nr * iz];
+        v = v + rot_half[s.ichn + s.n_rot * ir] * (s2sc[ixy] * cx<T>{T(in.x), T(in.y)});
       }
     }
     fbr[e] = v;
@@ -247,23 +250,33 @@ __global__ void interaction_kernel(cx<T>* psi, const cx<T>* spec, Shape s, int C
 
 // Projection of the initial channel onto the specular grid and its removal from the scattering grid, one thread
 // per interaction-region (Z, r) point (src/propagation/potential_operator.cpp:133-140).
+// The projection is summed in FP64 and the specular wave kept in FP64 whatever the storage type of psi.
 template <typename T>
-__global__ void projection_kernel(cx<T>* psi, cx<T>* spec, Shape s, const cx<T>* sc2s, const cx<T>* s2sc) {
+__global__ void projection_kernel(cx<T>* psi, cx<double>* spec, Shape s, const cx<double>* sc2s,
+                                  const cx<double>* s2sc) {
   const int izr = blockIdx.x * blockDim.x + threadIdx.x;
   if (izr >= s.n_zr_pot) return;
   const long stride = (long)s.n_zr * s.n_rot;
   cx<T>* p = psi + izr + (long)s.n_zr * s.ichn;
-  cx<T> chi = {T(0), T(0)};
-  for (int ixy = 0; ixy < s.n_xy; ++ixy) chi = chi + sc2s[ixy] * p[stride * ixy];
+  cx<double> chi = {0.0, 0.0};
+  for (int ixy = 0; ixy < s.n_xy; ++ixy) {
+    const cx<T> v = p[stride * ixy];
+    chi = chi + sc2s[ixy] * cx<double>{double(v.x), double(v.y)};
+  }
   const int ir = izr % s.nr, iz = izr / s.nr;
   spec[ir + s.nr * iz] = chi;
-  for (int ixy = 0; ixy < s.n_xy; ++ixy) p[stride * ixy] = p[stride * ixy] - s2sc[ixy] * chi;
+  for (int ixy = 0; ixy < s.n_xy; ++ixy) {
+    const cx<T> v = p[stride * ixy];
+    const cx<double> r = cx<double>{double(v.x), double(v.y)} - s2sc[ixy] * chi;
+    p[stride * ixy] = cx<T>{T(r.x), T(r.y)};
+  }
 }
 
 // ---------------------------------------------------------------------------------------------------------------
 // Analysis. One workgroup per (v, irotana): overlap over r with phi_vj(r) dR and the momentum shift, (x, y) FFT
 // (sign -, scale 1), pick the analysed diffraction slots (src/analysis/balint_kurti.cpp:52-72).
-__global__ void bk_scattering_kernel(const cx<double>* psi, Shape s, int iz_ana, int n_rot_ana, int n_dif,
+template <typename T>
+__global__ void bk_scattering_kernel(const cx<T>* psi, Shape s, int iz_ana, int n_rot_ana, int n_dif,
                                      int n_states, const int* rot_index, const int* slots, const double* weights,
                                      const cx<double>* shift_f, double xy_scale, FftPlan plan_x, FftPlan plan_y,
                                      cx<double>* bk) {
@@ -274,9 +287,12 @@ __global__ void bk_scattering_kernel(const cx<double>* psi, Shape s, int iz_ana,
   const int irot = rot_index[irotana];
   const double* w = weights + (long)s.nr * task;
   for (int ixy = threadIdx.x; ixy < s.n_xy; ixy += blockDim.x) {
-    const cx<double>* p = psi + (long)s.n_zr * (irot + (long)s.n_rot * ixy) + s.nr * iz_ana;
+    const cx<T>* p = psi + (long)s.n_zr * (irot + (long)s.n_rot * ixy) + s.nr * iz_ana;
     cx<double> acc = {0.0, 0.0};
-    for (int ir = 0; ir < s.nr; ++ir) acc = acc + scale(shift_f[ixy] * p[ir], w[ir]);
+    for (int ir = 0; ir < s.nr; ++ir) {
+      const cx<double> x = {double(p[ir].x), double(p[ir].y)};
+      acc = acc + scale(shift_f[ixy] * x, w[ir]);
+    }
     ov[ixy] = acc;
   }
   __syncthreads();

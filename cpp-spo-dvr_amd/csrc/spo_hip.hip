@@ -1,5 +1,10 @@
 // The MI355X propagation backend behind the C ABI of include/spo_hip.h: device buffers, FFT plans and the launch
-// sequence of one analysis step. Kernels: kernels.hpp; LDS FFT engine: fft_lds.hpp.
+// sequence of one analysis step. Kernels: kernels_ct.hpp (compile-time shapes) and kernels.hpp (any shape); LDS
+// FFT engines: fft_ct.hpp, fft_lds.hpp.
+//
+// The scattering wave and the tables it is multiplied with are stored in T (double, or float in the mixed mode;
+// the factors are built in FP64 on the host and rounded once). The specular wave, the projection onto it, the
+// Balint-Kurti coefficients and the S-matrix sums are FP64 in both modes.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -11,7 +16,7 @@
 #include <vector>
 
 #include "../../include/spo_hip.h"
-#include "kernels.hpp"
+#include "kernels_ct.hpp"
 
 using spo_dev::cx;
 
@@ -52,7 +57,10 @@ class Buffer {
   }
   Buffer(const Buffer&) = delete;
   Buffer& operator=(const Buffer&) = delete;
-  Buffer(Buffer&& o) noexcept : ptr_(o.ptr_), bytes_(o.bytes_) { o.ptr_ = nullptr; o.bytes_ = 0; }
+  Buffer(Buffer&& o) noexcept : ptr_(o.ptr_), bytes_(o.bytes_) {
+    o.ptr_ = nullptr;
+    o.bytes_ = 0;
+  }
   Buffer& operator=(Buffer&& o) noexcept {
     std::swap(ptr_, o.ptr_);
     std::swap(bytes_, o.bytes_);
@@ -63,7 +71,6 @@ class Buffer {
   }
   template <typename T>
   T* as() const { return static_cast<T*>(ptr_); }
-  std::size_t bytes() const { return bytes_; }
 
  private:
   void* ptr_ = nullptr;
@@ -76,7 +83,20 @@ Buffer upload(const void* host, std::size_t bytes) {
   return b;
 }
 
-// Radices of a length: 8 while it divides, then 4, 2, 3, 5, 7, 11.
+// Complex array (interleaved doubles) -> device array of cx<T>, optionally times a real factor (in FP64 first).
+template <typename T>
+Buffer upload_complex(const double* p, std::size_t n, double factor = 1.0) {
+  std::vector<cx<T>> v(n);
+  for (std::size_t i = 0; i < n; ++i) v[i] = {T(p[2 * i] * factor), T(p[2 * i + 1] * factor)};
+  return upload(v.data(), n * sizeof(cx<T>));
+}
+template <typename T>
+Buffer upload_real(const double* p, std::size_t n) {
+  std::vector<T> v(p, p + n);
+  return upload(v.data(), n * sizeof(T));
+}
+
+// Radices of the run-time engine: 8 while it divides, then 4, 2, 3, 5, 7, 11.
 std::vector<int> radices_of(int n) {
   std::vector<int> r;
   int m = n;
@@ -90,102 +110,276 @@ std::vector<int> radices_of(int n) {
   return r;
 }
 
-// An FFT plan with its twiddle table on the device.
+// An FFT length: twiddle table W[m] = exp(-2 pi i m / n) on the device (in T) and the run-time plan.
+template <typename T>
 struct DevicePlan {
   Buffer table;
   spo_dev::FftPlan plan{};
   explicit DevicePlan(int n) {
     const std::vector<int> r = radices_of(n);
-    std::vector<cx<double>> w(n);
+    std::vector<cx<T>> w(n);
     for (int m = 0; m < n; ++m) {
       const long double a = -2.0L * 3.14159265358979323846264338327950288L * m / n;
-      w[m] = {static_cast<double>(std::cos(a)), static_cast<double>(std::sin(a))};
+      w[m] = {static_cast<T>(std::cos(a)), static_cast<T>(std::sin(a))};
     }
-    table = upload(w.data(), w.size() * sizeof(cx<double>));
+    table = upload(w.data(), w.size() * sizeof(cx<T>));
     plan.n = n;
     plan.n_stages = static_cast<int>(r.size());
     for (std::size_t i = 0; i < r.size(); ++i) plan.radix[i] = r[i];
     plan.table = table.as<void>();
   }
+  const cx<T>* tw() const { return table.as<cx<T>>(); }
 };
-
-std::vector<cx<double>> complex_from(const double* p, std::size_t n) {
-  std::vector<cx<double>> v(n);
-  std::memcpy(v.data(), p, n * sizeof(cx<double>));
-  return v;
-}
 
 unsigned blocks_for(long n, int threads) { return static_cast<unsigned>((n + threads - 1) / threads); }
 
 constexpr int kThreads = 256;
 
+void set_lds(const void* kernel, std::size_t bytes) {
+  HIP_CHECK(hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(bytes)));
+}
+
+// ---- instantiations of the compile-time-shaped kernels ---------------------------------------------------------
+// T_Zr planes (NZ, NR, threads) and T_xy tiles (NX, NY, C); phi lengths of the interaction kernel.
+#define SPO_ZR_SHAPES(X) X(96, 48, 384) X(128, 32, 256) X(176, 56, 256) X(64, 16, 256) X(80, 16, 256)
+#define SPO_XY_SHAPES(X) X(16, 16, 16) X(8, 8, 64) X(4, 4, 64)
+#define SPO_PHI_LENGTHS(X) X(48) X(22) X(10) X(12) X(8) X(56)
+
+template <typename T>
+const void* zr_ct_symbol(int nz, int nr, int* threads) {
+#define X(A, B, TH)                                                            \
+  if (nz == A && nr == B) {                                                    \
+    *threads = TH;                                                             \
+    return reinterpret_cast<const void*>(spo_dev::zr_ct_kernel<T, A, B, TH>); \
+  }
+  SPO_ZR_SHAPES(X)
+#undef X
+  return nullptr;
+}
+
+template <typename T>
+const void* xy_ct_symbol(int nx, int ny, int* c) {
+#define X(A, B, CC)                                                                     \
+  if (nx == A && ny == B) {                                                             \
+    *c = CC;                                                                            \
+    return reinterpret_cast<const void*>(spo_dev::xy_ct_kernel<T, A, B, CC, kThreads>); \
+  }
+  SPO_XY_SHAPES(X)
+#undef X
+  return nullptr;
+}
+
+template <typename T>
+const void* interaction_ct_symbol(int nphi) {
+#define X(A) \
+  if (nphi == A) return reinterpret_cast<const void*>(spo_dev::interaction_ct_kernel<T, A, kThreads>);
+  SPO_PHI_LENGTHS(X)
+#undef X
+  return nullptr;
+}
+
+// Launch through the symbol with a packed argument list (hipLaunchKernel), so one call covers every instantiation.
+void launch(const void* fn, unsigned grid, unsigned threads, std::size_t lds, hipStream_t stream, void** args) {
+  HIP_CHECK(hipLaunchKernel(fn, dim3(grid), dim3(threads), args, lds, stream));
+}
+
 }  // namespace
 
+// ---------------------------------------------------------------------------------------------------------------
+// Backend state, common part.
 struct spo_hip_backend {
   spo_hip_shape shape{};
   spo_dev::Shape s{};
   hipStream_t stream = nullptr;
   int max_lds = 65536;
-
-  Buffer psi, spec, densities, plane, staging, bk, sums;
-  Buffer zr_full, zr_half, xy_full, xy_half, spec_full, spec_half;
-  Buffer rot_half, before, gas_step, spec_gas, z_opt, r_opt, z_spec_opt;
-  Buffer shift_f, shift_b, s2sc, sc2s;
-  Buffer legendre, block_first, block_size, block_offset, rot_block, rep_of, op_of, phi_index, exp_table;
-  Buffer rot_index, dif_slot, radial_weight, initial_radial, energies;
-  std::unique_ptr<DevicePlan> plan_z, plan_r, plan_x, plan_y, plan_phi, plan_zsp;
   spo_hip_analysis analysis{};
-  int xy_c = 1, v_c = 1;
-  bool zr_fused = false;
+
+  // FP64 in both modes: specular wave and its factors, the analysis.
+  Buffer spec, spec_full, spec_half, spec_gas, z_spec_opt, r_opt_d, s2sc_d, sc2s_d;
+  Buffer densities, partial, plane, staging, bk, sums;
+  Buffer rot_index, dif_slot, radial_weight, initial_radial, energies, shift_f_d;
+  std::unique_ptr<DevicePlan<double>> plan_r_d, plan_zsp, plan_x_d, plan_y_d;
   std::size_t staging_values = 0;
 
-  void kinetic_and_potential(bool half, bool kinetic_only);
-  void lines(cx<double>* data, spo_dev::Lines L, const DevicePlan& p, int sign, int mode, const cx<double>* factor,
-             int f_line, int f_elem);
+  // Kernel timing (spo_hip_profile_enable): one event pair per launch on `stream`.
+  struct Timed {
+    int kernel;
+    hipEvent_t start, stop;
+  };
+  bool profiling = false;
+  std::vector<std::string> kernel_names;
+  std::vector<Timed> timed_launches;
+  std::vector<hipEvent_t> event_pool;
+
+  hipEvent_t take_event() {
+    if (!event_pool.empty()) {
+      hipEvent_t e = event_pool.back();
+      event_pool.pop_back();
+      return e;
+    }
+    hipEvent_t e;
+    HIP_CHECK(hipEventCreate(&e));
+    return e;
+  }
+  // Runs `launch_fn` (one kernel launch on `stream`), bracketed by events when profiling.
+  template <typename F>
+  void timed(const char* name, F&& launch_fn) {
+    if (!profiling) {
+      launch_fn();
+      HIP_CHECK(hipGetLastError());
+      return;
+    }
+    int id = 0;
+    while (id < static_cast<int>(kernel_names.size()) && kernel_names[id] != name) ++id;
+    if (id == static_cast<int>(kernel_names.size())) kernel_names.emplace_back(name);
+    Timed t{id, take_event(), take_event()};
+    HIP_CHECK(hipEventRecord(t.start, stream));
+    launch_fn();
+    HIP_CHECK(hipGetLastError());
+    HIP_CHECK(hipEventRecord(t.stop, stream));
+    timed_launches.push_back(t);
+  }
+
+  virtual ~spo_hip_backend() {
+    for (const Timed& t : timed_launches) {
+      (void)hipEventDestroy(t.start);
+      (void)hipEventDestroy(t.stop);
+    }
+    for (hipEvent_t e : event_pool) (void)hipEventDestroy(e);
+    if (stream) {
+      (void)hipStreamSynchronize(stream);
+      (void)hipStreamDestroy(stream);
+    }
+  }
+
+  virtual void kinetic_and_potential(bool half, bool kinetic_only) = 0;
+  virtual void upload_wave(const double* scattering) = 0;
+  virtual void download_wave(double* scattering) = 0;
+  virtual void z_plane(int iz) = 0;
+  virtual void block_densities() = 0;
+  virtual void site_column(int ixy, double* out) = 0;
+  virtual void bk_scattering() = 0;
+
+  void specular_step(bool half, bool kinetic_only);
+  void lines_d(cx<double>* data, spo_dev::Lines L, const DevicePlan<double>& p, int sign, int mode,
+               const cx<double>* factor, int f_line, int f_elem);
+  void analyse(int surface_fixed, int iz_specular, double t, double* coefficients);
 };
 
-namespace {
-
-void set_lds(const void* kernel, std::size_t bytes) {
-  HIP_CHECK(hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(bytes)));
-}
-
-}  // namespace
-
-void spo_hip_backend::lines(cx<double>* data, spo_dev::Lines L, const DevicePlan& p, int sign, int mode,
-                            const cx<double>* factor, int f_line, int f_elem) {
+void spo_hip_backend::lines_d(cx<double>* data, spo_dev::Lines L, const DevicePlan<double>& p, int sign, int mode,
+                              const cx<double>* factor, int f_line, int f_elem) {
   L.lines_per_block = std::max(1, std::min(64, 65536 / (L.n * static_cast<int>(sizeof(cx<double>)))));
   const std::size_t lds = std::size_t(L.lines_per_block) * L.n * sizeof(cx<double>);
   const unsigned grid = blocks_for(L.count, L.lines_per_block);
-  hipLaunchKernelGGL(spo_dev::lines_fft_kernel<double>, dim3(grid), dim3(kThreads), lds, stream, data, L, p.plan,
-                     sign, mode, factor, f_line, f_elem);
-  HIP_CHECK(hipGetLastError());
+  timed("specular_lines_fft", [&] {
+    hipLaunchKernelGGL(spo_dev::lines_fft_kernel<double>, dim3(grid), dim3(kThreads), lds, stream, data, L, p.plan,
+                       sign, mode, factor, f_line, f_elem);
+  });
 }
 
-void spo_hip_backend::kinetic_and_potential(bool half, bool kinetic_only) {
-  cx<double>* ps = psi.as<cx<double>>();
+// Specular grid: r lines, Z lines with the kinetic factor (FFT scales folded in), r lines back, then the gas phase
+// beyond zasymp and the absorbers (src/propagation/specular_operator.cpp:40-71). FP64.
+void spo_hip_backend::specular_step(bool half, bool kinetic_only) {
   cx<double>* sp = spec.as<cx<double>>();
-  // Specular grid: r lines, Z lines with the kinetic factor, r lines back, then gas phase and absorbers.
-  {
-    spo_dev::Lines r_lines{s.nr, s.nzsp, s.nzsp, s.nr, 0, 1, 1};
-    spo_dev::Lines z_lines{s.nzsp, s.nr, s.nr, 1, 0, s.nr, 1};
-    lines(sp, r_lines, *plan_r, -1, 0, nullptr, 0, 0);
-    lines(sp, z_lines, *plan_zsp, -1, 1, (half ? spec_half : spec_full).as<cx<double>>(), 1, s.nr);
-    lines(sp, r_lines, *plan_r, +1, 0, nullptr, 0, 0);
-    if (!kinetic_only) {
-      hipLaunchKernelGGL(spo_dev::specular_potential_kernel<double>, dim3(blocks_for(long(s.nr) * s.nzsp, kThreads)),
-                         dim3(kThreads), 0, stream, sp, s, spec_gas.as<cx<double>>(), z_spec_opt.as<double>(),
-                         r_opt.as<double>());
-      HIP_CHECK(hipGetLastError());
-    }
+  spo_dev::Lines r_lines{s.nr, s.nzsp, s.nzsp, s.nr, 0, 1, 1};
+  spo_dev::Lines z_lines{s.nzsp, s.nr, s.nr, 1, 0, s.nr, 1};
+  lines_d(sp, r_lines, *plan_r_d, -1, 0, nullptr, 0, 0);
+  lines_d(sp, z_lines, *plan_zsp, -1, 1, (half ? spec_half : spec_full).as<cx<double>>(), 1, s.nr);
+  lines_d(sp, r_lines, *plan_r_d, +1, 0, nullptr, 0, 0);
+  if (kinetic_only) return;
+  timed("specular_potential", [&] {
+    hipLaunchKernelGGL(spo_dev::specular_potential_kernel<double>, dim3(blocks_for(long(s.nr) * s.nzsp, kThreads)),
+                       dim3(kThreads), 0, stream, sp, s, spec_gas.as<cx<double>>(), z_spec_opt.as<double>(),
+                       r_opt_d.as<double>());
+  });
+}
+
+void spo_hip_backend::analyse(int surface_fixed, int iz_specular, double t, double* coefficients) {
+  const spo_hip_analysis& a = analysis;
+  const std::size_t n = std::size_t(a.n_states) * a.n_v;
+  timed("bk_zero", [&] {
+    hipLaunchKernelGGL(spo_dev::zero_kernel, dim3(blocks_for(long(n), kThreads)), dim3(kThreads), 0, stream,
+                       bk.as<cx<double>>(), long(n));
+  });
+  bk_scattering();
+  if (surface_fixed) {
+    if (iz_specular < 0 || iz_specular >= s.nzsp) throw Status(SPO_HIP_E_ARGUMENT, "specular line out of range");
+    timed("bk_specular", [&] {
+      hipLaunchKernelGGL(spo_dev::bk_specular_kernel, dim3(1), dim3(64), 0, stream, spec.as<cx<double>>(), s.nr,
+                         iz_specular, a.n_v, a.n_states, a.initial_state, initial_radial.as<double>(), a.dr,
+                         bk.as<cx<double>>());
+    });
   }
+  const long total = long(n) * a.n_energies;
+  timed("s_accumulate", [&] {
+    hipLaunchKernelGGL(spo_dev::s_accumulate_kernel, dim3(blocks_for(total, kThreads)), dim3(kThreads), 0, stream,
+                       sums.as<cx<double>>(), bk.as<cx<double>>(), static_cast<int>(n), a.n_energies,
+                       energies.as<double>(), t, a.analysis_time_step);
+  });
+  if (coefficients)
+    HIP_CHECK(hipMemcpyAsync(coefficients, bk.as<void>(), n * sizeof(cx<double>), hipMemcpyDeviceToHost, stream));
+  HIP_CHECK(hipStreamSynchronize(stream));
+}
+
+// ---------------------------------------------------------------------------------------------------------------
+// The part that depends on the storage type of the scattering wave.
+namespace {
+
+template <typename T>
+struct State final : spo_hip_backend {
+  Buffer psi, zr_full, zr_half, xy_full, xy_half, rot_half, before, gas_step, z_opt;
+  Buffer shift_f, shift_b, s2sc, legendre, block_first, block_size, block_offset, rot_block, rep_of, op_of, phi_index;
+  Buffer exp_table, legendre_t;
+  int legendre_size = 0;
+  std::size_t zr_lds = 0, xy_lds = 0, v_lds = 0;
+  std::unique_ptr<DevicePlan<T>> plan_z, plan_r, plan_x, plan_y, plan_phi;
+  int xy_c = 1, v_c = 1;
+  const void* zr_fn = nullptr;
+  int zr_threads = kThreads;
+  const void* xy_fn = nullptr;
+  const void* v_fn = nullptr;
+  bool zr_fused = false;
+
+  void kinetic_and_potential(bool half, bool kinetic_only) override;
+  void upload_wave(const double* scattering) override;
+  void download_wave(double* scattering) override;
+  void z_plane(int iz) override;
+  void block_densities() override;
+  void site_column(int ixy, double* out) override;
+  void bk_scattering() override;
+  void lines(cx<T>* data, spo_dev::Lines L, const DevicePlan<T>& p, int sign, int mode, const cx<T>* factor,
+             int f_line, int f_elem);
+};
+
+template <typename T>
+void State<T>::lines(cx<T>* data, spo_dev::Lines L, const DevicePlan<T>& p, int sign, int mode, const cx<T>* factor,
+                     int f_line, int f_elem) {
+  L.lines_per_block = std::max(1, std::min(64, 65536 / (L.n * static_cast<int>(sizeof(cx<T>)))));
+  const std::size_t lds = std::size_t(L.lines_per_block) * L.n * sizeof(cx<T>);
+  const unsigned grid = blocks_for(L.count, L.lines_per_block);
+  timed("zr_lines_fft", [&] {
+    hipLaunchKernelGGL(spo_dev::lines_fft_kernel<T>, dim3(grid), dim3(kThreads), lds, stream, data, L, p.plan, sign,
+                       mode, factor, f_line, f_elem);
+  });
+}
+
+template <typename T>
+void State<T>::kinetic_and_potential(bool half, bool kinetic_only) {
+  specular_step(half, kinetic_only);
+  cx<T>* ps = psi.as<cx<T>>();
   // T_Zr.
-  const cx<double>* zr = (half ? zr_half : zr_full).as<cx<double>>();
+  const cx<T>* zr = (half ? zr_half : zr_full).as<cx<T>>();
   const int n_planes = s.n_rot * s.n_xy;
-  if (zr_fused) {
-    hipLaunchKernelGGL(spo_dev::zr_plane_kernel<double>, dim3(n_planes), dim3(kThreads),
-                       std::size_t(s.n_zr) * sizeof(cx<double>), stream, ps, s, plan_z->plan, plan_r->plan, zr);
-    HIP_CHECK(hipGetLastError());
+  if (zr_fn) {
+    const cx<T>* tz = plan_z->tw();
+    const cx<T>* tr = plan_r->tw();
+    void* args[] = {&ps, const_cast<cx<T>**>(&tz), const_cast<cx<T>**>(&tr), const_cast<cx<T>**>(&zr)};
+    timed("zr_plane", [&] { launch(zr_fn, n_planes, zr_threads, zr_lds, stream, args); });
+  } else if (zr_fused) {
+    timed("zr_plane_generic", [&] {
+      hipLaunchKernelGGL(spo_dev::zr_plane_kernel<T>, dim3(n_planes), dim3(kThreads), std::size_t(s.n_zr) * sizeof(cx<T>),
+                         stream, ps, s, plan_z->plan, plan_r->plan, zr);
+    });
   } else {
     spo_dev::Lines r_lines{s.nr, s.nz * n_planes, s.nz, s.nr, s.n_zr, 1, 1};
     spo_dev::Lines z_lines{s.nz, s.nr * n_planes, s.nr, 1, s.n_zr, s.nr, 1};
@@ -196,30 +390,347 @@ void spo_hip_backend::kinetic_and_potential(bool half, bool kinetic_only) {
   // T_xy and the diagonal part of V.
   {
     const int tiles = (s.n_zr + xy_c - 1) / xy_c;
-    const std::size_t lds = std::size_t(xy_c) * s.n_xy * sizeof(cx<double>);
-    hipLaunchKernelGGL(spo_dev::xy_kernel<double>, dim3(tiles * s.n_rot), dim3(kThreads), lds, stream, ps, s, xy_c,
-                       plan_x->plan, plan_y->plan, (half ? xy_half : xy_full).as<cx<double>>(),
-                       shift_f.as<cx<double>>(), shift_b.as<cx<double>>(), shape.needs_shift, kinetic_only ? 0 : 1,
-                       z_opt.as<double>(), before.as<cx<double>>(), gas_step.as<cx<double>>());
-    HIP_CHECK(hipGetLastError());
+    const std::size_t lds = xy_lds;
+    const cx<T>* factor = (half ? xy_half : xy_full).as<cx<T>>();
+    const cx<T>* sf = shift_f.as<cx<T>>();
+    const cx<T>* sb = shift_b.as<cx<T>>();
+    int needs_shift = shape.needs_shift;
+    int apply_potential = kinetic_only ? 0 : 1;
+    const T* zo = z_opt.as<T>();
+    const cx<T>* bf = before.as<cx<T>>();
+    const cx<T>* gs = gas_step.as<cx<T>>();
+    if (xy_fn) {
+      const cx<T>* tx = plan_x->tw();
+      const cx<T>* ty = plan_y->tw();
+      void* args[] = {&ps, &s, const_cast<cx<T>**>(&tx), const_cast<cx<T>**>(&ty), const_cast<cx<T>**>(&factor),
+                      const_cast<cx<T>**>(&sf), const_cast<cx<T>**>(&sb), &needs_shift, &apply_potential,
+                      const_cast<T**>(&zo), const_cast<cx<T>**>(&bf), const_cast<cx<T>**>(&gs)};
+      timed("xy_tile", [&] { launch(xy_fn, tiles * s.n_rot, kThreads, lds, stream, args); });
+    } else {
+      timed("xy_tile_generic", [&] {
+        hipLaunchKernelGGL(spo_dev::xy_kernel<T>, dim3(tiles * s.n_rot), dim3(kThreads), lds, stream, ps, s, xy_c,
+                           plan_x->plan, plan_y->plan, factor, sf, sb, needs_shift, apply_potential, zo, bf, gs);
+      });
+    }
   }
   if (kinetic_only || s.n_zr_pot == 0) return;
   // V in the interaction region, then the projection onto the specular grid.
-  {
-    spo_dev::AngularTables at{legendre.as<double>(),  block_first.as<int>(), block_size.as<int>(),
-                              block_offset.as<int>(), rot_block.as<int>(),   rep_of.as<int>(),
-                              op_of.as<int>(),        phi_index.as<int>()};
-    const int tiles = (s.n_zr_pot + v_c - 1) / v_c;
-    const std::size_t lds = std::size_t(v_c) * (s.n_rot + s.n_tp) * sizeof(cx<double>);
-    hipLaunchKernelGGL(spo_dev::interaction_kernel<double>, dim3(tiles * s.n_xy), dim3(kThreads), lds, stream, ps, sp,
-                       s, v_c, plan_phi->plan, at, exp_table.as<cx<double>>(), rot_half.as<cx<double>>(),
-                       s2sc.as<cx<double>>());
+  spo_dev::AngularTables at{legendre.as<double>(),  block_first.as<int>(), block_size.as<int>(),
+                            block_offset.as<int>(), rot_block.as<int>(),   rep_of.as<int>(),
+                            op_of.as<int>(),        phi_index.as<int>()};
+  const int tiles = (s.n_zr_pot + v_c - 1) / v_c;
+  const std::size_t lds = v_lds;
+  const cx<double>* sp = spec.as<cx<double>>();
+  const cx<T>* et = exp_table.as<cx<T>>();
+  const cx<T>* rh = rot_half.as<cx<T>>();
+  const cx<T>* ssc = s2sc.as<cx<T>>();
+  if (v_fn) {
+    const cx<T>* tp = plan_phi->tw();
+    const T* lt = legendre_t.as<T>();
+    int c = v_c;
+    int ls = legendre_size;
+    void* args[] = {&ps, const_cast<cx<double>**>(&sp), &s, &c, const_cast<cx<T>**>(&tp), &at,
+                    const_cast<cx<T>**>(&et), const_cast<cx<T>**>(&rh), const_cast<cx<T>**>(&ssc),
+                    const_cast<T**>(&lt), &ls};
+    timed("interaction", [&] { launch(v_fn, tiles * s.n_xy, kThreads, lds, stream, args); });
+  } else {
+    timed("interaction_generic", [&] {
+      hipLaunchKernelGGL(spo_dev::interaction_kernel<T>, dim3(tiles * s.n_xy), dim3(kThreads), lds, stream, ps, sp, s,
+                         v_c, plan_phi->plan, at, et, rh, ssc);
+    });
+  }
+  timed("projection", [&] {
+    hipLaunchKernelGGL(spo_dev::projection_kernel<T>, dim3(blocks_for(s.n_zr_pot, kThreads)), dim3(kThreads), 0,
+                       stream, ps, spec.as<cx<double>>(), s, sc2s_d.as<cx<double>>(), s2sc_d.as<cx<double>>());
+  });
+}
+
+template <typename T>
+void State<T>::upload_wave(const double* scattering) {
+  const std::size_t per_izr = std::size_t(s.n_rot) * s.n_xy;
+  const int chunk = static_cast<int>(std::max<std::size_t>(1, staging_values / per_izr));
+  for (int izr0 = 0; izr0 < s.n_zr; izr0 += chunk) {
+    const int count = std::min(chunk, s.n_zr - izr0);
+    HIP_CHECK(hipMemcpyAsync(staging.as<void>(), scattering + 2 * per_izr * izr0, per_izr * count * sizeof(cx<double>),
+                             hipMemcpyHostToDevice, stream));
+    hipLaunchKernelGGL(spo_dev::host_to_device_kernel<T>, dim3(blocks_for(long(per_izr) * count, kThreads)),
+                       dim3(kThreads), 0, stream, psi.as<cx<T>>(), s, staging.as<cx<double>>(), izr0, count);
     HIP_CHECK(hipGetLastError());
-    hipLaunchKernelGGL(spo_dev::projection_kernel<double>, dim3(blocks_for(s.n_zr_pot, kThreads)), dim3(kThreads), 0,
-                       stream, ps, sp, s, sc2s.as<cx<double>>(), s2sc.as<cx<double>>());
-    HIP_CHECK(hipGetLastError());
+    HIP_CHECK(hipStreamSynchronize(stream));
   }
 }
+
+template <typename T>
+void State<T>::download_wave(double* scattering) {
+  const std::size_t per_izr = std::size_t(s.n_rot) * s.n_xy;
+  const int chunk = static_cast<int>(std::max<std::size_t>(1, staging_values / per_izr));
+  for (int izr0 = 0; izr0 < s.n_zr; izr0 += chunk) {
+    const int count = std::min(chunk, s.n_zr - izr0);
+    hipLaunchKernelGGL(spo_dev::device_to_host_kernel<T>, dim3(blocks_for(long(per_izr) * count, kThreads)),
+                       dim3(kThreads), 0, stream, psi.as<cx<T>>(), s, staging.as<cx<double>>(), izr0, count);
+    HIP_CHECK(hipGetLastError());
+    HIP_CHECK(hipMemcpyAsync(scattering + 2 * per_izr * izr0, staging.as<void>(), per_izr * count * sizeof(cx<double>),
+                             hipMemcpyDeviceToHost, stream));
+    HIP_CHECK(hipStreamSynchronize(stream));
+  }
+}
+
+template <typename T>
+void State<T>::z_plane(int iz) {
+  const long n = long(s.n_rot) * s.n_xy * s.nr;
+  hipLaunchKernelGGL(spo_dev::z_plane_kernel<T>, dim3(blocks_for(n, kThreads)), dim3(kThreads), 0, stream,
+                     psi.as<cx<T>>(), s, iz, plane.as<cx<double>>());
+  HIP_CHECK(hipGetLastError());
+}
+
+template <typename T>
+void State<T>::block_densities() {
+  const int chunk = std::max(1, (s.n_xy + 15) / 16);
+  const int parts = (s.n_xy + chunk - 1) / chunk;
+  timed("densities", [&] {
+    hipLaunchKernelGGL(spo_dev::densities_partial_kernel<T>, dim3(blocks_for(s.n_zr, kThreads), parts), dim3(kThreads),
+                       0, stream, psi.as<cx<T>>(), s, chunk, partial.as<double>());
+  });
+  timed("densities_sum", [&] {
+    hipLaunchKernelGGL(spo_dev::densities_sum_kernel, dim3(blocks_for(s.n_zr, kThreads)), dim3(kThreads), 0, stream,
+                       partial.as<double>(), s.n_zr, parts, densities.as<double>());
+  });
+}
+
+template <typename T>
+void State<T>::site_column(int ixy, double* out) {
+  const std::size_t n = std::size_t(s.n_zr) * s.n_rot;
+  if constexpr (sizeof(T) == sizeof(double)) {
+    HIP_CHECK(hipMemcpyAsync(out, psi.as<cx<T>>() + n * ixy, n * sizeof(cx<T>), hipMemcpyDeviceToHost, stream));
+    HIP_CHECK(hipStreamSynchronize(stream));
+  } else {
+    std::vector<cx<T>> tmp(n);
+    HIP_CHECK(hipMemcpyAsync(tmp.data(), psi.as<cx<T>>() + n * ixy, n * sizeof(cx<T>), hipMemcpyDeviceToHost, stream));
+    HIP_CHECK(hipStreamSynchronize(stream));
+    for (std::size_t i = 0; i < n; ++i) {
+      out[2 * i] = tmp[i].x;
+      out[2 * i + 1] = tmp[i].y;
+    }
+  }
+}
+
+template <typename T>
+void State<T>::bk_scattering() {
+  const spo_hip_analysis& a = analysis;
+  timed("bk_scattering", [&] {
+    hipLaunchKernelGGL(spo_dev::bk_scattering_kernel<T>, dim3(a.n_v * a.n_rotation), dim3(kThreads),
+                       std::size_t(s.n_xy) * sizeof(cx<double>), stream, psi.as<cx<T>>(), s, a.iz_analysis,
+                       a.n_rotation, a.n_diffraction, a.n_states, rot_index.as<int>(), dif_slot.as<int>(),
+                       radial_weight.as<double>(), shift_f_d.as<cx<double>>(), a.xy_scale, plan_x_d->plan,
+                       plan_y_d->plan, bk.as<cx<double>>());
+  });
+}
+
+spo_dev::Shape make_shape(const spo_hip_shape* shape) {
+  spo_dev::Shape s{};
+  s.n_rot = shape->n_rot;
+  s.nx = shape->nx;
+  s.ny = shape->ny;
+  s.n_xy = shape->nx * shape->ny;
+  s.nr = shape->nr;
+  s.nz = shape->nz;
+  s.n_zr = shape->nr * shape->nz;
+  s.nzsp = shape->nz_specular;
+  s.nzpot = shape->nz_potential;
+  s.n_zr_pot = shape->nz_potential * shape->nr;
+  s.n_theta = shape->n_theta;
+  s.n_phi = shape->n_phi;
+  s.n_tp = shape->n_theta * shape->n_phi;
+  s.n_rep = shape->n_representatives;
+  s.ichn = shape->initial_channel;
+  s.mj_max = shape->mj_max;
+  s.n_blocks = 2 * shape->mj_max + 1;
+  s.r_abs_first = shape->r_absorber_first;
+  s.r_abs_last = shape->r_absorber_last;
+  return s;
+}
+
+template <typename T>
+std::unique_ptr<spo_hip_backend> create_state(const spo_hip_shape* shape, const spo_hip_tables* t,
+                                              const spo_hip_analysis* a, const double* initial_specular) {
+  auto b = std::make_unique<State<T>>();
+  b->shape = *shape;
+  b->s = make_shape(shape);
+  const spo_dev::Shape& s = b->s;
+  if (s.n_rot < 1 || s.nx < 2 || s.ny < 2 || s.nr < 2 || s.nz < 2 || s.nzsp < s.nz || s.nzpot < 0 || s.nzpot > s.nz ||
+      s.n_theta < 1 || s.n_phi < 1 || s.n_rep < 1 || s.ichn < 0 || s.ichn >= s.n_rot || shape->n_operations < 1 ||
+      shape->steps_per_analysis < 1 || s.r_abs_first < 0 || s.r_abs_last >= s.nr)
+    throw Status(SPO_HIP_E_ARGUMENT, "spo_hip_create: inconsistent shape");
+
+  int device = 0;
+  HIP_CHECK(hipGetDevice(&device));
+  HIP_CHECK(hipDeviceGetAttribute(&b->max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device));
+  const std::size_t psi_values = std::size_t(s.n_zr) * s.n_rot * s.n_xy;
+  const std::size_t exp_values = std::size_t(s.n_tp) * s.n_rep * s.n_zr_pot;
+  std::size_t free_bytes = 0, total_bytes = 0;
+  HIP_CHECK(hipMemGetInfo(&free_bytes, &total_bytes));
+  const std::size_t need = (psi_values + exp_values) * sizeof(cx<T>) + (std::size_t(1) << 28);
+  if (need > free_bytes)
+    throw Status(SPO_HIP_E_NOT_SUPPORTED, "not enough device memory: need " + std::to_string(need >> 20) +
+                                              " MiB, free " + std::to_string(free_bytes >> 20) + " MiB");
+  HIP_CHECK(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+
+  const std::size_t cT = sizeof(cx<T>), c16 = sizeof(cx<double>);
+  b->psi = Buffer(psi_values * cT);
+  HIP_CHECK(hipMemsetAsync(b->psi.template as<void>(), 0, psi_values * cT, b->stream));
+  b->spec = upload(initial_specular, std::size_t(s.nr) * s.nzsp * c16);
+  b->densities = Buffer(std::size_t(s.n_zr) * sizeof(double));
+  b->partial = Buffer(std::size_t(s.n_zr) * 16 * sizeof(double));
+  b->plane = Buffer(std::size_t(s.n_rot) * s.n_xy * s.nr * c16);
+  b->staging_values = std::max<std::size_t>(std::size_t(s.n_rot) * s.n_xy, std::size_t(4) << 20);
+  b->staging = Buffer(b->staging_values * c16);
+
+  // Kinetic factors with the forward and backward FFT scales folded in (multiplied in FP64, then stored).
+  const double zr_f = t->fft_scale_z * t->fft_scale_r;
+  const double zr_b = (1.0 / (s.nz * t->fft_scale_z)) * (1.0 / (s.nr * t->fft_scale_r));
+  const double sp_b = (1.0 / (s.nzsp * t->fft_scale_z)) * (1.0 / (s.nr * t->fft_scale_r));
+  const double xy_f = t->fft_scale_x * t->fft_scale_y;
+  const double xy_b = (1.0 / (s.nx * t->fft_scale_x)) * (1.0 / (s.ny * t->fft_scale_y));
+  auto fold = [](const double* p, std::size_t n, double f1, double f2) {
+    std::vector<double> v(p, p + 2 * n);
+    for (double& x : v) x = x * f1 * f2;
+    return v;
+  };
+  {
+    const auto zf = fold(t->zr_full, s.n_zr, zr_f, zr_b), zh = fold(t->zr_half, s.n_zr, zr_f, zr_b);
+    b->zr_full = upload_complex<T>(zf.data(), s.n_zr);
+    b->zr_half = upload_complex<T>(zh.data(), s.n_zr);
+    const auto sf = fold(t->specular_full, std::size_t(s.nr) * s.nzsp, zr_f, sp_b);
+    const auto sh = fold(t->specular_half, std::size_t(s.nr) * s.nzsp, zr_f, sp_b);
+    b->spec_full = upload(sf.data(), sf.size() * sizeof(double));
+    b->spec_half = upload(sh.data(), sh.size() * sizeof(double));
+    const auto xf = fold(t->xy_full, s.n_xy, xy_f, xy_b), xh = fold(t->xy_half, s.n_xy, xy_f, xy_b);
+    b->xy_full = upload_complex<T>(xf.data(), s.n_xy);
+    b->xy_half = upload_complex<T>(xh.data(), s.n_xy);
+  }
+  b->rot_half = upload_complex<T>(t->rotation_half, std::size_t(s.n_rot) * s.nr);
+  b->before = upload_complex<T>(t->before_interaction, std::size_t(s.n_rot) * s.nr);
+  b->gas_step = upload_complex<T>(t->gas_phase_step, std::size_t(s.n_rot) * s.nr);
+  b->spec_gas = upload(t->specular_gas_phase, std::size_t(s.nr) * c16);
+  b->z_opt = upload_real<T>(t->z_optical, s.nz);
+  b->r_opt_d = upload(t->r_optical, std::size_t(s.nr) * sizeof(double));
+  b->z_spec_opt = upload(t->z_specular_optical, std::size_t(s.nzsp) * sizeof(double));
+  b->shift_f = upload_complex<T>(t->shift_forward, s.n_xy);
+  b->shift_b = upload_complex<T>(t->shift_backward, s.n_xy);
+  b->shift_f_d = upload(t->shift_forward, std::size_t(s.n_xy) * c16);
+  b->s2sc = upload_complex<T>(t->specular_to_scattering, s.n_xy);
+  b->s2sc_d = upload(t->specular_to_scattering, std::size_t(s.n_xy) * c16);
+  b->sc2s_d = upload(t->scattering_to_specular, std::size_t(s.n_xy) * c16);
+
+  // Angular transform tables (the Legendre matrix stays FP64 and is rounded at use).
+  std::vector<int> offsets(s.n_blocks), rot_block(s.n_rot, -1);
+  int total = 0, covered = 0;
+  for (int blk = 0; blk < s.n_blocks; ++blk) {
+    offsets[blk] = total;
+    const int first = t->block_first[blk], size = t->block_size[blk];
+    if (first < 0 || size < 0 || first + size > s.n_rot) throw Status(SPO_HIP_E_ARGUMENT, "bad mJ block");
+    for (int j = 0; j < size; ++j) rot_block[first + j] = blk;
+    total += s.n_theta * size;
+    covered += size;
+  }
+  if (covered != s.n_rot) throw Status(SPO_HIP_E_ARGUMENT, "the mJ blocks do not cover the basis");
+  if (2 * s.mj_max + 1 > s.n_phi) throw Status(SPO_HIP_E_ARGUMENT, "nphi < 2 MJMax + 1");
+  b->legendre = upload(t->legendre, std::size_t(total) * sizeof(double));
+  b->legendre_t = upload_real<T>(t->legendre, std::size_t(total));
+  b->legendre_size = total;
+  b->block_first = upload(t->block_first, std::size_t(s.n_blocks) * sizeof(int));
+  b->block_size = upload(t->block_size, std::size_t(s.n_blocks) * sizeof(int));
+  b->block_offset = upload(offsets.data(), offsets.size() * sizeof(int));
+  b->rot_block = upload(rot_block.data(), rot_block.size() * sizeof(int));
+  for (int ixy = 0; ixy < s.n_xy; ++ixy)
+    if (t->representative_of[ixy] < 0 || t->representative_of[ixy] >= s.n_rep || t->operation_of[ixy] < 0 ||
+        t->operation_of[ixy] >= shape->n_operations)
+      throw Status(SPO_HIP_E_ARGUMENT, "bad symmetry map");
+  for (int k = 0; k < s.n_phi * shape->n_operations; ++k)
+    if (t->phi_index[k] < 0 || t->phi_index[k] >= s.n_phi) throw Status(SPO_HIP_E_ARGUMENT, "bad phi map");
+  b->rep_of = upload(t->representative_of, std::size_t(s.n_xy) * sizeof(int));
+  b->op_of = upload(t->operation_of, std::size_t(s.n_xy) * sizeof(int));
+  b->phi_index = upload(t->phi_index, std::size_t(s.n_phi) * shape->n_operations * sizeof(int));
+  b->exp_table = upload_complex<T>(t->exp_potential, exp_values);
+
+  // Analysis (FP64).
+  b->analysis = *a;
+  if (a->n_states != a->n_rotation * a->n_diffraction || a->n_v < 1 || a->n_energies < 1 || a->iz_analysis < 0 ||
+      a->iz_analysis >= s.nz || a->initial_state < 0 || a->initial_state >= a->n_states || a->n_diffraction > s.n_xy)
+    throw Status(SPO_HIP_E_ARGUMENT, "spo_hip_create: inconsistent analysis");
+  for (int k = 0; k < a->n_rotation; ++k)
+    if (a->rotation_index[k] < 0 || a->rotation_index[k] >= s.n_rot) throw Status(SPO_HIP_E_ARGUMENT, "bad rotation index");
+  for (int k = 0; k < a->n_diffraction; ++k)
+    if (a->diffraction_slot[k] < 0 || a->diffraction_slot[k] >= s.n_xy) throw Status(SPO_HIP_E_ARGUMENT, "bad slot");
+  b->rot_index = upload(a->rotation_index, std::size_t(a->n_rotation) * sizeof(int));
+  b->dif_slot = upload(a->diffraction_slot, std::size_t(a->n_diffraction) * sizeof(int));
+  b->radial_weight = upload(a->radial_weight, std::size_t(s.nr) * a->n_rotation * a->n_v * sizeof(double));
+  b->initial_radial = upload(a->initial_radial, std::size_t(s.nr) * a->n_v * sizeof(double));
+  b->energies = upload(a->total_energy_hartree, std::size_t(a->n_energies) * sizeof(double));
+  b->analysis.rotation_index = nullptr;
+  b->analysis.diffraction_slot = nullptr;
+  b->analysis.radial_weight = nullptr;
+  b->analysis.initial_radial = nullptr;
+  b->analysis.total_energy_hartree = nullptr;
+  b->bk = Buffer(std::size_t(a->n_states) * a->n_v * c16);
+  const std::size_t n_sums = std::size_t(a->n_states) * a->n_v * a->n_energies;
+  b->sums = Buffer(n_sums * c16);
+  HIP_CHECK(hipMemsetAsync(b->sums.template as<void>(), 0, n_sums * c16, b->stream));
+
+  // Plans, tile widths and the kernels for this shape.
+  b->plan_z = std::make_unique<DevicePlan<T>>(s.nz);
+  b->plan_r = std::make_unique<DevicePlan<T>>(s.nr);
+  b->plan_x = std::make_unique<DevicePlan<T>>(s.nx);
+  b->plan_y = std::make_unique<DevicePlan<T>>(s.ny);
+  b->plan_phi = std::make_unique<DevicePlan<T>>(s.n_phi);
+  b->plan_r_d = std::make_unique<DevicePlan<double>>(s.nr);
+  b->plan_zsp = std::make_unique<DevicePlan<double>>(s.nzsp);
+  b->plan_x_d = std::make_unique<DevicePlan<double>>(s.nx);
+  b->plan_y_d = std::make_unique<DevicePlan<double>>(s.ny);
+  const std::size_t lds_limit = static_cast<std::size_t>(b->max_lds);
+  // T_Zr: the compile-time kernel (padded rows) if instantiated and it fits, else the run-time one, else lines.
+  const std::size_t zr_ct_lds = std::size_t(s.nz) * (s.nr + 1) * cT;
+  b->zr_fn = zr_ct_lds <= lds_limit ? zr_ct_symbol<T>(s.nz, s.nr, &b->zr_threads) : nullptr;
+  if (b->zr_fn) {
+    b->zr_lds = zr_ct_lds;
+    set_lds(b->zr_fn, b->zr_lds);
+  } else if (std::size_t(s.n_zr) * cT <= lds_limit) {
+    b->zr_fused = true;
+    b->zr_lds = std::size_t(s.n_zr) * cT;
+    set_lds(reinterpret_cast<const void*>(spo_dev::zr_plane_kernel<T>), b->zr_lds);
+  }
+  // T_xy.
+  int xy_c = 0;
+  b->xy_fn = xy_ct_symbol<T>(s.nx, s.ny, &xy_c);
+  if (b->xy_fn) {
+    b->xy_c = xy_c;
+    b->xy_lds = std::size_t(s.nx) * (s.ny * (xy_c + 1) + 1) * cT;
+  } else {
+    b->xy_c = std::max(1, std::min(64, 4096 / s.n_xy));
+    b->xy_lds = std::size_t(b->xy_c) * s.n_xy * cT;
+  }
+  if (b->xy_lds > lds_limit) throw Status(SPO_HIP_E_NOT_SUPPORTED, "the (x, y) tile does not fit in LDS");
+  set_lds(b->xy_fn ? b->xy_fn : reinterpret_cast<const void*>(spo_dev::xy_kernel<T>), b->xy_lds);
+  // Interaction tile: the largest power of two of columns (at most 32) whose LDS fits in 80 KiB (two per CU).
+  b->v_fn = s.n_theta <= spo_dev::kMaxTheta ? interaction_ct_symbol<T>(s.n_phi) : nullptr;
+  auto v_bytes = [&](int c) {
+    return b->v_fn ? spo_dev::interaction_ct_lds<T>(c, s.n_theta, s.n_phi, b->legendre_size)
+                   : std::size_t(c) * (s.n_rot + s.n_tp) * cT;
+  };
+  int p2 = 1;
+  while (p2 < 32 && v_bytes(2 * p2) <= 81920) p2 *= 2;
+  b->v_c = p2;
+  b->v_lds = v_bytes(p2);
+  if (b->v_lds > lds_limit) throw Status(SPO_HIP_E_NOT_SUPPORTED, "an angular column does not fit in LDS");
+  set_lds(b->v_fn ? b->v_fn : reinterpret_cast<const void*>(spo_dev::interaction_kernel<T>), b->v_lds);
+  set_lds(reinterpret_cast<const void*>(spo_dev::lines_fft_kernel<T>), 65536);
+  set_lds(reinterpret_cast<const void*>(spo_dev::lines_fft_kernel<double>), 65536);
+  set_lds(reinterpret_cast<const void*>(spo_dev::bk_scattering_kernel<T>), std::size_t(s.n_xy) * c16);
+  HIP_CHECK(hipStreamSynchronize(b->stream));
+  return b;
+}
+
+}  // namespace
 
 extern "C" {
 
@@ -240,226 +751,38 @@ int spo_hip_create(const spo_hip_shape* shape, const spo_hip_tables* t, const sp
     int devices = 0;
     if (hipGetDeviceCount(&devices) != hipSuccess || devices < 1)
       throw Status(SPO_HIP_E_NOT_SUPPORTED, "no HIP device is visible");
-    if (shape->precision != SPO_HIP_FP64) throw Status(SPO_HIP_E_NOT_SUPPORTED, "unsupported precision");
-    auto b = std::make_unique<spo_hip_backend>();
-    b->shape = *shape;
-    spo_dev::Shape& s = b->s;
-    s.n_rot = shape->n_rot;
-    s.nx = shape->nx;
-    s.ny = shape->ny;
-    s.n_xy = shape->nx * shape->ny;
-    s.nr = shape->nr;
-    s.nz = shape->nz;
-    s.n_zr = shape->nr * shape->nz;
-    s.nzsp = shape->nz_specular;
-    s.nzpot = shape->nz_potential;
-    s.n_zr_pot = shape->nz_potential * shape->nr;
-    s.n_theta = shape->n_theta;
-    s.n_phi = shape->n_phi;
-    s.n_tp = shape->n_theta * shape->n_phi;
-    s.n_rep = shape->n_representatives;
-    s.ichn = shape->initial_channel;
-    s.mj_max = shape->mj_max;
-    s.n_blocks = 2 * shape->mj_max + 1;
-    s.r_abs_first = shape->r_absorber_first;
-    s.r_abs_last = shape->r_absorber_last;
-    if (s.n_rot < 1 || s.nx < 2 || s.ny < 2 || s.nr < 2 || s.nz < 2 || s.nzsp < s.nz || s.nzpot < 0 || s.nzpot > s.nz ||
-        s.n_theta < 1 || s.n_phi < 1 || s.n_rep < 1 || s.ichn < 0 || s.ichn >= s.n_rot || shape->n_operations < 1 ||
-        shape->steps_per_analysis < 1)
-      throw Status(SPO_HIP_E_ARGUMENT, "spo_hip_create: inconsistent shape");
-
-    int device = 0;
-    HIP_CHECK(hipGetDevice(&device));
-    HIP_CHECK(hipDeviceGetAttribute(&b->max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device));
-    const std::size_t psi_values = std::size_t(s.n_zr) * s.n_rot * s.n_xy;
-    const std::size_t exp_values = std::size_t(s.n_tp) * s.n_rep * s.n_zr_pot;
-    std::size_t free_bytes = 0, total_bytes = 0;
-    HIP_CHECK(hipMemGetInfo(&free_bytes, &total_bytes));
-    const std::size_t need = (psi_values + exp_values) * sizeof(cx<double>) + (std::size_t(1) << 28);
-    if (need > free_bytes)
-      throw Status(SPO_HIP_E_NOT_SUPPORTED, "not enough device memory: need " + std::to_string(need >> 20) +
-                                                " MiB, free " + std::to_string(free_bytes >> 20) + " MiB");
-    HIP_CHECK(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
-
-    const std::size_t c16 = sizeof(cx<double>);
-    b->psi = Buffer(psi_values * c16);
-    HIP_CHECK(hipMemsetAsync(b->psi.as<void>(), 0, psi_values * c16, b->stream));
-    b->spec = upload(initial_specular, std::size_t(s.nr) * s.nzsp * c16);
-    b->densities = Buffer(std::size_t(s.n_zr) * sizeof(double));
-    b->plane = Buffer(std::size_t(s.n_rot) * s.n_xy * s.nr * c16);
-    b->staging_values = std::max<std::size_t>(std::size_t(s.n_rot) * s.n_xy, std::size_t(4) << 20);
-    b->staging = Buffer(b->staging_values * c16);
-
-    // Kinetic factors with the forward and backward FFT scales folded in.
-    auto folded = [&](const double* f, std::size_t n, double fwd, double bwd) {
-      std::vector<cx<double>> v = complex_from(f, n);
-      for (auto& x : v) {
-        x.x = x.x * fwd * bwd;
-        x.y = x.y * fwd * bwd;
-      }
-      return upload(v.data(), n * c16);
-    };
-    const double zr_f = t->fft_scale_z * t->fft_scale_r;
-    const double zr_b = (1.0 / (s.nz * t->fft_scale_z)) * (1.0 / (s.nr * t->fft_scale_r));
-    const double sp_b = (1.0 / (s.nzsp * t->fft_scale_z)) * (1.0 / (s.nr * t->fft_scale_r));
-    const double xy_f = t->fft_scale_x * t->fft_scale_y;
-    const double xy_b = (1.0 / (s.nx * t->fft_scale_x)) * (1.0 / (s.ny * t->fft_scale_y));
-    b->zr_full = folded(t->zr_full, s.n_zr, zr_f, zr_b);
-    b->zr_half = folded(t->zr_half, s.n_zr, zr_f, zr_b);
-    b->spec_full = folded(t->specular_full, std::size_t(s.nr) * s.nzsp, zr_f, sp_b);
-    b->spec_half = folded(t->specular_half, std::size_t(s.nr) * s.nzsp, zr_f, sp_b);
-    b->xy_full = folded(t->xy_full, s.n_xy, xy_f, xy_b);
-    b->xy_half = folded(t->xy_half, s.n_xy, xy_f, xy_b);
-
-    b->rot_half = upload(t->rotation_half, std::size_t(s.n_rot) * s.nr * c16);
-    b->before = upload(t->before_interaction, std::size_t(s.n_rot) * s.nr * c16);
-    b->gas_step = upload(t->gas_phase_step, std::size_t(s.n_rot) * s.nr * c16);
-    b->spec_gas = upload(t->specular_gas_phase, std::size_t(s.nr) * c16);
-    b->z_opt = upload(t->z_optical, std::size_t(s.nz) * sizeof(double));
-    b->r_opt = upload(t->r_optical, std::size_t(s.nr) * sizeof(double));
-    b->z_spec_opt = upload(t->z_specular_optical, std::size_t(s.nzsp) * sizeof(double));
-    b->shift_f = upload(t->shift_forward, std::size_t(s.n_xy) * c16);
-    b->shift_b = upload(t->shift_backward, std::size_t(s.n_xy) * c16);
-    b->s2sc = upload(t->specular_to_scattering, std::size_t(s.n_xy) * c16);
-    b->sc2s = upload(t->scattering_to_specular, std::size_t(s.n_xy) * c16);
-
-    // Angular transform tables.
-    std::vector<int> offsets(s.n_blocks), rot_block(s.n_rot, -1);
-    int total = 0, covered = 0;
-    for (int blk = 0; blk < s.n_blocks; ++blk) {
-      offsets[blk] = total;
-      const int first = t->block_first[blk], size = t->block_size[blk];
-      if (first < 0 || size < 0 || first + size > s.n_rot) throw Status(SPO_HIP_E_ARGUMENT, "bad mJ block");
-      for (int j = 0; j < size; ++j) rot_block[first + j] = blk;
-      total += s.n_theta * size;
-      covered += size;
-    }
-    if (covered != s.n_rot) throw Status(SPO_HIP_E_ARGUMENT, "the mJ blocks do not cover the basis");
-    if (2 * s.mj_max + 1 > s.n_phi) throw Status(SPO_HIP_E_ARGUMENT, "nphi < 2 MJMax + 1");
-    b->legendre = upload(t->legendre, std::size_t(total) * sizeof(double));
-    b->block_first = upload(t->block_first, std::size_t(s.n_blocks) * sizeof(int));
-    b->block_size = upload(t->block_size, std::size_t(s.n_blocks) * sizeof(int));
-    b->block_offset = upload(offsets.data(), offsets.size() * sizeof(int));
-    b->rot_block = upload(rot_block.data(), rot_block.size() * sizeof(int));
-    for (int ixy = 0; ixy < s.n_xy; ++ixy)
-      if (t->representative_of[ixy] < 0 || t->representative_of[ixy] >= s.n_rep || t->operation_of[ixy] < 0 ||
-          t->operation_of[ixy] >= shape->n_operations)
-        throw Status(SPO_HIP_E_ARGUMENT, "bad symmetry map");
-    for (int k = 0; k < s.n_phi * shape->n_operations; ++k)
-      if (t->phi_index[k] < 0 || t->phi_index[k] >= s.n_phi) throw Status(SPO_HIP_E_ARGUMENT, "bad phi map");
-    b->rep_of = upload(t->representative_of, std::size_t(s.n_xy) * sizeof(int));
-    b->op_of = upload(t->operation_of, std::size_t(s.n_xy) * sizeof(int));
-    b->phi_index = upload(t->phi_index, std::size_t(s.n_phi) * shape->n_operations * sizeof(int));
-    b->exp_table = upload(t->exp_potential, exp_values * c16);
-
-    // Analysis.
-    b->analysis = *a;
-    if (a->n_states != a->n_rotation * a->n_diffraction || a->n_v < 1 || a->n_energies < 1 ||
-        a->iz_analysis < 0 || a->iz_analysis >= s.nz || a->initial_state < 0 || a->initial_state >= a->n_states ||
-        a->n_diffraction > s.n_xy)
-      throw Status(SPO_HIP_E_ARGUMENT, "spo_hip_create: inconsistent analysis");
-    for (int k = 0; k < a->n_rotation; ++k)
-      if (a->rotation_index[k] < 0 || a->rotation_index[k] >= s.n_rot) throw Status(SPO_HIP_E_ARGUMENT, "bad rotation index");
-    for (int k = 0; k < a->n_diffraction; ++k)
-      if (a->diffraction_slot[k] < 0 || a->diffraction_slot[k] >= s.n_xy) throw Status(SPO_HIP_E_ARGUMENT, "bad slot");
-    b->rot_index = upload(a->rotation_index, std::size_t(a->n_rotation) * sizeof(int));
-    b->dif_slot = upload(a->diffraction_slot, std::size_t(a->n_diffraction) * sizeof(int));
-    b->radial_weight = upload(a->radial_weight, std::size_t(s.nr) * a->n_rotation * a->n_v * sizeof(double));
-    b->initial_radial = upload(a->initial_radial, std::size_t(s.nr) * a->n_v * sizeof(double));
-    b->energies = upload(a->total_energy_hartree, std::size_t(a->n_energies) * sizeof(double));
-    b->analysis.rotation_index = nullptr;
-    b->analysis.diffraction_slot = nullptr;
-    b->analysis.radial_weight = nullptr;
-    b->analysis.initial_radial = nullptr;
-    b->analysis.total_energy_hartree = nullptr;
-    b->bk = Buffer(std::size_t(a->n_states) * a->n_v * c16);
-    const std::size_t n_sums = std::size_t(a->n_states) * a->n_v * a->n_energies;
-    b->sums = Buffer(n_sums * c16);
-    HIP_CHECK(hipMemsetAsync(b->sums.as<void>(), 0, n_sums * c16, b->stream));
-
-    // Plans and tile widths.
-    b->plan_z = std::make_unique<DevicePlan>(s.nz);
-    b->plan_r = std::make_unique<DevicePlan>(s.nr);
-    b->plan_x = std::make_unique<DevicePlan>(s.nx);
-    b->plan_y = std::make_unique<DevicePlan>(s.ny);
-    b->plan_phi = std::make_unique<DevicePlan>(s.n_phi);
-    b->plan_zsp = std::make_unique<DevicePlan>(s.nzsp);
-    const std::size_t lds_limit = static_cast<std::size_t>(b->max_lds);
-    b->zr_fused = std::size_t(s.n_zr) * c16 <= lds_limit;
-    b->xy_c = std::max(1, std::min(64, 4096 / s.n_xy));
-    int vc = static_cast<int>(81920 / ((s.n_rot + s.n_tp) * c16));
-    int p2 = 1;
-    while (p2 * 2 <= std::min(vc, 32)) p2 *= 2;
-    b->v_c = std::max(1, p2);
-    if (std::size_t(s.n_xy) * c16 > lds_limit || std::size_t(b->v_c) * (s.n_rot + s.n_tp) * c16 > lds_limit)
-      throw Status(SPO_HIP_E_NOT_SUPPORTED, "a tile does not fit in LDS");
-    set_lds(reinterpret_cast<const void*>(spo_dev::lines_fft_kernel<double>), 65536);
-    set_lds(reinterpret_cast<const void*>(spo_dev::xy_kernel<double>), std::size_t(b->xy_c) * s.n_xy * c16);
-    set_lds(reinterpret_cast<const void*>(spo_dev::interaction_kernel<double>),
-            std::size_t(b->v_c) * (s.n_rot + s.n_tp) * c16);
-    if (b->zr_fused) set_lds(reinterpret_cast<const void*>(spo_dev::zr_plane_kernel<double>), std::size_t(s.n_zr) * c16);
-    set_lds(reinterpret_cast<const void*>(spo_dev::bk_scattering_kernel), std::size_t(s.n_xy) * c16);
-    HIP_CHECK(hipStreamSynchronize(b->stream));
+    std::unique_ptr<spo_hip_backend> b;
+    if (shape->precision == SPO_HIP_FP64)
+      b = create_state<double>(shape, t, a, initial_specular);
+    else if (shape->precision == SPO_HIP_MIXED)
+      b = create_state<float>(shape, t, a, initial_specular);
+    else
+      throw Status(SPO_HIP_E_NOT_SUPPORTED, "unsupported precision");
     *out = b.release();
   });
 }
 
 int spo_hip_destroy(spo_hip_backend* b) {
-  if (!b) return SPO_HIP_OK;
-  if (b->stream) {
-    (void)hipStreamSynchronize(b->stream);
-    (void)hipStreamDestroy(b->stream);
-  }
   delete b;
   return SPO_HIP_OK;
 }
 
 int spo_hip_upload_wave(spo_hip_backend* b, const double* scattering, const double* specular) {
   return guarded([&] {
-    const spo_dev::Shape& s = b->s;
-    const std::size_t c16 = sizeof(cx<double>);
-    if (scattering) {
-      const std::size_t per_izr = std::size_t(s.n_rot) * s.n_xy;
-      const int chunk = static_cast<int>(std::max<std::size_t>(1, b->staging_values / per_izr));
-      for (int izr0 = 0; izr0 < s.n_zr; izr0 += chunk) {
-        const int count = std::min(chunk, s.n_zr - izr0);
-        HIP_CHECK(hipMemcpyAsync(b->staging.as<void>(), scattering + 2 * per_izr * izr0, per_izr * count * c16,
-                                 hipMemcpyHostToDevice, b->stream));
-        hipLaunchKernelGGL(spo_dev::host_to_device_kernel<double>, dim3(blocks_for(long(per_izr) * count, kThreads)),
-                           dim3(kThreads), 0, b->stream, b->psi.as<cx<double>>(), s,
-                           b->staging.as<cx<double>>(), izr0, count);
-        HIP_CHECK(hipGetLastError());
-        HIP_CHECK(hipStreamSynchronize(b->stream));
-      }
-    }
+    if (scattering) b->upload_wave(scattering);
     if (specular)
-      HIP_CHECK(hipMemcpyAsync(b->spec.as<void>(), specular, std::size_t(s.nr) * s.nzsp * c16, hipMemcpyHostToDevice,
-                               b->stream));
+      HIP_CHECK(hipMemcpyAsync(b->spec.as<void>(), specular, std::size_t(b->s.nr) * b->s.nzsp * sizeof(cx<double>),
+                               hipMemcpyHostToDevice, b->stream));
     HIP_CHECK(hipStreamSynchronize(b->stream));
   });
 }
 
 int spo_hip_download_wave(spo_hip_backend* b, double* scattering, double* specular) {
   return guarded([&] {
-    const spo_dev::Shape& s = b->s;
-    const std::size_t c16 = sizeof(cx<double>);
-    if (scattering) {
-      const std::size_t per_izr = std::size_t(s.n_rot) * s.n_xy;
-      const int chunk = static_cast<int>(std::max<std::size_t>(1, b->staging_values / per_izr));
-      for (int izr0 = 0; izr0 < s.n_zr; izr0 += chunk) {
-        const int count = std::min(chunk, s.n_zr - izr0);
-        hipLaunchKernelGGL(spo_dev::device_to_host_kernel<double>, dim3(blocks_for(long(per_izr) * count, kThreads)),
-                           dim3(kThreads), 0, b->stream, b->psi.as<cx<double>>(), s,
-                           b->staging.as<cx<double>>(), izr0, count);
-        HIP_CHECK(hipGetLastError());
-        HIP_CHECK(hipMemcpyAsync(scattering + 2 * per_izr * izr0, b->staging.as<void>(), per_izr * count * c16,
-                                 hipMemcpyDeviceToHost, b->stream));
-        HIP_CHECK(hipStreamSynchronize(b->stream));
-      }
-    }
+    if (scattering) b->download_wave(scattering);
     if (specular)
-      HIP_CHECK(hipMemcpyAsync(specular, b->spec.as<void>(), std::size_t(s.nr) * s.nzsp * c16, hipMemcpyDeviceToHost,
-                               b->stream));
+      HIP_CHECK(hipMemcpyAsync(specular, b->spec.as<void>(), std::size_t(b->s.nr) * b->s.nzsp * sizeof(cx<double>),
+                               hipMemcpyDeviceToHost, b->stream));
     HIP_CHECK(hipStreamSynchronize(b->stream));
   });
 }
@@ -492,10 +815,8 @@ int spo_hip_get_z_plane(spo_hip_backend* b, int iz, double* out) {
   return guarded([&] {
     const spo_dev::Shape& s = b->s;
     if (iz < 0 || iz >= s.nz) throw Status(SPO_HIP_E_ARGUMENT, "z plane index out of range");
+    b->z_plane(iz);
     const long n = long(s.n_rot) * s.n_xy * s.nr;
-    hipLaunchKernelGGL(spo_dev::z_plane_kernel<double>, dim3(blocks_for(n, kThreads)), dim3(kThreads), 0, b->stream,
-                       b->psi.as<cx<double>>(), s, iz, b->plane.as<cx<double>>());
-    HIP_CHECK(hipGetLastError());
     HIP_CHECK(hipMemcpyAsync(out, b->plane.as<void>(), n * sizeof(cx<double>), hipMemcpyDeviceToHost, b->stream));
     HIP_CHECK(hipStreamSynchronize(b->stream));
   });
@@ -503,57 +824,22 @@ int spo_hip_get_z_plane(spo_hip_backend* b, int iz, double* out) {
 
 int spo_hip_block_densities(spo_hip_backend* b, double* out) {
   return guarded([&] {
-    const spo_dev::Shape& s = b->s;
-    hipLaunchKernelGGL(spo_dev::densities_kernel<double>, dim3(blocks_for(s.n_zr, kThreads)), dim3(kThreads), 0,
-                       b->stream, b->psi.as<cx<double>>(), s, b->densities.as<double>());
-    HIP_CHECK(hipGetLastError());
-    HIP_CHECK(hipMemcpyAsync(out, b->densities.as<void>(), std::size_t(s.n_zr) * sizeof(double), hipMemcpyDeviceToHost,
-                             b->stream));
+    b->block_densities();
+    HIP_CHECK(hipMemcpyAsync(out, b->densities.as<void>(), std::size_t(b->s.n_zr) * sizeof(double),
+                             hipMemcpyDeviceToHost, b->stream));
     HIP_CHECK(hipStreamSynchronize(b->stream));
   });
 }
 
 int spo_hip_site_column(spo_hip_backend* b, int ixy, double* out) {
   return guarded([&] {
-    const spo_dev::Shape& s = b->s;
-    if (ixy < 0 || ixy >= s.n_xy) throw Status(SPO_HIP_E_ARGUMENT, "site index out of range");
-    const std::size_t n = std::size_t(s.n_zr) * s.n_rot;
-    HIP_CHECK(hipMemcpyAsync(out, b->psi.as<cx<double>>() + n * ixy, n * sizeof(cx<double>), hipMemcpyDeviceToHost,
-                             b->stream));
-    HIP_CHECK(hipStreamSynchronize(b->stream));
+    if (ixy < 0 || ixy >= b->s.n_xy) throw Status(SPO_HIP_E_ARGUMENT, "site index out of range");
+    b->site_column(ixy, out);
   });
 }
 
 int spo_hip_analyse(spo_hip_backend* b, int surface_fixed, int iz_specular, double t, double* coefficients) {
-  return guarded([&] {
-    const spo_dev::Shape& s = b->s;
-    const spo_hip_analysis& a = b->analysis;
-    const std::size_t n = std::size_t(a.n_states) * a.n_v;
-    hipLaunchKernelGGL(spo_dev::zero_kernel, dim3(blocks_for(long(n), kThreads)), dim3(kThreads), 0, b->stream,
-                       b->bk.as<cx<double>>(), long(n));
-    HIP_CHECK(hipGetLastError());
-    hipLaunchKernelGGL(spo_dev::bk_scattering_kernel, dim3(a.n_v * a.n_rotation), dim3(kThreads),
-                       std::size_t(s.n_xy) * sizeof(cx<double>), b->stream, b->psi.as<cx<double>>(), s, a.iz_analysis,
-                       a.n_rotation, a.n_diffraction, a.n_states, b->rot_index.as<int>(), b->dif_slot.as<int>(),
-                       b->radial_weight.as<double>(), b->shift_f.as<cx<double>>(), a.xy_scale, b->plan_x->plan,
-                       b->plan_y->plan, b->bk.as<cx<double>>());
-    HIP_CHECK(hipGetLastError());
-    if (surface_fixed) {
-      if (iz_specular < 0 || iz_specular >= s.nzsp) throw Status(SPO_HIP_E_ARGUMENT, "specular line out of range");
-      hipLaunchKernelGGL(spo_dev::bk_specular_kernel, dim3(1), dim3(64), 0, b->stream, b->spec.as<cx<double>>(), s.nr,
-                         iz_specular, a.n_v, a.n_states, a.initial_state, b->initial_radial.as<double>(), a.dr,
-                         b->bk.as<cx<double>>());
-      HIP_CHECK(hipGetLastError());
-    }
-    const long total = long(n) * a.n_energies;
-    hipLaunchKernelGGL(spo_dev::s_accumulate_kernel, dim3(blocks_for(total, kThreads)), dim3(kThreads), 0, b->stream,
-                       b->sums.as<cx<double>>(), b->bk.as<cx<double>>(), static_cast<int>(n), a.n_energies,
-                       b->energies.as<double>(), t, a.analysis_time_step);
-    HIP_CHECK(hipGetLastError());
-    if (coefficients)
-      HIP_CHECK(hipMemcpyAsync(coefficients, b->bk.as<void>(), n * sizeof(cx<double>), hipMemcpyDeviceToHost, b->stream));
-    HIP_CHECK(hipStreamSynchronize(b->stream));
-  });
+  return guarded([&] { b->analyse(surface_fixed, iz_specular, t, coefficients); });
 }
 
 int spo_hip_get_s_sums(spo_hip_backend* b, double* out) {
@@ -562,6 +848,37 @@ int spo_hip_get_s_sums(spo_hip_backend* b, double* out) {
     const std::size_t n = std::size_t(a.n_states) * a.n_v * a.n_energies;
     HIP_CHECK(hipMemcpyAsync(out, b->sums.as<void>(), n * sizeof(cx<double>), hipMemcpyDeviceToHost, b->stream));
     HIP_CHECK(hipStreamSynchronize(b->stream));
+  });
+}
+
+int spo_hip_profile_enable(spo_hip_backend* b, int on) {
+  return guarded([&] {
+    HIP_CHECK(hipStreamSynchronize(b->stream));
+    for (const auto& t : b->timed_launches) {
+      b->event_pool.push_back(t.start);
+      b->event_pool.push_back(t.stop);
+    }
+    b->timed_launches.clear();
+    b->profiling = on != 0;
+  });
+}
+
+int spo_hip_profile_read(spo_hip_backend* b, spo_hip_kernel_time* out, int capacity, int* count) {
+  return guarded([&] {
+    HIP_CHECK(hipStreamSynchronize(b->stream));
+    std::vector<spo_hip_kernel_time> agg(b->kernel_names.size());
+    for (std::size_t k = 0; k < agg.size(); ++k) {
+      std::memset(&agg[k], 0, sizeof(agg[k]));
+      std::strncpy(agg[k].name, b->kernel_names[k].c_str(), sizeof(agg[k].name) - 1);
+    }
+    for (const auto& t : b->timed_launches) {
+      float ms = 0.0f;
+      HIP_CHECK(hipEventElapsedTime(&ms, t.start, t.stop));
+      agg[t.kernel].launches += 1;
+      agg[t.kernel].total_ms += ms;
+    }
+    *count = static_cast<int>(agg.size());
+    for (int k = 0; k < *count && k < capacity; ++k) out[k] = agg[k];
   });
 }
 

@@ -7,6 +7,7 @@
 
 #include <functional>
 #include <memory>
+#include <string>
 #include <vector>
 
 #include "analysis.hpp"
@@ -63,6 +64,15 @@ class PropagationBackend {
   virtual void analyse(const AnalysisSurface& surface, double time, std::vector<cplx>* coefficients) = 0;
   // The FP64 sums (is, v, ien).
   virtual std::vector<cplx> s_matrix_sums() = 0;
+
+  // Per-kernel device timing (a backend without kernels reports nothing).
+  struct KernelTime {
+    std::string name;
+    long launches = 0;
+    double total_ms = 0.0;
+  };
+  virtual void set_kernel_timing(bool) {}
+  virtual std::vector<KernelTime> kernel_times() { return {}; }
 };
 
 using BackendFactory = std::function<std::unique_ptr<PropagationBackend>(const BackendInput&)>;

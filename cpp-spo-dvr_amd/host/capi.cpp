@@ -253,6 +253,18 @@ int spo_run_array(const spo_run* run, const char* name, double* out, size_t capa
       for (double e : s.gas.energies) v.push_back(e * spo::Units::hartree_in_ev);
     } else if (n == "symmetry.representatives") {
       v.assign(s.potential.symmetry.representatives.begin(), s.potential.symmetry.representatives.end());
+    } else if (n == "symmetry.representative_of") {
+      v.assign(s.potential.symmetry.representative_of.begin(), s.potential.symmetry.representative_of.end());
+    } else if (n == "symmetry.operation_of") {
+      v.assign(s.potential.symmetry.operation_of.begin(), s.potential.symmetry.operation_of.end());
+    } else if (n == "symmetry.phi_index") {
+      for (const auto& row : s.potential.symmetry.phi_index) v.insert(v.end(), row.begin(), row.end());
+    } else if (n == "analysis.limits") {
+      // jana, nviba, nana, nstata, inchna (1-based), as the reference's stdout prints them
+      v = {double(s.gas.j_max_analysis), double(s.analysis.n_v), double(s.analysis.n_rotation()),
+           double(s.analysis.n_states()), double(s.analysis.initial + 1)};
+    } else if (n == "energies.total") {
+      v = s.energies.total;
     } else if (n == "tables.z_optical") v = s.tables.z_optical;
     else if (n == "tables.r_optical") v = s.tables.r_optical;
     else if (n == "initial.specular") {
@@ -266,6 +278,23 @@ int spo_run_array(const spo_run* run, const char* name, double* out, size_t capa
     if (out) {
       if (capacity < v.size()) throw std::invalid_argument("spo_run_array: capacity too small");
       std::memcpy(out, v.data(), v.size() * sizeof(double));
+    }
+  });
+}
+
+int spo_run_kernel_timing(spo_run* run, int on) {
+  return guarded([&] { run->propagation->backend().set_kernel_timing(on != 0); });
+}
+
+int spo_run_kernel_times(spo_run* run, char* names, long* launches, double* total_ms, int capacity, int* count) {
+  return guarded([&] {
+    const auto times = run->propagation->backend().kernel_times();
+    *count = static_cast<int>(times.size());
+    for (int k = 0; k < *count && k < capacity; ++k) {
+      std::strncpy(names + 48 * k, times[k].name.c_str(), 47);
+      names[48 * k + 47] = '\0';
+      launches[k] = times[k].launches;
+      total_ms[k] = times[k].total_ms;
     }
   });
 }

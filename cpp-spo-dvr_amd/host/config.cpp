@@ -1,8 +1,11 @@
 #include "config.hpp"
 
+#include "grids.hpp"
+
 #include <array>
 #include <cmath>
 #include <numeric>
+#include <utility>
 
 namespace spo {
 
@@ -185,6 +188,7 @@ InputParameters read_input(const json::Value& d) {
   get(d, "ZSpOptLowEnergy", p.zsp_opt_low.energy);
   get(d, "ZSpOptLowFactor", p.zsp_opt_low.factor);
   get(d, "TolSp", p.tol_specular);
+  get(d, "Precision", p.precision);
   return p;
 }
 
@@ -202,6 +206,17 @@ Config build_config(const InputParameters& input) {
   if (p.nr < 2) throw InputError("nR", "nR = 1 is not supported");
   if (p.nz_specular < p.nz) throw InputError("nZSpecular", "nZSpecular < nZ");
   if (p.nx < 2 || p.ny < 2) throw NotSupportedError("nX = 1 or nY = 1 is not supported");
+  if (p.precision != "fp64" && p.precision != "mixed")
+    throw InputError("Precision", "Precision must be \"fp64\" or \"mixed\", not \"" + p.precision + "\"");
+  // src/input/validate.cpp:48-66: every FFT grid must have a length 2^h 3^i 5^j 7^k 11^m (h >= 1, i <= 2).
+  const std::pair<const char*, int> fft_grids[] = {
+      {"nZ", p.nz}, {"nZSpecular", p.nz_specular}, {"nR", p.nr}, {"nX", p.nx}, {"nY", p.ny}};
+  for (const auto& [key, n] : fft_grids)
+    if (n < 1 || nearest_fft_size(n, 1) != n) throw InputError(key, std::string(key) + " is not a valid FFT size");
+  {
+    const std::string& g = d.system.point_group;
+    if ((g == "C4v" || g == "C6v" || g == "C3v") && p.nx != p.ny) throw InputError("nY", "nX /= nY");
+  }
 
   // src/input/process.cpp:37-94 (ProcessInput).
   d.mj_max_basis = p.mj_max < 1 ? p.j_max : p.mj_max;

@@ -62,6 +62,9 @@ struct InputParameters {
   std::optional<double> zsp_opt_energy, zsp_opt_factor;
   LowAbsorber z_opt_low, zsp_opt_low;
   double tol_specular = 0.0;
+  // Storage precision of the device backend's scattering wave ("fp64", or "mixed": complex64 with FP64 phases,
+  // specular grid and analysis). Not a key of the reference program.
+  std::string precision = "fp64";
 };
 
 struct DerivedParameters {

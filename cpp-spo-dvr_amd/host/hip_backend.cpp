@@ -69,6 +69,16 @@ class HipBackend final : public PropagationBackend {
     check(spo_hip_get_s_sums(handle_, reinterpret_cast<double*>(out.data())));
     return out;
   }
+  void set_kernel_timing(bool on) override { check(spo_hip_profile_enable(handle_, on ? 1 : 0)); }
+  std::vector<KernelTime> kernel_times() override {
+    int count = 0;
+    check(spo_hip_profile_read(handle_, nullptr, 0, &count));
+    std::vector<spo_hip_kernel_time> raw(static_cast<std::size_t>(count));
+    check(spo_hip_profile_read(handle_, raw.data(), count, &count));
+    std::vector<KernelTime> out;
+    for (const auto& r : raw) out.push_back({r.name, r.launches, r.total_ms});
+    return out;
+  }
 
  private:
   spo_hip_backend* handle_ = nullptr;
@@ -105,7 +115,9 @@ HipBackend::HipBackend(const BackendInput& in) {
   shape.r_absorber_first = in.grids.r_optical.first;
   shape.r_absorber_last = in.grids.r_optical.last;
   shape.needs_shift = t.needs_shift ? 1 : 0;
-  shape.precision = SPO_HIP_FP64;
+  if (p.precision == "fp64") shape.precision = SPO_HIP_FP64;
+  else if (p.precision == "mixed") shape.precision = SPO_HIP_MIXED;
+  else throw InputError("Precision", "expected \"fp64\" or \"mixed\"");
 
   const PotentialStepFactors f = potential_step_factors(n_rot, nr, t);
   const std::vector<cplx> spec_gas = specular_gas_phase_factors(nr, n_rot, in.basis.initial, t);

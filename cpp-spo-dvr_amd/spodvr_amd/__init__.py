@@ -88,6 +88,9 @@ def host_library() -> ctypes.CDLL:
         lib.spo_run_array.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_void_p, ctypes.c_size_t,
                                       ctypes.POINTER(ctypes.c_size_t)]
         lib.spo_pes_value.argtypes = [ctypes.c_char_p] + [ctypes.c_double] * 6 + [ctypes.POINTER(ctypes.c_double)]
+        lib.spo_run_kernel_timing.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        lib.spo_run_kernel_times.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_void_p, ctypes.c_void_p,
+                                             ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
         _host = lib
     return _host
 
@@ -251,6 +254,27 @@ class Run:
     def s_sums(self) -> np.ndarray:
         out = np.zeros(self.info.n_energies * self.info.n_states * self.info.n_v, dtype=np.complex128)
         _check(host_library().spo_run_s_sums(self._h, _ptr(out)))
+        return out
+
+    def kernel_timing(self, on: bool) -> None:
+        """Starts (and clears) or stops per-kernel HIP-event timing on the backend's stream."""
+        _check(host_library().spo_run_kernel_timing(self._h, int(on)))
+
+    def kernel_times(self) -> dict:
+        """{kernel name: (launches, total milliseconds)} since timing was enabled."""
+        lib = host_library()
+        count = ctypes.c_int()
+        _check(lib.spo_run_kernel_times(self._h, None, None, None, 0, ctypes.byref(count)))
+        n = count.value
+        names = ctypes.create_string_buffer(48 * max(n, 1))
+        launches = np.zeros(max(n, 1), dtype=np.int64)
+        ms = np.zeros(max(n, 1), dtype=np.float64)
+        _check(lib.spo_run_kernel_times(self._h, names, _ptr(launches), _ptr(ms), n, ctypes.byref(count)))
+        raw = names.raw
+        out = {}
+        for k in range(n):
+            name = raw[48 * k:48 * (k + 1)].split(b"\0", 1)[0].decode()
+            out[name] = (int(launches[k]), float(ms[k]))
         return out
 
     def array(self, name: str) -> np.ndarray:

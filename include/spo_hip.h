@@ -27,7 +27,10 @@ enum {
                                   (reference: NotSupportedError -> exit 3, src/app/backend_factory.cpp:27, :52) */
 };
 
-enum { SPO_HIP_FP64 = 0 };
+/* Storage precision of the scattering wave and of the factors it is multiplied with. SPO_HIP_MIXED stores them
+ * as complex64 (factors built in FP64 on the host and rounded once); the specular wave, the projection onto it,
+ * the Balint-Kurti coefficients and the S-matrix sums stay FP64. */
+enum { SPO_HIP_FP64 = 0, SPO_HIP_MIXED = 1 };
 
 typedef struct spo_hip_backend spo_hip_backend;
 
@@ -117,6 +120,17 @@ int spo_hip_site_column(spo_hip_backend* backend, int ixy, double* out);
 int spo_hip_analyse(spo_hip_backend* backend, int surface_fixed, int iz_specular, double t, double* coefficients);
 /* The sums, (is, v, ien) with is fastest. */
 int spo_hip_get_s_sums(spo_hip_backend* backend, double* out);
+
+/* Kernel timing with HIP events recorded on the backend's stream around every launch (no counterpart in the
+ * reference; for the roofline measurement). Enabling clears the record. */
+typedef struct spo_hip_kernel_time {
+  char name[48];
+  long launches;
+  double total_ms;
+} spo_hip_kernel_time;
+int spo_hip_profile_enable(spo_hip_backend* backend, int on);
+/* Per kernel: launches and summed event time since profiling was enabled; count receives the number of kernels. */
+int spo_hip_profile_read(spo_hip_backend* backend, spo_hip_kernel_time* out, int capacity, int* count);
 
 #ifdef __cplusplus
 }

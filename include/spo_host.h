@@ -95,6 +95,12 @@ int spo_run_s_sums(const spo_run* run, double* out);
  * values for complex arrays); out may be NULL to query the count. */
 int spo_run_array(const spo_run* run, const char* name, double* out, size_t capacity, size_t* count);
 
+/* Per-kernel device timing of the backend (HIP events around every launch); enabling clears the record.
+ * spo_run_kernel_times fills up to capacity entries: names (48 chars each, NUL-terminated), launches and summed
+ * milliseconds; count receives the number of kernels. A backend without device kernels reports none. */
+int spo_run_kernel_timing(spo_run* run, int on);
+int spo_run_kernel_times(spo_run* run, char* names, long* launches, double* total_ms, int capacity, int* count);
+
 /* V(x, y, Z, r, theta, phi) in eV of a potential by SystemID. */
 int spo_pes_value(const char* system_id, double x, double y, double z, double r, double theta, double phi,
                   double* out);

@@ -1,4 +1,7 @@
 // TEST INFRASTRUCTURE — part of the CPU oracle (see oracle_fft.hpp).
+//
+// Stockham autosort over `batch` interleaved sequences. The work arrays hold real and imaginary parts apart
+// (element i of sequence b at [i * batch + b]), so the innermost loop over b is unit-stride and vectorises.
 #include "oracle_fft.hpp"
 
 #include <cmath>
@@ -10,91 +13,106 @@ namespace {
 
 constexpr long double kPiL = 3.14159265358979323846264338327950288L;
 
-struct C {
-  double re, im;
-};
-inline C add(C a, C b) { return {a.re + b.re, a.im + b.im}; }
-inline C sub(C a, C b) { return {a.re - b.re, a.im - b.im}; }
-inline C mul(C a, C b) { return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re}; }
-inline C scale(C a, double s) { return {a.re * s, a.im * s}; }
-// a * (sign i)
-inline C rot90(C a, int sign) { return sign < 0 ? C{a.im, -a.re} : C{-a.im, a.re}; }
-
-// In-register DFT of radix R with exponent sign `sign`; roots[q] = exp(sign 2 pi i q / R).
+// In-register DFT of radix R with exponent sign `sign`; (cr[q], ci[q]) = exp(sign 2 pi i q / R).
 template <int R>
-inline void dft(C* v, int sign, const C* roots) {
+inline void dft(double* vr, double* vi, int sign, const double* cr, const double* ci) {
   if constexpr (R == 2) {
-    const C a = v[0], b = v[1];
-    v[0] = add(a, b);
-    v[1] = sub(a, b);
-  } else if constexpr (R == 3) {
-    const double c = -0.5, s = 0.86602540378443864676 * sign;
-    const C t1 = add(v[1], v[2]), t2 = sub(v[1], v[2]);
-    const C m = {v[0].re + c * t1.re, v[0].im + c * t1.im};
-    const C js = {-s * t2.im, s * t2.re};
-    v[0] = add(v[0], t1);
-    v[1] = add(m, js);
-    v[2] = sub(m, js);
+    const double ar = vr[0], ai = vi[0], br = vr[1], bi = vi[1];
+    vr[0] = ar + br;
+    vi[0] = ai + bi;
+    vr[1] = ar - br;
+    vi[1] = ai - bi;
   } else if constexpr (R == 4) {
-    const C a0 = add(v[0], v[2]), a1 = sub(v[0], v[2]);
-    const C b0 = add(v[1], v[3]), b1 = rot90(sub(v[1], v[3]), sign);
-    v[0] = add(a0, b0);
-    v[2] = sub(a0, b0);
-    v[1] = add(a1, b1);
-    v[3] = sub(a1, b1);
+    const double a0r = vr[0] + vr[2], a0i = vi[0] + vi[2];
+    const double a1r = vr[0] - vr[2], a1i = vi[0] - vi[2];
+    const double b0r = vr[1] + vr[3], b0i = vi[1] + vi[3];
+    const double dr = vr[1] - vr[3], di = vi[1] - vi[3];
+    // (dr, di) * sign i
+    const double b1r = sign < 0 ? di : -di;
+    const double b1i = sign < 0 ? -dr : dr;
+    vr[0] = a0r + b0r;
+    vi[0] = a0i + b0i;
+    vr[2] = a0r - b0r;
+    vi[2] = a0i - b0i;
+    vr[1] = a1r + b1r;
+    vi[1] = a1i + b1i;
+    vr[3] = a1r - b1r;
+    vi[3] = a1i - b1i;
   } else {
-    C out[R];
-    for (int k = 0; k < R; ++k) {
-      C s = v[0];
-      for (int q = 1; q < R; ++q) s = add(s, mul(v[q], roots[(k * q) % R]));
-      out[k] = s;
+    // Odd prime R: pairs (q, R - q) share cos and flip sin, so y_k and y_{R-k} come from the same sums.
+    constexpr int H = (R - 1) / 2;
+    double ar[H], ai[H], br[H], bi[H];
+    for (int q = 1; q <= H; ++q) {
+      ar[q - 1] = vr[q] + vr[R - q];
+      ai[q - 1] = vi[q] + vi[R - q];
+      br[q - 1] = vr[q] - vr[R - q];
+      bi[q - 1] = vi[q] - vi[R - q];
     }
-    for (int k = 0; k < R; ++k) v[k] = out[k];
+    double orr[R], oi[R];
+    orr[0] = vr[0];
+    oi[0] = vi[0];
+    for (int q = 0; q < H; ++q) {
+      orr[0] += ar[q];
+      oi[0] += ai[q];
+    }
+    for (int k = 1; k <= H; ++k) {
+      double pr = vr[0], pi = vi[0], qr = 0.0, qi = 0.0;
+      for (int q = 1; q <= H; ++q) {
+        const int idx = (k * q) % R;  // cr[idx] = cos(2 pi idx / R), ci[idx] = sign sin(2 pi idx / R)
+        pr += ar[q - 1] * cr[idx];
+        pi += ai[q - 1] * cr[idx];
+        qr += br[q - 1] * ci[idx];
+        qi += bi[q - 1] * ci[idx];
+      }
+      // y_k = p + i q, y_{R-k} = p - i q
+      orr[k] = pr - qi;
+      oi[k] = pi + qr;
+      orr[R - k] = pr + qi;
+      oi[R - k] = pi - qr;
+    }
+    for (int k = 0; k < R; ++k) {
+      vr[k] = orr[k];
+      vi[k] = oi[k];
+    }
   }
 }
 
 template <int R>
-void stage(const C* in, C* out, int n, int span, int sign, const C* tw, const C* roots) {
+void stage(const double* in_r, const double* in_i, double* out_r, double* out_i, int n, int span, long batch,
+           int sign, const cplx* tw, const cplx* roots) {
   const int m = n / R;
   const int blocks = m / span;
+  double cr[R], ci[R];
+  for (int q = 0; q < R; ++q) {
+    cr[q] = roots[q].real();
+    ci[q] = sign < 0 ? roots[q].imag() : -roots[q].imag();
+  }
   for (int p = 0; p < blocks; ++p)
     for (int k = 0; k < span; ++k) {
       const int j = p * span + k;
-      C v[R];
-      v[0] = in[j];
-      const C* w = tw + static_cast<long>(k) * (R - 1);
+      const cplx* w = tw + static_cast<long>(k) * (R - 1);
+      double tr[R], ti[R];
+      tr[0] = 1.0;
+      ti[0] = 0.0;
       for (int r = 1; r < R; ++r) {
-        const C x = in[j + r * m];
-        const C t = sign < 0 ? w[r - 1] : C{w[r - 1].re, -w[r - 1].im};
-        v[r] = k == 0 ? x : mul(x, t);
+        tr[r] = w[r - 1].real();
+        ti[r] = sign < 0 ? w[r - 1].imag() : -w[r - 1].imag();
       }
-      dft<R>(v, sign, roots);
-      C* o = out + static_cast<long>(p) * span * R + k;
-      for (int r = 0; r < R; ++r) o[r * span] = v[r];
-    }
-}
-
-// The same stage for `batch` interleaved sequences: element i of sequence b at in[i * batch + b].
-template <int R>
-void stage_batch(const C* in, C* out, int n, int span, int batch, int sign, const C* tw, const C* roots) {
-  const int m = n / R;
-  const int blocks = m / span;
-  for (int p = 0; p < blocks; ++p)
-    for (int k = 0; k < span; ++k) {
-      const int j = p * span + k;
-      const C* w = tw + static_cast<long>(k) * (R - 1);
-      C t[R];
-      for (int r = 1; r < R; ++r) t[r] = sign < 0 ? w[r - 1] : C{w[r - 1].re, -w[r - 1].im};
-      C* o = out + (static_cast<long>(p) * span * R + k) * batch;
-      for (int b = 0; b < batch; ++b) {
-        C v[R];
-        v[0] = in[static_cast<long>(j) * batch + b];
-        for (int r = 1; r < R; ++r) {
-          const C x = in[static_cast<long>(j + r * m) * batch + b];
-          v[r] = k == 0 ? x : mul(x, t[r]);
+      const long o = static_cast<long>(p) * span * R + k;
+#pragma GCC ivdep
+      for (long b = 0; b < batch; ++b) {
+        double vr[R], vi[R];
+        for (int r = 0; r < R; ++r) {
+          const double xr = in_r[static_cast<long>(j + r * m) * batch + b];
+          const double xi = in_i[static_cast<long>(j + r * m) * batch + b];
+          vr[r] = xr * tr[r] - xi * ti[r];
+          vi[r] = xr * ti[r] + xi * tr[r];
         }
-        dft<R>(v, sign, roots);
-        for (int r = 0; r < R; ++r) o[static_cast<long>(r) * span * batch + b] = v[r];
+        dft<R>(vr, vi, sign, cr, ci);
+        for (int r = 0; r < R; ++r) {
+          out_r[(o + static_cast<long>(r) * span) * batch + b] = vr[r];
+          out_i[(o + static_cast<long>(r) * span) * batch + b] = vi[r];
+        }
       }
     }
 }
@@ -130,70 +148,48 @@ Fft1d::Fft1d(int n) : n_(n) {
   }
 }
 
-void Fft1d::run(cplx* x, int stride, int sign, cplx* work) const {
-  const int n = n_;
-  C* a = reinterpret_cast<C*>(work);
-  C* b = a + n;
-  for (int i = 0; i < n; ++i) a[i] = reinterpret_cast<const C*>(x)[static_cast<long>(i) * stride];
-  int span = 1;
-  long offset = 0;
-  long root_offset = 0;
-  for (const int radix : radices_) {
-    C roots[16];
-    for (int q = 0; q < radix; ++q) {
-      const C r = reinterpret_cast<const C*>(roots_.data())[root_offset + q];
-      roots[q] = sign < 0 ? r : C{r.re, -r.im};
-    }
-    const C* tw = reinterpret_cast<const C*>(table_.data()) + offset;
-    switch (radix) {
-      case 2: stage<2>(a, b, n, span, sign, tw, roots); break;
-      case 3: stage<3>(a, b, n, span, sign, tw, roots); break;
-      case 4: stage<4>(a, b, n, span, sign, tw, roots); break;
-      case 5: stage<5>(a, b, n, span, sign, tw, roots); break;
-      case 7: stage<7>(a, b, n, span, sign, tw, roots); break;
-      case 11: stage<11>(a, b, n, span, sign, tw, roots); break;
-      default: stage<13>(a, b, n, span, sign, tw, roots); break;
-    }
-    offset += static_cast<long>(span) * (radix - 1);
-    root_offset += radix;
-    std::swap(a, b);
-    span *= radix;
-  }
-  for (int i = 0; i < n; ++i) reinterpret_cast<C*>(x)[static_cast<long>(i) * stride] = a[i];
-}
+void Fft1d::run(cplx* x, int stride, int sign, cplx* work) const { run_batch(x, stride, 1, sign, work); }
 
 void Fft1d::run_batch(cplx* x, long stride, int batch, int sign, cplx* work) const {
   const int n = n_;
-  C* a = reinterpret_cast<C*>(work);
-  C* b = a + static_cast<long>(n) * batch;
+  const long nb = static_cast<long>(n) * batch;
+  double* ar = reinterpret_cast<double*>(work);
+  double* ai = ar + nb;
+  double* br = ai + nb;
+  double* bi = br + nb;
+  const double* xd = reinterpret_cast<const double*>(x);
   for (int i = 0; i < n; ++i)
-    for (int e = 0; e < batch; ++e) a[static_cast<long>(i) * batch + e] = reinterpret_cast<const C*>(x)[i * stride + e];
+    for (int e = 0; e < batch; ++e) {
+      ar[static_cast<long>(i) * batch + e] = xd[2 * (i * stride + e)];
+      ai[static_cast<long>(i) * batch + e] = xd[2 * (i * stride + e) + 1];
+    }
   int span = 1;
   long offset = 0;
   long root_offset = 0;
   for (const int radix : radices_) {
-    C roots[16];
-    for (int q = 0; q < radix; ++q) {
-      const C r = reinterpret_cast<const C*>(roots_.data())[root_offset + q];
-      roots[q] = sign < 0 ? r : C{r.re, -r.im};
-    }
-    const C* tw = reinterpret_cast<const C*>(table_.data()) + offset;
+    const cplx* tw = table_.data() + offset;
+    const cplx* roots = roots_.data() + root_offset;
     switch (radix) {
-      case 2: stage_batch<2>(a, b, n, span, batch, sign, tw, roots); break;
-      case 3: stage_batch<3>(a, b, n, span, batch, sign, tw, roots); break;
-      case 4: stage_batch<4>(a, b, n, span, batch, sign, tw, roots); break;
-      case 5: stage_batch<5>(a, b, n, span, batch, sign, tw, roots); break;
-      case 7: stage_batch<7>(a, b, n, span, batch, sign, tw, roots); break;
-      case 11: stage_batch<11>(a, b, n, span, batch, sign, tw, roots); break;
-      default: stage_batch<13>(a, b, n, span, batch, sign, tw, roots); break;
+      case 2: stage<2>(ar, ai, br, bi, n, span, batch, sign, tw, roots); break;
+      case 3: stage<3>(ar, ai, br, bi, n, span, batch, sign, tw, roots); break;
+      case 4: stage<4>(ar, ai, br, bi, n, span, batch, sign, tw, roots); break;
+      case 5: stage<5>(ar, ai, br, bi, n, span, batch, sign, tw, roots); break;
+      case 7: stage<7>(ar, ai, br, bi, n, span, batch, sign, tw, roots); break;
+      case 11: stage<11>(ar, ai, br, bi, n, span, batch, sign, tw, roots); break;
+      default: stage<13>(ar, ai, br, bi, n, span, batch, sign, tw, roots); break;
     }
     offset += static_cast<long>(span) * (radix - 1);
     root_offset += radix;
-    std::swap(a, b);
+    std::swap(ar, br);
+    std::swap(ai, bi);
     span *= radix;
   }
+  double* xo = reinterpret_cast<double*>(x);
   for (int i = 0; i < n; ++i)
-    for (int e = 0; e < batch; ++e) reinterpret_cast<C*>(x)[i * stride + e] = a[static_cast<long>(i) * batch + e];
+    for (int e = 0; e < batch; ++e) {
+      xo[2 * (i * stride + e)] = ar[static_cast<long>(i) * batch + e];
+      xo[2 * (i * stride + e) + 1] = ai[static_cast<long>(i) * batch + e];
+    }
 }
 
 }  // namespace oracle

@@ -44,6 +44,12 @@ SMOKE_DAILIGHT = dict(_COMMON, **{
     "UseSwitching": True, "ZminSwitch": 5.0, "ZmaxSwitch": 7.0,
 })
 
+# Shapes without a compile-time kernel instantiation (NZ 72, 6 x 6, nphi 14): the run-time-shaped kernels.
+SMOKE_GENERIC = dict(SMOKE_DAILIGHT, **{
+    "Title": "small Dai-Light grid on shapes served by the run-time-shaped kernels",
+    "dZ": 0.14, "nZ": 72, "nX": 6, "nY": 6, "JMax": 6, "MJMax": 6, "ZOptEnd": 8.94,
+})
+
 HARDWALL_6D = dict(_COMMON, **{
     "Title": "HardWallPES 6D workload",
     "SystemID": "HardWallPES",

@@ -17,6 +17,7 @@ TOL = 1e-12
 
 CASE_PARAMS = {
     "smoke": lambda: cases.write_case("smoke_dailight", cases.SMOKE_DAILIGHT),
+    "generic": lambda: cases.write_case("smoke_generic", cases.SMOKE_GENERIC),
     "hardwall_short": lambda: cases.golden_input("hardwall_short"),
     "cu111_dailight_short": lambda: cases.golden_input("cu111_dailight_short"),
     "cu111_dailight_short_sym": lambda: cases.golden_input("cu111_dailight_short_sym"),
@@ -58,7 +59,7 @@ def test_one_t_block_matches_oracle(case, half, kinetic_only):
     assert relative_difference(s0, sc) > 1e-3 and relative_difference(p0, pc) > 1e-3
 
 
-@pytest.mark.parametrize("case", ["smoke", "cu111_dailight_short_sym", "cu111_dailight_oblique"])
+@pytest.mark.parametrize("case", ["smoke", "generic", "cu111_dailight_short_sym", "cu111_dailight_oblique"])
 def test_analysis_step_matches_oracle(case):
     print(f"case {case} seed 11 analysis step", flush=True)
     gpu, cpu = _pair(case)
@@ -74,3 +75,20 @@ def test_analysis_step_matches_oracle(case):
     iz = gpu.info.iz_analysis
     assert relative_difference(cpu.z_plane(iz), gpu.z_plane(iz)) < TOL
     assert relative_difference(cpu.site_column(3), gpu.site_column(3)) < TOL
+
+
+@pytest.mark.parametrize("case", ["smoke", "generic", "cu111_dailight_short_sym", "cu111_dailight_oblique"])
+def test_mixed_precision_t_block_against_fp64_oracle(case):
+    """The complex64 storage mode against the FP64 oracle on one full T block from the same wave packet:
+    single-precision rounding only (max |a - b| / max |a| < 1e-5)."""
+    print(f"case {case} seed 21 mixed precision", flush=True)
+    gpu, cpu = _pair(case, {"Precision": "mixed"})
+    _randomise(gpu, cpu, 21)
+    gpu.kinetic_and_potential(False, False)
+    cpu.kinetic_and_potential(False, False)
+    sg, pg = gpu.get_wave()
+    sc, pc = cpu.get_wave()
+    ds, dp = relative_difference(sc, sg), relative_difference(pc, pg)
+    print(f"  scattering {ds:.3e} specular {dp:.3e}", flush=True)
+    assert ds < 1e-5 and dp < 1e-5
+    assert ds > 1e-9  # the mode really stores complex64

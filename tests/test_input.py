@@ -1,0 +1,34 @@
+"""Input validation of the set-up the backend is fed from (src/input/validate.cpp:48-66 of the reference): grid
+lengths that are not FFT sizes, nX /= nY under a square or hexagonal point group, and unsupported options are
+refused before any device work, with the key named in the message."""
+
+import pytest
+
+import spodvr_amd
+from tests import cases
+from oracle import oracle
+
+
+@pytest.mark.parametrize("key,value", [("nZ", 17), ("nZSpecular", 130), ("nR", 13), ("nX", 9), ("nY", 1)])
+def test_invalid_fft_sizes_are_refused(key, value):
+    params = dict(cases.SMOKE_DAILIGHT, **{key: value})
+    with pytest.raises(spodvr_amd.SpoError, match=key):
+        oracle.run(cases.write_case("invalid_" + key, params))
+
+
+def test_unequal_xy_grids_are_refused_for_c6v():
+    params = dict(cases.SMOKE_DAILIGHT, nX=4, nY=8)
+    with pytest.raises(spodvr_amd.SpoError, match="nY"):
+        oracle.run(cases.write_case("invalid_nxny", params))
+
+
+def test_projection_operator_other_than_2_is_not_supported():
+    params = dict(cases.SMOKE_DAILIGHT, ProjOp=1)
+    with pytest.raises(spodvr_amd.SpoError):
+        oracle.run(cases.write_case("invalid_projop", params))
+
+
+def test_unknown_precision_is_refused():
+    params = dict(cases.SMOKE_DAILIGHT, Precision="fp16")
+    with pytest.raises(spodvr_amd.SpoError, match="Precision"):
+        oracle.run(cases.write_case("invalid_precision", params))
