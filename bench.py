@@ -12,7 +12,10 @@ Printed: one JSON line with the throughput, the roofline of the dominant kernel 
 its mean HIP-event duration in the timed region) and the CPU baseline (the FP64 OpenMP oracle on the same grid,
 timed on this host over a bounded number of time steps).
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload dz|hardwall] [--precision fp64|mixed]
+By default both storage precisions run on the same workload: `value` is the mixed-precision throughput (complex64
+wave, FP64 phases, analysis and sums), and the FP64 run is reported beside it with the ratio of the two.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload dz|hardwall] [--precision both|fp64|mixed]
 """
 
 from __future__ import annotations
@@ -84,34 +87,16 @@ def cpu_baseline(path: str, sample_steps: int, wave) -> dict:
                       f"({dt:.2f} s per step, FP64 OpenMP oracle, {cores} threads)"}
 
 
-def main() -> int:
-    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=50, help="time steps timed (rounded up to whole analysis steps)")
-    ap.add_argument("--warmup", type=int, default=10, help="untimed time steps before the timed region")
-    ap.add_argument("--workload", default="dz", choices=sorted(WORKLOADS))
-    ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
-    ap.add_argument("--cpu-steps", type=int, default=1, help="time steps of the CPU baseline sample")
-    ap.add_argument("--no-cpu-baseline", action="store_true")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for {args.gpus}); the propagation of one run is not sharded "
-              "across GPUs in this build", file=sys.stderr)
-        return 2
-
+def measure(path: str, precision: str, steps: int, warmup: int) -> dict:
+    """One precision on the GPU: warm-up analysis steps, then the timed analysis steps with per-kernel HIP events.
+    Every seam call returns with the device idle (the analysis step ends in a synchronous copy of the BK
+    coefficients), so the wall clock brackets exactly the timed work."""
     import spodvr_amd
-    from tests import cases
-
-    name, description = WORKLOADS[args.workload]
-    params = dict(getattr(cases, name))
-    params["Precision"] = args.precision
-    path = cases.write_case(f"bench_{args.workload}_{args.precision}", params)
-    run = spodvr_amd.Run(path, device="hip")
+    run = spodvr_amd.Run(path, {"Precision": precision}, device="hip")
     info = run.info
     spa = info.steps_per_analysis
-    timed_analysis = max(1, -(-args.steps // spa))
-    warm_analysis = max(1, -(-args.warmup // spa)) if args.warmup > 0 else 0
-    steps = timed_analysis * spa
+    timed_analysis = max(1, -(-steps // spa))
+    warm_analysis = max(1, -(-warmup // spa)) if warmup > 0 else 0
     wave = synthetic_wave(run)
     run.set_wave(*wave)
     if warm_analysis:
@@ -122,48 +107,82 @@ def main() -> int:
     elapsed = time.perf_counter() - t0
     kernels = run.kernel_times()
     run.kernel_timing(False)
-
-    points = dvr_points(info)
-    value = points * steps / elapsed
-    complex_bytes = 16 if args.precision == "fp64" else 8
+    run.close()
+    n_steps = timed_analysis * spa
+    complex_bytes = 16 if precision == "fp64" else 8
     dominant = max(kernels.items(), key=lambda kv: kv[1][1]) if kernels else None
     roofline = None
     if dominant:
         kname, (launches, total_ms) = dominant
-        avg_s = total_ms / launches / 1e3
         alg = algorithmic_bytes(kname, info, complex_bytes)
         if alg is not None:
-            achieved = alg / avg_s / 1e9
-            roofline = {"bound": "hbm", "kernel": kname, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                        "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            achieved = alg / (total_ms / launches / 1e3) / 1e9
+            roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": kname,
                         "algorithmic_bytes_per_launch": alg, "mean_launch_ms": round(total_ms / launches, 4),
                         "launches": launches}
-    kernel_summary = {k: {"launches": n, "ms_per_time_step": round(ms / steps, 4)} for k, (n, ms) in kernels.items()}
+    step_bytes = sum(algorithmic_bytes(k, info, complex_bytes) or 0.0 for k in ("zr_plane", "xy_tile", "interaction"))
+    return {"info": info, "wave": wave, "steps": n_steps, "warmup": warm_analysis * spa,
+            "value": dvr_points(info) * n_steps / elapsed, "ms_per_step": elapsed / n_steps * 1e3,
+            "roofline": roofline, "step_roofline_frac": round(step_bytes / (elapsed / n_steps) / 1e9 / HBM_PEAK_GBS, 4),
+            "kernels": {k: {"launches": n, "ms_per_time_step": round(ms / n_steps, 4)} for k, (n, ms) in kernels.items()}}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100, help="time steps timed (rounded up to whole analysis steps)")
+    ap.add_argument("--warmup", type=int, default=10, help="untimed time steps before the timed region")
+    ap.add_argument("--workload", default="dz", choices=sorted(WORKLOADS))
+    ap.add_argument("--precision", default="both", choices=["both", "fp64", "mixed"],
+                    help="both: the mixed-precision line with the FP64 run beside it")
+    ap.add_argument("--cpu-steps", type=int, default=10, help="time steps of the CPU baseline sample")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}); the propagation of one run is not sharded "
+              "across GPUs in this build", file=sys.stderr)
+        return 2
+
+    from tests import cases
+
+    name, description = WORKLOADS[args.workload]
+    path = cases.write_case(f"bench_{args.workload}", dict(getattr(cases, name)))
+    primary = "mixed" if args.precision in ("both", "mixed") else "fp64"
+    fp64 = measure(path, "fp64", args.steps, args.warmup) if args.precision in ("both", "fp64") else None
+    mixed = measure(path, "mixed", args.steps, args.warmup) if args.precision in ("both", "mixed") else None
+    main_run = mixed if primary == "mixed" else fp64
+    info = main_run["info"]
 
     cpu = None
     if not args.no_cpu_baseline:
-        run.close()
-        cpu = cpu_baseline(path, args.cpu_steps, wave)
+        cpu = cpu_baseline(path, args.cpu_steps, main_run["wave"])
+
+    def brief(m):
+        return None if m is None else {"value": m["value"], "ms_per_step": m["ms_per_step"], "roofline": m["roofline"],
+                                        "step_roofline_frac": m["step_roofline_frac"], "kernels": m["kernels"]}
 
     line = {
         "metric": METRIC,
-        "value": value,
+        "value": main_run["value"],
         "unit": "grid-point-steps/s",
         "n_gpus": 1,
-        "steps": steps,
-        "warmup": warm_analysis * spa,
-        "ms_per_step": elapsed / steps * 1e3,
+        "steps": main_run["steps"],
+        "warmup": main_run["warmup"],
+        "ms_per_step": main_run["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if args.precision == "fp64" else "c64 storage / f64 phases and analysis",
+        "dtype": "f32 (complex64 wave, FP64 phases and analysis)" if primary == "mixed" else "f64",
         "data": "synthetic",
-        "config": {"workload": description, "precision": args.precision, "dvr_points": points,
+        "config": {"workload": description, "precision": primary, "dvr_points": dvr_points(info),
                    "psi_values": info.n_rot * info.nx * info.ny * info.nr * info.nz,
-                   "steps_per_analysis": spa, "n_rot": info.n_rot, "nz_potential": info.nz_potential,
-                   "n_representatives": info.n_representatives},
-        "roofline": roofline,
-        "kernels": kernel_summary,
+                   "steps_per_analysis": info.steps_per_analysis, "n_rot": info.n_rot,
+                   "nz_potential": info.nz_potential, "n_representatives": info.n_representatives},
+        "roofline": main_run["roofline"],
+        "mixed_over_fp64": (mixed["value"] / fp64["value"]) if (mixed and fp64) else None,
+        "fp64": brief(fp64),
+        "mixed": brief(mixed),
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
