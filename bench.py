@@ -57,6 +57,37 @@ def algorithmic_bytes(kernel: str, info, complex_bytes: int) -> float | None:
     return None
 
 
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic.txt")
+
+
+def measured_traffic(kernel: str, workload: str, precision: str) -> float | None:
+    """HBM bytes per launch of a kernel from the rocprofv3 FETCH_SIZE and WRITE_SIZE passes summarised in
+    profiles/hbm_traffic.txt (each counter scaled by what it reads for a device copy of known size); None when the
+    kernel, workload and precision have not been profiled."""
+    if not os.path.exists(TRAFFIC_FILE):
+        return None
+    for line in open(TRAFFIC_FILE):
+        parts = line.split()
+        if len(parts) == 5 and not line.startswith("#") and parts[:3] == [kernel, workload, precision]:
+            return float(parts[3]) + float(parts[4])
+    return None
+
+
+def copy_calibration(n_bytes: int, reps: int = 10) -> None:
+    """Device-to-device copies of n_bytes through torch (the reference copy kernel whose FETCH_SIZE and
+    WRITE_SIZE calibrate the counters: each dispatch reads and writes exactly n_bytes)."""
+    import torch
+    a = torch.ones(n_bytes // 4, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / reps
+    print(json.dumps({"copy_bytes": n_bytes, "reps": reps, "ms": dt * 1e3, "GB/s": 2 * n_bytes / dt / 1e9}), flush=True)
+
+
 def synthetic_wave(run, seed: int = 2024):
     import numpy as np
     rng = np.random.default_rng(seed)
@@ -87,7 +118,7 @@ def cpu_baseline(path: str, sample_steps: int, wave) -> dict:
                       f"({dt:.2f} s per step, FP64 OpenMP oracle, {cores} threads)"}
 
 
-def measure(path: str, precision: str, steps: int, warmup: int) -> dict:
+def measure(path: str, workload: str, precision: str, steps: int, warmup: int) -> dict:
     """One precision on the GPU: warm-up analysis steps, then the timed analysis steps with per-kernel HIP events.
     Every seam call returns with the device idle (the analysis step ends in a synchronous copy of the BK
     coefficients), so the wall clock brackets exactly the timed work."""
@@ -118,7 +149,8 @@ def measure(path: str, precision: str, steps: int, warmup: int) -> dict:
         if alg is not None:
             achieved = alg / (total_ms / launches / 1e3) / 1e9
             roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                        "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": kname,
+                        "frac": round(achieved / HBM_PEAK_GBS, 4),
+                        "traffic": measured_traffic(kname, workload, precision), "kernel": kname,
                         "algorithmic_bytes_per_launch": alg, "mean_launch_ms": round(total_ms / launches, 4),
                         "launches": launches}
     step_bytes = sum(algorithmic_bytes(k, info, complex_bytes) or 0.0 for k in ("zr_plane", "xy_tile", "interaction"))
@@ -138,7 +170,12 @@ def main() -> int:
                     help="both: the mixed-precision line with the FP64 run beside it")
     ap.add_argument("--cpu-steps", type=int, default=10, help="time steps of the CPU baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                    help="only run device copies of BYTES (the counter calibration of the traffic figures)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return 0
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the propagation of one run is not sharded "
               "across GPUs in this build", file=sys.stderr)
@@ -149,8 +186,8 @@ def main() -> int:
     name, description = WORKLOADS[args.workload]
     path = cases.write_case(f"bench_{args.workload}", dict(getattr(cases, name)))
     primary = "mixed" if args.precision in ("both", "mixed") else "fp64"
-    fp64 = measure(path, "fp64", args.steps, args.warmup) if args.precision in ("both", "fp64") else None
-    mixed = measure(path, "mixed", args.steps, args.warmup) if args.precision in ("both", "mixed") else None
+    fp64 = measure(path, args.workload, "fp64", args.steps, args.warmup) if args.precision in ("both", "fp64") else None
+    mixed = measure(path, args.workload, "mixed", args.steps, args.warmup) if args.precision in ("both", "mixed") else None
     main_run = mixed if primary == "mixed" else fp64
     info = main_run["info"]
 

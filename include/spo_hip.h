@@ -87,38 +87,55 @@ typedef struct spo_hip_analysis {
   const double* total_energy_hartree; /* per energy */
 } spo_hip_analysis;
 
+/* The message of the last failed call on this thread (the reference throws CudaError / NotSupportedError with
+ * the message, src/cuda/cuda_error.hpp:10-13). */
 const char* spo_hip_last_error(void);
 
-/* Number of HIP devices visible (0 on a machine without a GPU; never an error). */
+/* Number of HIP devices visible (0 on a machine without a GPU; never an error). Replaces the device probe of
+ * the reference's backend factory (src/app/backend_factory.cpp:22-29, :52). */
 int spo_hip_device_count(int* count);
 
-/* Uploads the tables and the initial specular wave (ir, iz); the scattering wave starts at zero. */
+/* Uploads the tables and the initial specular wave (ir, iz); the scattering wave starts at zero. Replaces the
+ * BackendFactory call for a GPU device (src/propagation/propagation_backend.hpp:73-74; the CUDA backend's
+ * constructor, src/cuda/cuda_backend.cpp:93). tables->exp_potential is exp(i A) with A = -V dt, built by the
+ * caller in FP64 from the potential grid in hartree (the reference premultiplies by -dt inside its backends,
+ * src/propagation/cpu_backend.cpp:17-28). */
 int spo_hip_create(const spo_hip_shape* shape, const spo_hip_tables* tables, const spo_hip_analysis* analysis,
                    const double* initial_specular, spo_hip_backend** out);
+/* Frees the backend (the reference backend's destructor; RAII buffers, src/cuda/device_buffer.hpp:25-53). */
 int spo_hip_destroy(spo_hip_backend* backend);
 
-/* Full wave packet, host layouts (replaces read_wave_packet / wave_packet_on_host). */
+/* Full wave packet, host layouts: upload replaces read_wave_packet (src/propagation/propagation_backend.hpp:39-42),
+ * download replaces wave_packet_on_host (:43-45). */
 int spo_hip_upload_wave(spo_hip_backend* backend, const double* scattering, const double* specular);
 int spo_hip_download_wave(spo_hip_backend* backend, double* scattering, double* specular);
 
-/* T(dt/2) V, (T V) x (n-1), T(dt/2) (src/propagation/split_operator_schedule.cpp:7-13). */
+/* T(dt/2) V, (T V) x (n-1), T(dt/2) (src/propagation/split_operator_schedule.cpp:7-13). Replaces
+ * propagate_analysis_step (src/propagation/propagation_backend.hpp:47-48). */
 int spo_hip_propagate_analysis_step(spo_hip_backend* backend);
-/* One T block: specular grid, T_Zr, T_xy, then V unless kinetic_only (src/propagation/split_operator.cpp:17-21). */
+/* One T block: specular grid, T_Zr, T_xy, then V unless kinetic_only (src/propagation/split_operator.cpp:17-21);
+ * the unit the reference's CUDA tests check stage by stage (tests/cuda/test_cuda_backend.cpp:94-106). */
 int spo_hip_kinetic_and_potential(spo_hip_backend* backend, int half_step, int kinetic_only);
 
+/* The specular wave (ir, iz) after the last step; replaces specular() (src/propagation/propagation_backend.hpp:50). */
 int spo_hip_get_specular(spo_hip_backend* backend, double* specular);
-/* nr blocks of Z index iz, layout (irot, ixy, ir). */
+/* nr blocks of Z index iz, layout (irot, ixy, ir); replaces z_plane(iz) (src/propagation/propagation_backend.hpp:51-52). */
 int spo_hip_get_z_plane(spo_hip_backend* backend, int iz, double* out);
-/* sum |psi|^2 over (irot, ixy) per izr, n_zr values. */
+/* sum |psi|^2 over (irot, ixy) per izr, n_zr values; replaces block_densities()
+ * (src/propagation/propagation_backend.hpp:53-55). */
 int spo_hip_block_densities(spo_hip_backend* backend, double* out);
-/* One (x, y) point, n_zr * n_rot complex values, layout (izr, irot). */
+/* One (x, y) point, n_zr * n_rot complex values, layout (izr, irot); replaces site_column(ixy, out)
+ * (src/propagation/propagation_backend.hpp:56-58). */
 int spo_hip_site_column(spo_hip_backend* backend, int ixy, double* out);
 
 /* Balint-Kurti coefficients at the analysis line; with surface_fixed the initial state's from the specular wave
  * at iz_specular. Then sums(is, v, ien) += c(is, v) exp(i E t) dtAna in FP64. coefficients (n_states * n_v
- * complex) may be NULL. */
+ * complex) may be NULL. Replaces BalintKurtiAnalyzer::coefficients on the host copy of z_plane
+ * (src/analysis/balint_kurti.cpp:39-84) and SMatrixAccumulator::add (src/analysis/s_matrix.cpp:49-56): the
+ * plane stays on the device and only the coefficients cross PCIe. */
 int spo_hip_analyse(spo_hip_backend* backend, int surface_fixed, int iz_specular, double t, double* coefficients);
-/* The sums, (is, v, ien) with is fastest. */
+/* The sums, (is, v, ien) with is fastest: the input of SMatrixAccumulator::result_at and combine_s_matrix
+ * (src/analysis/s_matrix.cpp:70-176), which stay on the host in FP64. */
 int spo_hip_get_s_sums(spo_hip_backend* backend, double* out);
 
 /* Kernel timing with HIP events recorded on the backend's stream around every launch (no counterpart in the
