@@ -11,56 +11,212 @@
 
 namespace spo_dev {
 
-// T_Zr: one (irot, ixy) plane of NZ x NR points per workgroup, in LDS as rows of NR + 1: r FFTs (rows), Z FFTs
-// (columns), x exp(-i (T_Z + T_r) dt) with the FFT scales folded in, back. One read and one write of psi.
-template <int NZ, int NR>
-constexpr int zr_lds_elements() {
-  return NZ * (NR + 1);
+// T_Zr with register-resident Stockham passes: NR = A1 * A2 and NZ = B1 * B2, two radix passes per axis, each pass
+// a set of whole butterflies per thread with one LDS exchange between passes. The first r pass reads its
+// butterflies straight from HBM and the last inverse r pass writes straight to HBM; the last forward Z pass, the
+// factor exp(-i (T_Z + T_r) dt) (FFT scales folded in) and the first inverse Z pass run on the same registers
+// (the inverse runs its radices in reverse order, so its first pass takes exactly the elements the forward pass
+// left in each thread). Seven passes, six LDS round trips, versus a tile in and out of LDS around every stage.
+//
+// Stockham positions for N = R1 * R2: pass 1 (radix R1) takes x[j + r R2] (j < R2) and writes j R1 + r; pass 2
+// (radix R2) takes j + r R1 (j < R1), twiddles by W_N^{r j} and writes back to the same places, in natural order.
+// Plane in LDS: row iz at iz * (NR + 1). The HBM passes (1 and 7) run lanes along j so that 16 lanes touch
+// consecutive addresses; with A1 odd, pass 1's LDS writes (stride A1) also spread over the banks. Passes 2 and 6
+// run lanes along iz (stride NR + 1, odd) and the Z passes along kr (stride 1).
+template <int NBUT, int THREADS>
+constexpr int per_thread() {
+  return (NBUT + THREADS - 1) / THREADS;
 }
 
-template <typename T, int NZ, int NR, int THREADS>
-__global__ __launch_bounds__(THREADS) void zr_ct_kernel(cx<T>* psi, const cx<T>* table_z, const cx<T>* table_r,
-                                                        const cx<T>* factor) {
+template <typename T, int R>
+__device__ __forceinline__ void twiddle(cx<T>* v, const cx<T>* table, int j, int sign) {
+  if (j == 0) return;
+#pragma unroll
+  for (int r = 1; r < R; ++r) {
+    cx<T> w = table[r * j];
+    if (sign > 0) w = conj(w);
+    v[r] = v[r] * w;
+  }
+}
+
+template <typename T, int NZ, int NR, int A1, int A2, int B1, int B2, int THREADS>
+__global__ __launch_bounds__(THREADS) void zr_reg_kernel(cx<T>* psi, const cx<T>* table_z, const cx<T>* table_r,
+                                                         const cx<T>* factor) {
+  static_assert(A1 * A2 == NR && B1 * B2 == NZ, "zr_reg_kernel: radices must multiply to the lengths");
   extern __shared__ unsigned char smem_raw[];
-  cx<T>* tile = reinterpret_cast<cx<T>*>(smem_raw);
+  cx<T>* t = reinterpret_cast<cx<T>*>(smem_raw);
   constexpr int P = NZ * NR;
   constexpr int LD = NR + 1;
   cx<T>* plane = psi + static_cast<long>(blockIdx.x) * P;
-#pragma unroll 4
-  for (int e = threadIdx.x; e < P; e += THREADS) tile[(e / NR) * LD + e % NR] = plane[e];
-  __syncthreads();
-  using Rows = CtLayout<0, LD, 0, 1>;  // one r sequence per iz
-  using Cols = CtLayout<0, 1, 0, LD>;  // one Z sequence per ir, stride NR + 1
-  ct_fft<T, NR, THREADS, Rows>(tile, NZ, table_r, -1);
-  ct_fft<T, NZ, THREADS, Cols>(tile, NR, table_z, -1);
-  for (int e = threadIdx.x; e < P; e += THREADS) {
-    cx<T>& v = tile[(e / NR) * LD + e % NR];
-    v = factor[e] * v;
+  const int tid = threadIdx.x;
+
+  // Pass 1, r forward, radix A1: butterfly (iz, j < A2) from HBM.
+  {
+    constexpr int N = NZ * A2, Q = per_thread<N, THREADS>();
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      const int b = tid + q * THREADS;
+      if (N % THREADS != 0 && b >= N) break;
+      const int j = b % A2, iz = b / A2;
+      cx<T> v[A1];
+#pragma unroll
+      for (int r = 0; r < A1; ++r) v[r] = plane[iz * NR + j + r * A2];
+      rdft<A1, T>(v, -1);
+#pragma unroll
+      for (int r = 0; r < A1; ++r) t[iz * LD + j * A1 + r] = v[r];
+    }
   }
   __syncthreads();
-  ct_fft<T, NZ, THREADS, Cols>(tile, NR, table_z, +1);
-  ct_fft<T, NR, THREADS, Rows>(tile, NZ, table_r, +1);
-#pragma unroll 4
-  for (int e = threadIdx.x; e < P; e += THREADS) plane[e] = tile[(e / NR) * LD + e % NR];
+  // Pass 2, r forward, radix A2, in place: butterfly (iz, j < A1), lanes along iz (row stride NR + 1).
+  {
+    constexpr int N = NZ * A1, Q = per_thread<N, THREADS>();
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      const int b = tid + q * THREADS;
+      if (N % THREADS != 0 && b >= N) break;
+      const int iz = b % NZ, j = b / NZ;
+      cx<T> v[A2];
+#pragma unroll
+      for (int r = 0; r < A2; ++r) v[r] = t[iz * LD + j + r * A1];
+      twiddle<T, A2>(v, table_r, j, -1);
+      rdft<A2, T>(v, -1);
+#pragma unroll
+      for (int r = 0; r < A2; ++r) t[iz * LD + j + r * A1] = v[r];
+    }
+  }
+  __syncthreads();
+  // Pass 3, Z forward, radix B1: butterfly (kr, j < B2); reads all, then writes all.
+  {
+    constexpr int N = NR * B2, Q = per_thread<N, THREADS>();
+    cx<T> v[Q][B1];
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      const int b = tid + q * THREADS;
+      if (N % THREADS != 0 && b >= N) break;
+      const int kr = b % NR, j = b / NR;
+#pragma unroll
+      for (int r = 0; r < B1; ++r) v[q][r] = t[(j + r * B2) * LD + kr];
+      rdft<B1, T>(v[q], -1);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      const int b = tid + q * THREADS;
+      if (N % THREADS != 0 && b >= N) break;
+      const int kr = b % NR, j = b / NR;
+#pragma unroll
+      for (int r = 0; r < B1; ++r) t[(j * B1 + r) * LD + kr] = v[q][r];
+    }
+  }
+  __syncthreads();
+  // Pass 4, Z forward radix B2 (twiddled), x factor, Z inverse radix B2: butterfly (kr, j < B1).
+  {
+    constexpr int N = NR * B1, Q = per_thread<N, THREADS>();
+    cx<T> v[Q][B2];
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      const int b = tid + q * THREADS;
+      if (N % THREADS != 0 && b >= N) break;
+      const int kr = b % NR, j = b / NR;
+#pragma unroll
+      for (int r = 0; r < B2; ++r) v[q][r] = t[(j + r * B1) * LD + kr];
+      twiddle<T, B2>(v[q], table_z, j, -1);
+      rdft<B2, T>(v[q], -1);
+#pragma unroll
+      for (int r = 0; r < B2; ++r) v[q][r] = factor[(j + r * B1) * NR + kr] * v[q][r];
+      rdft<B2, T>(v[q], +1);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      const int b = tid + q * THREADS;
+      if (N % THREADS != 0 && b >= N) break;
+      const int kr = b % NR, j = b / NR;
+#pragma unroll
+      for (int r = 0; r < B2; ++r) t[(j * B2 + r) * LD + kr] = v[q][r];
+    }
+  }
+  __syncthreads();
+  // Pass 5, Z inverse, radix B1, in place: butterfly (kr, j < B2).
+  {
+    constexpr int N = NR * B2, Q = per_thread<N, THREADS>();
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      const int b = tid + q * THREADS;
+      if (N % THREADS != 0 && b >= N) break;
+      const int kr = b % NR, j = b / NR;
+      cx<T> v[B1];
+#pragma unroll
+      for (int r = 0; r < B1; ++r) v[r] = t[(j + r * B2) * LD + kr];
+      twiddle<T, B1>(v, table_z, j, +1);
+      rdft<B1, T>(v, +1);
+#pragma unroll
+      for (int r = 0; r < B1; ++r) t[(j + r * B2) * LD + kr] = v[r];
+    }
+  }
+  __syncthreads();
+  // Pass 6, r inverse, radix A2: butterfly (iz, j < A1), lanes along iz; reads all, then writes all.
+  {
+    constexpr int N = NZ * A1, Q = per_thread<N, THREADS>();
+    cx<T> v[Q][A2];
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      const int b = tid + q * THREADS;
+      if (N % THREADS != 0 && b >= N) break;
+      const int iz = b % NZ, j = b / NZ;
+#pragma unroll
+      for (int r = 0; r < A2; ++r) v[q][r] = t[iz * LD + j + r * A1];
+      rdft<A2, T>(v[q], +1);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      const int b = tid + q * THREADS;
+      if (N % THREADS != 0 && b >= N) break;
+      const int iz = b % NZ, j = b / NZ;
+#pragma unroll
+      for (int r = 0; r < A2; ++r) t[iz * LD + j * A2 + r] = v[q][r];
+    }
+  }
+  __syncthreads();
+  // Pass 7, r inverse, radix A1 (twiddled): butterfly (iz, j < A2) to HBM.
+  {
+    constexpr int N = NZ * A2, Q = per_thread<N, THREADS>();
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      const int b = tid + q * THREADS;
+      if (N % THREADS != 0 && b >= N) break;
+      const int j = b % A2, iz = b / A2;
+      cx<T> v[A1];
+#pragma unroll
+      for (int r = 0; r < A1; ++r) v[r] = t[iz * LD + j + r * A2];
+      twiddle<T, A1>(v, table_r, j, +1);
+      rdft<A1, T>(v, +1);
+#pragma unroll
+      for (int r = 0; r < A1; ++r) plane[iz * NR + j + r * A2] = v[r];
+    }
+  }
 }
 
-// T_xy (+ the diagonal part of V): C consecutive izr of one irot, all NX x NY points. LDS element (c, iy, ix) at
-// ix * XS + iy * (C + 1) + c with XS = NY (C + 1) + 1 (both strides odd for even C).
-template <int NX, int NY, int C>
-constexpr int xy_lds_elements() {
-  return NX * (NY * (C + 1) + 1);
+// T_xy (+ the diagonal part of V) for grids whose NX and NY are single radices (whole-line DFTs in registers):
+// C consecutive izr of one irot, C * NX threads. Thread (c, ix) loads its y line of NY values straight from HBM,
+// transforms it and parks it in LDS; thread (c, iy) then takes an x line, transforms it forward, multiplies the
+// T_xy factor, transforms it back and parks it again; thread (c, ix) finishes its y line and stores it to HBM with
+// the shift and the potential factors applied. Two LDS round trips, two barriers. LDS element (c, iy, ix) at
+// ix * XS + iy * (C + 1) + c, XS = NY (C + 1) + 1.
+template <int N>
+constexpr bool single_radix() {
+  return is_radix(N);
 }
 
-template <typename T, int NX, int NY, int C, int THREADS>
-__global__ __launch_bounds__(THREADS) void xy_ct_kernel(cx<T>* psi, Shape s, const cx<T>* table_x,
-                                                        const cx<T>* table_y, const cx<T>* factor,
-                                                        const cx<T>* shift_f, const cx<T>* shift_b, int needs_shift,
-                                                        int apply_potential, const T* z_opt, const cx<T>* before,
-                                                        const cx<T>* gas_step) {
+template <typename T, int NX, int NY, int C>
+__global__ __launch_bounds__(C * NX) void xy_reg_kernel(cx<T>* psi, Shape s, const cx<T>* factor,
+                                                         const cx<T>* shift_f, const cx<T>* shift_b, int needs_shift,
+                                                         int apply_potential, const T* z_opt, const cx<T>* before,
+                                                         const cx<T>* gas_step) {
+  static_assert(single_radix<NX>() && single_radix<NY>() && NX == NY, "xy_reg_kernel: one radix per axis, NX = NY");
   extern __shared__ unsigned char smem_raw[];
   cx<T>* tile = reinterpret_cast<cx<T>*>(smem_raw);
-  constexpr int NXY = NX * NY;
-  constexpr int TOTAL = C * NXY;
   constexpr int YS = C + 1;
   constexpr int XS = NY * YS + 1;
   const int tiles_per_rot = (s.n_zr + C - 1) / C;
@@ -69,39 +225,59 @@ __global__ __launch_bounds__(THREADS) void xy_ct_kernel(cx<T>* psi, Shape s, con
   const int cw = min(C, s.n_zr - izr0);
   const long rot_off = static_cast<long>(s.n_zr) * irot;
   const long xy_stride = static_cast<long>(s.n_zr) * s.n_rot;
-  for (int e = threadIdx.x; e < TOTAL; e += THREADS) {
-    const int c = e % C, ixy = e / C;
-    cx<T> v = {T(0), T(0)};
-    if (c < cw) v = psi[izr0 + c + rot_off + xy_stride * ixy];
-    if (needs_shift) v = v * shift_f[ixy];
-    tile[(ixy / NY) * XS + (ixy % NY) * YS + c] = v;
+  const int c = threadIdx.x % C;
+  const int a = threadIdx.x / C;  // ix in the y phases, iy in the x phase
+  const bool live = c < cw;
+  cx<T>* g = psi + izr0 + c + rot_off;
+
+  cx<T> v[NY];
+#pragma unroll
+  for (int iy = 0; iy < NY; ++iy) {
+    const int ixy = a * NY + iy;
+    v[iy] = live ? g[xy_stride * ixy] : cx<T>{T(0), T(0)};
+    if (needs_shift) v[iy] = v[iy] * shift_f[ixy];
   }
+  rdft<NY, T>(v, -1);
+#pragma unroll
+  for (int iy = 0; iy < NY; ++iy) tile[a * XS + iy * YS + c] = v[iy];
   __syncthreads();
-  using Ys = CtLayout<C, 1, XS, YS>;  // for each (c, ix): iy at stride C + 1
-  using Xs = CtLayout<C, 1, YS, XS>;  // for each (c, iy): ix at stride XS
-  ct_fft<T, NY, THREADS, Ys>(tile, C * NX, table_y, -1);
-  ct_fft<T, NX, THREADS, Xs>(tile, C * NY, table_x, -1);
-  for (int e = threadIdx.x; e < TOTAL; e += THREADS) {
-    const int c = e % C, ixy = e / C;
-    cx<T>& v = tile[(ixy / NY) * XS + (ixy % NY) * YS + c];
-    v = factor[ixy] * v;
-  }
+
+  cx<T> w[NX];
+#pragma unroll
+  for (int ix = 0; ix < NX; ++ix) w[ix] = tile[ix * XS + a * YS + c];
+  rdft<NX, T>(w, -1);
+#pragma unroll
+  for (int ix = 0; ix < NX; ++ix) w[ix] = factor[ix * NY + a] * w[ix];
+  rdft<NX, T>(w, +1);
+#pragma unroll
+  for (int ix = 0; ix < NX; ++ix) tile[ix * XS + a * YS + c] = w[ix];
   __syncthreads();
-  ct_fft<T, NX, THREADS, Xs>(tile, C * NY, table_x, +1);
-  ct_fft<T, NY, THREADS, Ys>(tile, C * NX, table_y, +1);
-  for (int e = threadIdx.x; e < TOTAL; e += THREADS) {
-    const int c = e % C, ixy = e / C;
-    if (c >= cw) continue;
-    cx<T> v = tile[(ixy / NY) * XS + (ixy % NY) * YS + c];
-    if (needs_shift) v = v * shift_b[ixy];
-    const int izr = izr0 + c;
-    if (apply_potential) {
-      const int iz = izr / s.nr, ir = izr % s.nr;
-      const cx<T> f = iz >= s.nzpot ? gas_step[irot + s.n_rot * ir] : before[irot + s.n_rot * ir];
-      v = scale(f * v, z_opt[iz]);
-    }
-    psi[izr + rot_off + xy_stride * ixy] = v;
+
+#pragma unroll
+  for (int iy = 0; iy < NY; ++iy) v[iy] = tile[a * XS + iy * YS + c];
+  rdft<NY, T>(v, +1);
+  if (!live) return;
+  const int izr = izr0 + c;
+  const int iz = izr / s.nr, ir = izr % s.nr;
+  cx<T> f = {T(1), T(0)};
+  T zo = T(1);
+  if (apply_potential) {
+    f = iz >= s.nzpot ? gas_step[irot + s.n_rot * ir] : before[irot + s.n_rot * ir];
+    zo = z_opt[iz];
   }
+#pragma unroll
+  for (int iy = 0; iy < NY; ++iy) {
+    const int ixy = a * NY + iy;
+    cx<T> u = v[iy];
+    if (needs_shift) u = u * shift_b[ixy];
+    if (apply_potential) u = scale(f * u, zo);
+    g[xy_stride * ixy] = u;
+  }
+}
+
+template <int NX, int NY, int C>
+constexpr int xy_reg_lds_elements() {
+  return NX * (NY * (C + 1) + 1);
 }
 
 // V in the interaction region for C consecutive izr of one (x, y) column, phi FFTs of length NPHI.

@@ -1,0 +1,281 @@
+// The interaction-region potential step with the Gauss-Legendre theta contractions on the matrix cores.
+//
+// Per workgroup: C = 8 NT consecutive izr (< nz_potential * nr) of one (x, y) site, WAVES wavefronts.
+//   1. FBR -> DVR, theta part: for every mJ block m, D[theta, col] = sum_J L_m[theta, J] c[J, col] as 16 x 16 x 4
+//      MFMA products. The B operand (4 J x 16 real columns: re and im of 8 complex columns) is loaded straight from
+//      HBM, 16 lanes reading 8 consecutive izr of one irot (128 B in FP64); A is a pre-arranged fragment of the
+//      Legendre matrix, one value per lane. D goes to the DVR tile in LDS at phi slot m (mod n_phi).
+//   2. phi FFTs (sign +1) over every (col, theta) row, x exp(i A[rep(ixy), theta, phi_map(op(ixy), phi)]), phi
+//      FFTs back (sign -1).
+//   3. DVR -> FBR: C'[J, col] = sum_theta L_m[theta, J] D'[theta, col] (A = L_m transposed, B read from the tile),
+//      x rot(dt/2), stored straight to HBM.
+// The initial channel's specular coupling term rot(dt/2) s2sc(ixy) chi(Z, r) is added to its FBR value as it is
+// loaded. Workgroups run tile-major over a site order that groups the sites of one symmetry representative on one
+// XCD (blockIdx mod 8), so the exp(i A) rows of a representative are fetched into one L2.
+//
+// MFMA 16x16x4 operand layouts (lane l of 64): A[i][k] = lane i + 16 k, B[k][j] = lane j + 16 k; result
+// element v of lane l is D[l / 16 + 4 v][l % 16] for f64 and D[4 (l / 16) + v][l % 16] for f32.
+#pragma once
+
+#include "fft_ct.hpp"
+#include "kernels.hpp"
+#include "kernels_ct.hpp"
+
+namespace spo_dev {
+
+template <typename T>
+struct Mfma;
+
+template <>
+struct Mfma<double> {
+  typedef double acc_t __attribute__((ext_vector_type(4)));
+  static __device__ __forceinline__ acc_t fma(double a, double b, acc_t c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int row(int lane, int v) { return lane / 16 + 4 * v; }
+};
+
+template <>
+struct Mfma<float> {
+  typedef float acc_t __attribute__((ext_vector_type(4)));
+  static __device__ __forceinline__ acc_t fma(float a, float b, acc_t c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int row(int lane, int v) { return 4 * (lane / 16) + v; }
+};
+
+// Pre-arranged Legendre fragments (built on the host), 64 values per fragment in lane order:
+//   forward  [fwd_offset[b] + tt * k_fwd(b) + k]: lane (i, kk) holds L_m[theta = 16 tt + i][J = 4 k + kk]
+//   backward [bwd_offset[b] + jt * k_bwd + k]:    lane (i, kk) holds L_m[theta = 4 k + kk][J = 16 jt + i]
+// with i = lane % 16, kk = lane / 16, zero outside the block; k_fwd(b) = ceil(nb / 4), k_bwd = ceil(n_theta / 4).
+struct MfmaTables {
+  const int* fwd_offset;
+  const int* bwd_offset;
+  const int* site_order;  // blockIdx % n_xy -> ixy
+};
+
+// The phi step in three register passes over the DVR rows (NPHI = R1 R2, rows <= ROWS_MAX): radix R1 forward;
+// radix R2 forward, x exp(i A) in natural phi order, radix R2 backward on the same registers; radix R1 backward.
+// Sign +1 towards the DVR, -1 back; W = exp(-2 pi i m / NPHI).
+template <typename T, int NPHI, int R1, int R2, int ROWS_MAX, int THREADS>
+__device__ __forceinline__ void phi_step_fused(cx<T>* dvr, int rows, int n_theta, int cw, const cx<T>* table,
+                                               const int* phi_map, const cx<T>* exp_rows, long exp_stride) {
+  static_assert(R1 * R2 == NPHI, "phi radices");
+  constexpr int LD = NPHI + 1;
+  const int tid = threadIdx.x;
+  {
+    constexpr int Q = per_thread<ROWS_MAX * R2, THREADS>();
+    cx<T> v[Q][R1];
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      const int b = tid + q * THREADS;
+      if (b < rows * R2) {
+        const int j = b % R2, row = b / R2;
+#pragma unroll
+        for (int r = 0; r < R1; ++r) v[q][r] = dvr[row * LD + j + r * R2];
+        rdft<R1, T>(v[q], +1);
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      const int b = tid + q * THREADS;
+      if (b < rows * R2) {
+        const int j = b % R2, row = b / R2;
+#pragma unroll
+        for (int r = 0; r < R1; ++r) dvr[row * LD + j * R1 + r] = v[q][r];
+      }
+    }
+    __syncthreads();
+  }
+  {
+    constexpr int Q = per_thread<ROWS_MAX * R1, THREADS>();
+    cx<T> v[Q][R2];
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      const int b = tid + q * THREADS;
+      if (b < rows * R1) {
+        const int j = b % R1, row = b / R1;
+        const int th = row % n_theta, cc = row / n_theta;
+#pragma unroll
+        for (int r = 0; r < R2; ++r) v[q][r] = dvr[row * LD + j + r * R1];
+        twiddle<T, R2>(v[q], table, j, +1);
+        rdft<R2, T>(v[q], +1);
+        if (cc < cw) {
+          const cx<T>* e = exp_rows + exp_stride * cc + NPHI * th;
+#pragma unroll
+          for (int r = 0; r < R2; ++r) v[q][r] = e[phi_map[j + r * R1]] * v[q][r];
+        }
+        rdft<R2, T>(v[q], -1);
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      const int b = tid + q * THREADS;
+      if (b < rows * R1) {
+        const int j = b % R1, row = b / R1;
+#pragma unroll
+        for (int r = 0; r < R2; ++r) dvr[row * LD + j * R2 + r] = v[q][r];
+      }
+    }
+    __syncthreads();
+  }
+  {
+    constexpr int Q = per_thread<ROWS_MAX * R2, THREADS>();
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      const int b = tid + q * THREADS;
+      if (b < rows * R2) {
+        const int j = b % R2, row = b / R2;
+        cx<T> v[R1];
+#pragma unroll
+        for (int r = 0; r < R1; ++r) v[r] = dvr[row * LD + j + r * R2];
+        twiddle<T, R1>(v, table, j, -1);
+        rdft<R1, T>(v, -1);
+#pragma unroll
+        for (int r = 0; r < R1; ++r) dvr[row * LD + j + r * R2] = v[r];
+      }
+    }
+    __syncthreads();
+  }
+}
+
+template <typename T, int NPHI, int NT, int WAVES, int R1 = 0, int R2 = 0>
+__global__ __launch_bounds__(64 * WAVES) void interaction_mfma_kernel(
+    cx<T>* psi, const cx<double>* spec, Shape s, const cx<T>* table_phi, AngularTables at, MfmaTables mt,
+    const T* frag_fwd, const T* frag_bwd, int max_jt, const cx<T>* exp_table, const cx<T>* rot_half,
+    const cx<T>* s2sc) {
+  using M = Mfma<T>;
+  using acc_t = typename M::acc_t;
+  constexpr int C = 8 * NT;
+  constexpr int THREADS = 64 * WAVES;
+  constexpr int LD = NPHI + 1;
+  extern __shared__ unsigned char smem_raw[];
+  int* phi_map = reinterpret_cast<int*>(smem_raw);
+  cx<T>* dvr = reinterpret_cast<cx<T>*>(smem_raw + ((NPHI * sizeof(int) + 15) / 16) * 16);
+  T* dvr_t = reinterpret_cast<T*>(dvr);
+
+  const int n_theta = s.n_theta;
+  const int tile = blockIdx.x / s.n_xy;
+  const int ixy = mt.site_order[blockIdx.x % s.n_xy];
+  const int izr0 = tile * C;
+  const int cw = min(C, s.n_zr_pot - izr0);
+  const long site = static_cast<long>(s.n_zr) * s.n_rot * ixy;
+  const int tid = threadIdx.x;
+  const int lane = tid % 64;
+  const int wave = tid / 64;
+  const int n = lane % 16;   // B / D column: complex column n / 2 of the N tile, component n % 2
+  const int kk = lane / 16;  // B row within the k step
+  const int rows = C * n_theta;
+
+  const int op = at.op_of[ixy];
+  for (int e = tid; e < NPHI; e += THREADS) phi_map[e] = at.phi_index[e + NPHI * op];
+  // Slots of |mJ| > MJMax carry no basis function.
+  const int n_unused = NPHI - s.n_blocks;
+  for (int e = tid; e < rows * n_unused; e += THREADS) dvr[(e / n_unused) * LD + s.mj_max + 1 + e % n_unused] = {};
+
+  // 1. FBR -> DVR (theta). Task = (mJ block, theta tile, N tile).
+  const int n_tt = (n_theta + 15) / 16;
+  const int n_tasks_f = s.n_blocks * n_tt * NT;
+  for (int task = wave; task < n_tasks_f; task += WAVES) {
+    const int nt = task % NT;
+    const int tt = (task / NT) % n_tt;
+    const int bb = task / (NT * n_tt);
+    const int f0 = at.block_first[bb], nb = at.block_size[bb];
+    const int m = bb - s.mj_max;
+    const int slot = m >= 0 ? m : NPHI + m;
+    const int cc = nt * 8 + n / 2, ri = n % 2;
+    const int izr = izr0 + cc;
+    const bool live = cc < cw;
+    const int k_steps = (nb + 3) / 4;
+    const T* frag = frag_fwd + 64L * (mt.fwd_offset[bb] + tt * k_steps);
+    acc_t acc = {};
+    for (int k = 0; k < k_steps; ++k) {
+      const int j = 4 * k + kk;
+      T b = T(0);
+      if (live && j < nb) {
+        const int irot = f0 + j;
+        b = reinterpret_cast<const T*>(psi)[2 * (izr + site + static_cast<long>(s.n_zr) * irot) + ri];
+        if (irot == s.ichn) {
+          const int ir = izr % s.nr, iz = izr / s.nr;
+          const cx<double> in = spec[ir + s.nr * iz];
+          const cx<T> add = rot_half[s.ichn + s.n_rot * ir] * (s2sc[ixy] * cx<T>{T(in.x), T(in.y)});
+          b = b + (ri ? add.y : add.x);
+        }
+      }
+      acc = M::fma(frag[64 * k + lane], b, acc);
+    }
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int th = 16 * tt + M::row(lane, v);
+      if (th < n_theta) dvr_t[2 * ((cc * n_theta + th) * LD + slot) + ri] = acc[v];
+    }
+  }
+  __syncthreads();
+
+  // 2. phi FFTs, the potential phase, phi FFTs back.
+  const int rep = at.rep_of[ixy];
+  const int n_tp = n_theta * NPHI;
+  if constexpr (R1 > 0) {
+    const long stride = static_cast<long>(n_tp) * s.n_rep;
+    phi_step_fused<T, NPHI, R1, R2, C * 16, THREADS>(dvr, rows, n_theta, cw, table_phi, phi_map,
+                                                     exp_table + n_tp * (rep + static_cast<long>(s.n_rep) * izr0),
+                                                     stride);
+  } else {
+    using Rows = CtLayout<0, LD, 0, 1>;
+    ct_fft<T, NPHI, THREADS, Rows>(dvr, rows, table_phi, +1);
+    for (int e = tid; e < rows * NPHI; e += THREADS) {
+      const int ip = e % NPHI;
+      const int row = e / NPHI;  // cc * n_theta + theta
+      const int th = row % n_theta;
+      const int cc = row / n_theta;
+      if (cc >= cw) continue;
+      cx<T>& v = dvr[row * LD + ip];
+      v = exp_table[phi_map[ip] + NPHI * th + static_cast<long>(n_tp) * (rep + static_cast<long>(s.n_rep) * (izr0 + cc))] * v;
+    }
+    __syncthreads();
+    ct_fft<T, NPHI, THREADS, Rows>(dvr, rows, table_phi, -1);
+  }
+
+  // 3. DVR -> FBR, x rot(dt/2), to HBM. Task = (mJ block, J tile, N tile); max_jt = ceil(largest block / 16).
+  const int k_bwd = (n_theta + 3) / 4;
+  for (int task = wave; task < s.n_blocks * max_jt * NT; task += WAVES) {
+    const int nt = task % NT;
+    const int jt = (task / NT) % max_jt;
+    const int bb = task / (NT * max_jt);
+    const int f0 = at.block_first[bb], nb = at.block_size[bb];
+    if (16 * jt >= nb) continue;  // wave-uniform
+    const int m = bb - s.mj_max;
+    const int slot = m >= 0 ? m : NPHI + m;
+    const int cc = nt * 8 + n / 2, ri = n % 2;
+    const T* frag = frag_bwd + 64L * (mt.bwd_offset[bb] + jt * k_bwd);
+    acc_t acc = {};
+    for (int k = 0; k < k_bwd; ++k) {
+      const int th = 4 * k + kk;
+      const T b = th < n_theta ? dvr_t[2 * ((cc * n_theta + th) * LD + slot) + ri] : T(0);
+      acc = M::fma(frag[64 * k + lane], b, acc);
+    }
+    const int izr = izr0 + cc;
+    const int ir = izr % s.nr;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const T mine = acc[v];
+      const T other = __shfl_xor(mine, 1);
+      const int j = 16 * jt + M::row(lane, v);
+      const cx<T> c = ri ? cx<T>{other, mine} : cx<T>{mine, other};
+      if (cc < cw && j < nb) {
+        const int irot = f0 + j;
+        const cx<T> out = rot_half[irot + s.n_rot * ir] * c;
+        reinterpret_cast<T*>(psi)[2 * (izr + site + static_cast<long>(s.n_zr) * irot) + ri] = ri ? out.y : out.x;
+      }
+    }
+  }
+}
+
+template <typename T>
+inline std::size_t interaction_mfma_lds(int C, int n_theta, int nphi) {
+  return ((nphi * sizeof(int) + 15) / 16) * 16 + std::size_t(C) * n_theta * (nphi + 1) * sizeof(cx<T>);
+}
+
+}  // namespace spo_dev

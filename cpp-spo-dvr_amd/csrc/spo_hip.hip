@@ -9,6 +9,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <stdexcept>
@@ -17,6 +18,7 @@
 
 #include "../../include/spo_hip.h"
 #include "kernels_ct.hpp"
+#include "kernels_mfma.hpp"
 
 using spo_dev::cx;
 
@@ -140,31 +142,55 @@ void set_lds(const void* kernel, std::size_t bytes) {
 }
 
 // ---- instantiations of the compile-time-shaped kernels ---------------------------------------------------------
-// T_Zr planes (NZ, NR, threads) and T_xy tiles (NX, NY, C); phi lengths of the interaction kernel.
-#define SPO_ZR_SHAPES(X) X(96, 48, 384) X(128, 32, 256) X(176, 56, 256) X(64, 16, 256) X(80, 16, 256)
-#define SPO_XY_SHAPES(X) X(16, 16, 16) X(8, 8, 64) X(4, 4, 64)
+// Phi lengths of the interaction kernel with the contractions on the vector units.
 #define SPO_PHI_LENGTHS(X) X(48) X(22) X(10) X(12) X(8) X(56)
+// T_Zr planes with register-resident passes (NZ, NR, A1, A2, B1, B2, threads): NR = A1 A2, NZ = B1 B2.
+#define SPO_ZR_REG_SHAPES(X)                                                                                 \
+  X(96, 48, 3, 16, 8, 12, 192) X(128, 32, 4, 8, 16, 8, 256) X(64, 16, 4, 4, 8, 8, 128)                        \
+  X(80, 16, 4, 4, 8, 10, 128) X(176, 56, 7, 8, 11, 16, 256)
+// Interaction with the theta contractions on MFMA (phi length, N tiles of 8 columns, wavefronts, phi radices R1 x R2
+// of the fused phi step).
+#define SPO_MFMA_SHAPES(X) \
+  X(48, 1, 8, 12, 4) X(22, 1, 8, 11, 2) X(12, 1, 8, 4, 3) X(10, 1, 8, 5, 2) X(8, 1, 8, 4, 2) X(56, 1, 8, 8, 7)
+// T_xy tiles with whole-line DFTs in registers (NX = NY a single radix; C izr per tile, C * NX threads).
+#define SPO_XY_REG_SHAPES(X) X(16, 16, 16) X(8, 8, 32) X(4, 4, 64)
 
 template <typename T>
-const void* zr_ct_symbol(int nz, int nr, int* threads) {
-#define X(A, B, TH)                                                            \
-  if (nz == A && nr == B) {                                                    \
-    *threads = TH;                                                             \
-    return reinterpret_cast<const void*>(spo_dev::zr_ct_kernel<T, A, B, TH>); \
+const void* zr_reg_symbol(int nz, int nr, int* threads) {
+#define X(A, B, A1, A2, B1, B2, TH)                                                                   \
+  if (nz == A && nr == B) {                                                                           \
+    *threads = TH;                                                                                    \
+    return reinterpret_cast<const void*>(spo_dev::zr_reg_kernel<T, A, B, A1, A2, B1, B2, TH>);        \
   }
-  SPO_ZR_SHAPES(X)
+  SPO_ZR_REG_SHAPES(X)
 #undef X
   return nullptr;
 }
 
 template <typename T>
-const void* xy_ct_symbol(int nx, int ny, int* c) {
-#define X(A, B, CC)                                                                     \
-  if (nx == A && ny == B) {                                                             \
-    *c = CC;                                                                            \
-    return reinterpret_cast<const void*>(spo_dev::xy_ct_kernel<T, A, B, CC, kThreads>); \
+const void* xy_reg_symbol(int nx, int ny, int* c, int* threads) {
+#define X(A, B, CC)                                                                  \
+  if (nx == A && ny == B) {                                                          \
+    *c = CC;                                                                         \
+    *threads = CC * A;                                                               \
+    return reinterpret_cast<const void*>(spo_dev::xy_reg_kernel<T, A, B, CC>);       \
   }
-  SPO_XY_SHAPES(X)
+  SPO_XY_REG_SHAPES(X)
+#undef X
+  return nullptr;
+}
+
+// fused: the phi step in three register passes (phi_step_fused) rather than the staged LDS FFTs.
+template <typename T>
+const void* interaction_mfma_symbol(int nphi, bool fused, int* nt, int* waves) {
+#define X(A, NT, W, R1, R2)                                                                                  \
+  if (nphi == A) {                                                                                           \
+    *nt = NT;                                                                                                \
+    *waves = W;                                                                                              \
+    return fused ? reinterpret_cast<const void*>(spo_dev::interaction_mfma_kernel<T, A, NT, W, R1, R2>)      \
+                 : reinterpret_cast<const void*>(spo_dev::interaction_mfma_kernel<T, A, NT, W>);             \
+  }
+  SPO_MFMA_SHAPES(X)
 #undef X
   return nullptr;
 }
@@ -330,6 +356,9 @@ struct State final : spo_hip_backend {
   Buffer psi, zr_full, zr_half, xy_full, xy_half, rot_half, before, gas_step, z_opt;
   Buffer shift_f, shift_b, s2sc, legendre, block_first, block_size, block_offset, rot_block, rep_of, op_of, phi_index;
   Buffer exp_table, legendre_t;
+  Buffer frag_fwd, frag_bwd, fwd_offset, bwd_offset, site_order;
+  const void* v_mfma = nullptr;
+  int mfma_nt = 1, mfma_waves = 4, max_jt = 1;
   int legendre_size = 0;
   std::size_t zr_lds = 0, xy_lds = 0, v_lds = 0;
   std::unique_ptr<DevicePlan<T>> plan_z, plan_r, plan_x, plan_y, plan_phi;
@@ -337,6 +366,8 @@ struct State final : spo_hip_backend {
   const void* zr_fn = nullptr;
   int zr_threads = kThreads;
   const void* xy_fn = nullptr;
+  bool xy_reg = false;
+  int xy_threads = kThreads;
   const void* v_fn = nullptr;
   bool zr_fused = false;
 
@@ -399,13 +430,11 @@ void State<T>::kinetic_and_potential(bool half, bool kinetic_only) {
     const T* zo = z_opt.as<T>();
     const cx<T>* bf = before.as<cx<T>>();
     const cx<T>* gs = gas_step.as<cx<T>>();
-    if (xy_fn) {
-      const cx<T>* tx = plan_x->tw();
-      const cx<T>* ty = plan_y->tw();
-      void* args[] = {&ps, &s, const_cast<cx<T>**>(&tx), const_cast<cx<T>**>(&ty), const_cast<cx<T>**>(&factor),
-                      const_cast<cx<T>**>(&sf), const_cast<cx<T>**>(&sb), &needs_shift, &apply_potential,
-                      const_cast<T**>(&zo), const_cast<cx<T>**>(&bf), const_cast<cx<T>**>(&gs)};
-      timed("xy_tile", [&] { launch(xy_fn, tiles * s.n_rot, kThreads, lds, stream, args); });
+    if (xy_reg) {
+      void* args[] = {&ps, &s, const_cast<cx<T>**>(&factor), const_cast<cx<T>**>(&sf), const_cast<cx<T>**>(&sb),
+                      &needs_shift, &apply_potential, const_cast<T**>(&zo), const_cast<cx<T>**>(&bf),
+                      const_cast<cx<T>**>(&gs)};
+      timed("xy_tile", [&] { launch(xy_fn, tiles * s.n_rot, xy_threads, lds, stream, args); });
     } else {
       timed("xy_tile_generic", [&] {
         hipLaunchKernelGGL(spo_dev::xy_kernel<T>, dim3(tiles * s.n_rot), dim3(kThreads), lds, stream, ps, s, xy_c,
@@ -424,7 +453,17 @@ void State<T>::kinetic_and_potential(bool half, bool kinetic_only) {
   const cx<T>* et = exp_table.as<cx<T>>();
   const cx<T>* rh = rot_half.as<cx<T>>();
   const cx<T>* ssc = s2sc.as<cx<T>>();
-  if (v_fn) {
+  if (v_mfma) {
+    const cx<T>* tp = plan_phi->tw();
+    spo_dev::MfmaTables mt{fwd_offset.as<int>(), bwd_offset.as<int>(), site_order.as<int>()};
+    const T* ff = frag_fwd.as<T>();
+    const T* fb = frag_bwd.as<T>();
+    int mj = max_jt;
+    void* args[] = {&ps, const_cast<cx<double>**>(&sp), &s, const_cast<cx<T>**>(&tp), &at, &mt,
+                    const_cast<T**>(&ff), const_cast<T**>(&fb), &mj, const_cast<cx<T>**>(&et),
+                    const_cast<cx<T>**>(&rh), const_cast<cx<T>**>(&ssc)};
+    timed("interaction", [&] { launch(v_mfma, tiles * s.n_xy, 64 * mfma_waves, lds, stream, args); });
+  } else if (v_fn) {
     const cx<T>* tp = plan_phi->tw();
     const T* lt = legendre_t.as<T>();
     int c = v_c;
@@ -688,11 +727,11 @@ std::unique_ptr<spo_hip_backend> create_state(const spo_hip_shape* shape, const 
   b->plan_x_d = std::make_unique<DevicePlan<double>>(s.nx);
   b->plan_y_d = std::make_unique<DevicePlan<double>>(s.ny);
   const std::size_t lds_limit = static_cast<std::size_t>(b->max_lds);
-  // T_Zr: the compile-time kernel (padded rows) if instantiated and it fits, else the run-time one, else lines.
-  const std::size_t zr_ct_lds = std::size_t(s.nz) * (s.nr + 1) * cT;
-  b->zr_fn = zr_ct_lds <= lds_limit ? zr_ct_symbol<T>(s.nz, s.nr, &b->zr_threads) : nullptr;
+  // T_Zr: the register-pass kernel (padded rows) if instantiated and it fits, else the run-time one, else lines.
+  const std::size_t zr_reg_lds = std::size_t(s.nz) * (s.nr + 1) * cT;
+  b->zr_fn = zr_reg_lds <= lds_limit ? zr_reg_symbol<T>(s.nz, s.nr, &b->zr_threads) : nullptr;
   if (b->zr_fn) {
-    b->zr_lds = zr_ct_lds;
+    b->zr_lds = zr_reg_lds;
     set_lds(b->zr_fn, b->zr_lds);
   } else if (std::size_t(s.n_zr) * cT <= lds_limit) {
     b->zr_fused = true;
@@ -701,7 +740,8 @@ std::unique_ptr<spo_hip_backend> create_state(const spo_hip_shape* shape, const 
   }
   // T_xy.
   int xy_c = 0;
-  b->xy_fn = xy_ct_symbol<T>(s.nx, s.ny, &xy_c);
+  b->xy_fn = xy_reg_symbol<T>(s.nx, s.ny, &xy_c, &b->xy_threads);
+  b->xy_reg = b->xy_fn != nullptr;
   if (b->xy_fn) {
     b->xy_c = xy_c;
     b->xy_lds = std::size_t(s.nx) * (s.ny * (xy_c + 1) + 1) * cT;
@@ -723,6 +763,73 @@ std::unique_ptr<spo_hip_backend> create_state(const spo_hip_shape* shape, const 
   b->v_lds = v_bytes(p2);
   if (b->v_lds > lds_limit) throw Status(SPO_HIP_E_NOT_SUPPORTED, "an angular column does not fit in LDS");
   set_lds(b->v_fn ? b->v_fn : reinterpret_cast<const void*>(spo_dev::interaction_kernel<T>), b->v_lds);
+  // Interaction with the theta contractions on MFMA: Legendre fragments in lane order, XCD-grouped site order.
+  // SPO_HIP_INTERACTION selects a variant for comparisons: "mfma" (default: MFMA contractions, fused register phi
+  // step), "mfma_staged" (MFMA contractions, staged LDS phi FFTs), "lds" (contractions on the vector units).
+  {
+    const char* env = std::getenv("SPO_HIP_INTERACTION");
+    const std::string choice = env ? env : "mfma";
+    if (choice != "mfma" && choice != "mfma_staged" && choice != "lds")
+      throw Status(SPO_HIP_E_ARGUMENT, "SPO_HIP_INTERACTION must be mfma, mfma_staged or lds");
+    int nt = 1, waves = 4;
+    const bool fused = choice == "mfma" && s.n_theta <= 16;
+    const void* fn = interaction_mfma_symbol<T>(s.n_phi, fused, &nt, &waves);
+    const std::size_t lds = spo_dev::interaction_mfma_lds<T>(8 * nt, s.n_theta, s.n_phi);
+    if (fn && lds <= lds_limit && choice != "lds") {
+      const int n_tt = (s.n_theta + 15) / 16, k_bwd = (s.n_theta + 3) / 4;
+      std::vector<int> fo(s.n_blocks), bo(s.n_blocks);
+      int nf = 0, nbw = 0, max_nb = 0;
+      for (int blk = 0; blk < s.n_blocks; ++blk) {
+        const int nb = t->block_size[blk];
+        fo[blk] = nf;
+        nf += n_tt * ((nb + 3) / 4);
+        bo[blk] = nbw;
+        nbw += ((nb + 15) / 16) * k_bwd;
+        max_nb = std::max(max_nb, nb);
+      }
+      std::vector<double> ff(64 * std::size_t(std::max(nf, 1)), 0.0), fb(64 * std::size_t(std::max(nbw, 1)), 0.0);
+      for (int blk = 0; blk < s.n_blocks; ++blk) {
+        const int nb = t->block_size[blk], ks = (nb + 3) / 4;
+        const double* L = t->legendre + offsets[blk];  // (theta, J)
+        for (int lane = 0; lane < 64; ++lane) {
+          const int i = lane % 16, kk = lane / 16;
+          for (int tt = 0; tt < n_tt; ++tt)
+            for (int k = 0; k < ks; ++k) {
+              const int th = 16 * tt + i, j = 4 * k + kk;
+              if (th < s.n_theta && j < nb) ff[64 * std::size_t(fo[blk] + tt * ks + k) + lane] = L[th * nb + j];
+            }
+          for (int jt = 0; jt < (nb + 15) / 16; ++jt)
+            for (int k = 0; k < k_bwd; ++k) {
+              const int th = 4 * k + kk, j = 16 * jt + i;
+              if (th < s.n_theta && j < nb) fb[64 * std::size_t(bo[blk] + jt * k_bwd + k) + lane] = L[th * nb + j];
+            }
+        }
+      }
+      std::vector<int> order(s.n_xy);
+      for (int i = 0; i < s.n_xy; ++i) order[i] = i;
+      if (s.n_xy % 8 == 0) {
+        std::vector<int> sorted(order);
+        std::stable_sort(sorted.begin(), sorted.end(), [&](int x, int y) {
+          return t->representative_of[x] < t->representative_of[y];
+        });
+        const int per = s.n_xy / 8;
+        for (int x = 0; x < 8; ++x)
+          for (int k = 0; k < per; ++k) order[8 * k + x] = sorted[per * x + k];
+      }
+      b->frag_fwd = upload_real<T>(ff.data(), ff.size());
+      b->frag_bwd = upload_real<T>(fb.data(), fb.size());
+      b->fwd_offset = upload(fo.data(), fo.size() * sizeof(int));
+      b->bwd_offset = upload(bo.data(), bo.size() * sizeof(int));
+      b->site_order = upload(order.data(), order.size() * sizeof(int));
+      b->max_jt = (max_nb + 15) / 16;
+      b->v_mfma = fn;
+      b->mfma_nt = nt;
+      b->mfma_waves = waves;
+      b->v_c = 8 * nt;
+      b->v_lds = lds;
+      set_lds(fn, lds);
+    }
+  }
   set_lds(reinterpret_cast<const void*>(spo_dev::lines_fft_kernel<T>), 65536);
   set_lds(reinterpret_cast<const void*>(spo_dev::lines_fft_kernel<double>), 65536);
   set_lds(reinterpret_cast<const void*>(spo_dev::bk_scattering_kernel<T>), std::size_t(s.n_xy) * c16);

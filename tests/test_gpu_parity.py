@@ -25,11 +25,22 @@ CASE_PARAMS = {
 }
 
 
-def _pair(case, overrides=None):
+def _pair(case, overrides=None, interaction=None):
+    """A HIP run and an oracle run of one case. interaction="lds" makes the HIP backend use its LDS-contraction
+    interaction kernel instead of the MFMA one (SPO_HIP_INTERACTION is read when the backend is created)."""
+    import os
     import spodvr_amd
     from oracle import oracle
     path = CASE_PARAMS[case]()
-    gpu = spodvr_amd.Run(path, overrides, device="hip")
+    old = os.environ.pop("SPO_HIP_INTERACTION", None)
+    if interaction:
+        os.environ["SPO_HIP_INTERACTION"] = interaction
+    try:
+        gpu = spodvr_amd.Run(path, overrides, device="hip")
+    finally:
+        os.environ.pop("SPO_HIP_INTERACTION", None)
+        if old is not None:
+            os.environ["SPO_HIP_INTERACTION"] = old
     cpu = oracle.run(path, overrides)
     return gpu, cpu
 
@@ -92,3 +103,41 @@ def test_mixed_precision_t_block_against_fp64_oracle(case):
     print(f"  scattering {ds:.3e} specular {dp:.3e}", flush=True)
     assert ds < 1e-5 and dp < 1e-5
     assert ds > 1e-9  # the mode really stores complex64
+
+
+@pytest.mark.parametrize("case", ["smoke", "cu111_dailight_short_sym", "cu111_dailight_oblique"])
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("interaction", ["lds", "mfma_staged"])
+def test_interaction_variants_match_oracle(case, precision, interaction):
+    """The other interaction kernels (vector-unit contractions; MFMA contractions with staged phi FFTs) on one full
+    T block."""
+    print(f"case {case} seed 31 interaction {interaction} {precision}", flush=True)
+    gpu, cpu = _pair(case, {"Precision": precision}, interaction=interaction)
+    _randomise(gpu, cpu, 31)
+    gpu.kinetic_and_potential(False, False)
+    cpu.kinetic_and_potential(False, False)
+    sg, pg = gpu.get_wave()
+    sc, pc = cpu.get_wave()
+    ds, dp = relative_difference(sc, sg), relative_difference(pc, pg)
+    print(f"  scattering {ds:.3e} specular {dp:.3e}", flush=True)
+    tol = TOL if precision == "fp64" else 1e-5
+    assert ds < tol and dp < tol
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_two_runs_are_bitwise_identical(precision):
+    """Determinism (the reference's CUDA test TwoRunsAreBitwiseIdentical, tests/cuda/test_cuda_backend.cpp:388):
+    no atomics and fixed reduction orders, so two runs of the same input agree bit for bit."""
+    import spodvr_amd
+    print(f"case smoke two runs {precision}", flush=True)
+    path = CASE_PARAMS["smoke"]()
+    out = []
+    for _ in range(2):
+        run = spodvr_amd.Run(path, {"Precision": precision}, device="hip")
+        run.set_wave(pseudo_random(run.scattering_size, 41), pseudo_random(run.specular_size, 42))
+        run.steps(3)
+        s, p = run.get_wave()
+        out.append((s, p, run.bk_records(), run.s_sums(), run.block_densities()))
+        run.close()
+    for a, b in zip(*out):
+        assert np.array_equal(a, b)
