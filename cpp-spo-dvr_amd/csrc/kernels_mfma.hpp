@@ -3,19 +3,27 @@
 // Per workgroup: C = 8 NT consecutive izr (< nz_potential * nr) of one (x, y) site, WAVES wavefronts.
 //   1. FBR -> DVR, theta part: for every mJ block m, D[theta, col] = sum_J L_m[theta, J] c[J, col] as 16 x 16 x 4
 //      MFMA products. The B operand (4 J x 16 real columns: re and im of 8 complex columns) is loaded straight from
-//      HBM, 16 lanes reading 8 consecutive izr of one irot (128 B in FP64); A is a pre-arranged fragment of the
-//      Legendre matrix, one value per lane. D goes to the DVR tile in LDS at phi slot m (mod n_phi).
-//   2. phi FFTs (sign +1) over every (col, theta) row, x exp(i A[rep(ixy), theta, phi_map(op(ixy), phi)]), phi
-//      FFTs back (sign -1).
+//      HBM, 16 lanes reading 8 consecutive izr of one irot (128 B in FP64), all of a wavefront's B values before
+//      its first product; A is a pre-arranged fragment of the Legendre matrix, one value per lane. D goes to the
+//      DVR tile in LDS at phi slot m (mod n_phi).
+//   2. phi FFTs (sign +1) over every (theta, col) row, x exp(i A[rep(ixy), theta, phi_map(op(ixy), phi)]), phi
+//      FFTs back (sign -1): three register passes (phi_step_fused), or staged LDS FFTs.
 //   3. DVR -> FBR: C'[J, col] = sum_theta L_m[theta, J] D'[theta, col] (A = L_m transposed, B read from the tile),
 //      x rot(dt/2), stored straight to HBM.
 // The initial channel's specular coupling term rot(dt/2) s2sc(ixy) chi(Z, r) is added to its FBR value as it is
 // loaded. Workgroups run tile-major over a site order that groups the sites of one symmetry representative on one
 // XCD (blockIdx mod 8), so the exp(i A) rows of a representative are fetched into one L2.
 //
+// DVR tile: row theta * C + col, phi along the row. With the fused phi step and n_phi a multiple of 16 the rows are
+// n_phi long and element phi of row w sits at column phi ^ (w mod 16): the 16 rows a wavefront touches at one phi
+// (phases 1 and 3), or at one column (phase 2, lanes along rows), land on 16 distinct bank groups without padding,
+// which keeps the FP64 tile of 8 x 13 x 48 values at two workgroups per CU. Otherwise rows are padded to n_phi + 1.
+//
 // MFMA 16x16x4 operand layouts (lane l of 64): A[i][k] = lane i + 16 k, B[k][j] = lane j + 16 k; result
 // element v of lane l is D[l / 16 + 4 v][l % 16] for f64 and D[4 (l / 16) + v][l % 16] for f32.
 #pragma once
+
+#include <type_traits>
 
 #include "fft_ct.hpp"
 #include "kernels.hpp"
@@ -54,14 +62,26 @@ struct MfmaTables {
   const int* site_order;  // blockIdx % n_xy -> ixy
 };
 
-// The phi step in three register passes over the DVR rows (NPHI = R1 R2, rows <= ROWS_MAX): radix R1 forward;
-// radix R2 forward, x exp(i A) in natural phi order, radix R2 backward on the same registers; radix R1 backward.
-// Sign +1 towards the DVR, -1 back; W = exp(-2 pi i m / NPHI).
-template <typename T, int NPHI, int R1, int R2, int ROWS_MAX, int THREADS>
-__device__ __forceinline__ void phi_step_fused(cx<T>* dvr, int rows, int n_theta, int cw, const cx<T>* table,
-                                               const int* phi_map, const cx<T>* exp_rows, long exp_stride) {
+// Position of DVR element (row, phi) in the tile.
+template <int NPHI, bool SWZ>
+struct DvrTile {
+  static constexpr int ld = SWZ ? NPHI : NPHI + 1;
+  static __device__ __forceinline__ int at(int row, int phi) { return row * ld + (SWZ ? (phi ^ (row & 15)) : phi); }
+};
+
+template <int NPHI, int R1>
+constexpr bool dvr_swizzled() {
+  return R1 > 0 && NPHI % 16 == 0;
+}
+
+// The phi step in three register passes over the DVR rows (NPHI = R1 R2; rows = C n_theta <= C 16): radix R1
+// forward; radix R2 forward, x exp(i A) in natural phi order, radix R2 backward on the same registers; radix R1
+// backward. Butterfly b: row b % (16 C), lanes along rows. Sign +1 towards the DVR, -1 back; W = exp(-2 pi i m/NPHI).
+template <typename T, int NPHI, int R1, int R2, int C, int THREADS, class Tile>
+__device__ __forceinline__ void phi_step_fused(cx<T>* dvr, int rows, int cw, const cx<T>* table, const int* phi_map,
+                                               const cx<T>* exp_rows, int exp_stride) {
   static_assert(R1 * R2 == NPHI, "phi radices");
-  constexpr int LD = NPHI + 1;
+  constexpr int ROWS_MAX = 16 * C;
   const int tid = threadIdx.x;
   {
     constexpr int Q = per_thread<ROWS_MAX * R2, THREADS>();
@@ -69,10 +89,10 @@ __device__ __forceinline__ void phi_step_fused(cx<T>* dvr, int rows, int n_theta
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
       const int b = tid + q * THREADS;
-      if (b < rows * R2) {
-        const int j = b % R2, row = b / R2;
+      const int row = b % ROWS_MAX, j = b / ROWS_MAX;
+      if (row < rows && j < R2) {
 #pragma unroll
-        for (int r = 0; r < R1; ++r) v[q][r] = dvr[row * LD + j + r * R2];
+        for (int r = 0; r < R1; ++r) v[q][r] = dvr[Tile::at(row, j + r * R2)];
         rdft<R1, T>(v[q], +1);
       }
     }
@@ -80,10 +100,10 @@ __device__ __forceinline__ void phi_step_fused(cx<T>* dvr, int rows, int n_theta
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
       const int b = tid + q * THREADS;
-      if (b < rows * R2) {
-        const int j = b % R2, row = b / R2;
+      const int row = b % ROWS_MAX, j = b / ROWS_MAX;
+      if (row < rows && j < R2) {
 #pragma unroll
-        for (int r = 0; r < R1; ++r) dvr[row * LD + j * R1 + r] = v[q][r];
+        for (int r = 0; r < R1; ++r) dvr[Tile::at(row, j * R1 + r)] = v[q][r];
       }
     }
     __syncthreads();
@@ -91,21 +111,28 @@ __device__ __forceinline__ void phi_step_fused(cx<T>* dvr, int rows, int n_theta
   {
     constexpr int Q = per_thread<ROWS_MAX * R1, THREADS>();
     cx<T> v[Q][R2];
+    cx<T> ev[Q][R2];  // the exp(i A) values, loaded first so that their latency overlaps the LDS work
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
       const int b = tid + q * THREADS;
-      if (b < rows * R1) {
-        const int j = b % R1, row = b / R1;
-        const int th = row % n_theta, cc = row / n_theta;
+      const int row = b % ROWS_MAX, j = b / ROWS_MAX;
+      const int th = row / C, cc = row % C;
+      const bool on = row < rows && j < R1 && cc < cw;
+      const cx<T>* e = exp_rows + exp_stride * cc + NPHI * th;
 #pragma unroll
-        for (int r = 0; r < R2; ++r) v[q][r] = dvr[row * LD + j + r * R1];
+      for (int r = 0; r < R2; ++r) ev[q][r] = on ? e[phi_map[j + r * R1]] : cx<T>{T(1), T(0)};
+    }
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      const int b = tid + q * THREADS;
+      const int row = b % ROWS_MAX, j = b / ROWS_MAX;
+      if (row < rows && j < R1) {
+#pragma unroll
+        for (int r = 0; r < R2; ++r) v[q][r] = dvr[Tile::at(row, j + r * R1)];
         twiddle<T, R2>(v[q], table, j, +1);
         rdft<R2, T>(v[q], +1);
-        if (cc < cw) {
-          const cx<T>* e = exp_rows + exp_stride * cc + NPHI * th;
 #pragma unroll
-          for (int r = 0; r < R2; ++r) v[q][r] = e[phi_map[j + r * R1]] * v[q][r];
-        }
+        for (int r = 0; r < R2; ++r) v[q][r] = ev[q][r] * v[q][r];
         rdft<R2, T>(v[q], -1);
       }
     }
@@ -113,10 +140,10 @@ __device__ __forceinline__ void phi_step_fused(cx<T>* dvr, int rows, int n_theta
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
       const int b = tid + q * THREADS;
-      if (b < rows * R1) {
-        const int j = b % R1, row = b / R1;
+      const int row = b % ROWS_MAX, j = b / ROWS_MAX;
+      if (row < rows && j < R1) {
 #pragma unroll
-        for (int r = 0; r < R2; ++r) dvr[row * LD + j * R2 + r] = v[q][r];
+        for (int r = 0; r < R2; ++r) dvr[Tile::at(row, j * R2 + r)] = v[q][r];
       }
     }
     __syncthreads();
@@ -126,15 +153,15 @@ __device__ __forceinline__ void phi_step_fused(cx<T>* dvr, int rows, int n_theta
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
       const int b = tid + q * THREADS;
-      if (b < rows * R2) {
-        const int j = b % R2, row = b / R2;
+      const int row = b % ROWS_MAX, j = b / ROWS_MAX;
+      if (row < rows && j < R2) {
         cx<T> v[R1];
 #pragma unroll
-        for (int r = 0; r < R1; ++r) v[r] = dvr[row * LD + j + r * R2];
+        for (int r = 0; r < R1; ++r) v[r] = dvr[Tile::at(row, j + r * R2)];
         twiddle<T, R1>(v, table, j, -1);
         rdft<R1, T>(v, -1);
 #pragma unroll
-        for (int r = 0; r < R1; ++r) dvr[row * LD + j + r * R2] = v[r];
+        for (int r = 0; r < R1; ++r) dvr[Tile::at(row, j + r * R2)] = v[r];
       }
     }
     __syncthreads();
@@ -148,9 +175,9 @@ __global__ __launch_bounds__(64 * WAVES) void interaction_mfma_kernel(
     const cx<T>* s2sc) {
   using M = Mfma<T>;
   using acc_t = typename M::acc_t;
+  using Tile = DvrTile<NPHI, dvr_swizzled<NPHI, R1>()>;
   constexpr int C = 8 * NT;
   constexpr int THREADS = 64 * WAVES;
-  constexpr int LD = NPHI + 1;
   extern __shared__ unsigned char smem_raw[];
   int* phi_map = reinterpret_cast<int*>(smem_raw);
   cx<T>* dvr = reinterpret_cast<cx<T>*>(smem_raw + ((NPHI * sizeof(int) + 15) / 16) * 16);
@@ -161,55 +188,91 @@ __global__ __launch_bounds__(64 * WAVES) void interaction_mfma_kernel(
   const int ixy = mt.site_order[blockIdx.x % s.n_xy];
   const int izr0 = tile * C;
   const int cw = min(C, s.n_zr_pot - izr0);
-  const long site = static_cast<long>(s.n_zr) * s.n_rot * ixy;
+  // This site's column of psi (n_zr * n_rot values, so 32-bit offsets within it) as components.
+  T* site_t = reinterpret_cast<T*>(psi + static_cast<long>(s.n_zr) * s.n_rot * ixy);
   const int tid = threadIdx.x;
   const int lane = tid % 64;
-  const int wave = tid / 64;
+  const int wave = __builtin_amdgcn_readfirstlane(tid / 64);
   const int n = lane % 16;   // B / D column: complex column n / 2 of the N tile, component n % 2
   const int kk = lane / 16;  // B row within the k step
+  const int ri = n % 2;
   const int rows = C * n_theta;
 
   const int op = at.op_of[ixy];
   for (int e = tid; e < NPHI; e += THREADS) phi_map[e] = at.phi_index[e + NPHI * op];
   // Slots of |mJ| > MJMax carry no basis function.
   const int n_unused = NPHI - s.n_blocks;
-  for (int e = tid; e < rows * n_unused; e += THREADS) dvr[(e / n_unused) * LD + s.mj_max + 1 + e % n_unused] = {};
+  for (int e = tid; e < rows * n_unused; e += THREADS) dvr[Tile::at(e / n_unused, s.mj_max + 1 + e % n_unused)] = {};
 
-  // 1. FBR -> DVR (theta). Task = (mJ block, theta tile, N tile).
+  // 1. FBR -> DVR (theta). Task = (mJ block, theta tile, N tile); tasks are wavefront-uniform. The initial
+  // channel's specular coupling term has been added to psi beforehand (specular_coupling_kernel).
   const int n_tt = (n_theta + 15) / 16;
   const int n_tasks_f = s.n_blocks * n_tt * NT;
-  for (int task = wave; task < n_tasks_f; task += WAVES) {
-    const int nt = task % NT;
-    const int tt = (task / NT) % n_tt;
-    const int bb = task / (NT * n_tt);
-    const int f0 = at.block_first[bb], nb = at.block_size[bb];
-    const int m = bb - s.mj_max;
-    const int slot = m >= 0 ? m : NPHI + m;
-    const int cc = nt * 8 + n / 2, ri = n % 2;
-    const int izr = izr0 + cc;
-    const bool live = cc < cw;
-    const int k_steps = (nb + 3) / 4;
-    const T* frag = frag_fwd + 64L * (mt.fwd_offset[bb] + tt * k_steps);
-    acc_t acc = {};
-    for (int k = 0; k < k_steps; ++k) {
-      const int j = 4 * k + kk;
-      T b = T(0);
-      if (live && j < nb) {
-        const int irot = f0 + j;
-        b = reinterpret_cast<const T*>(psi)[2 * (izr + site + static_cast<long>(s.n_zr) * irot) + ri];
-        if (irot == s.ichn) {
-          const int ir = izr % s.nr, iz = izr / s.nr;
-          const cx<double> in = spec[ir + s.nr * iz];
-          const cx<T> add = rot_half[s.ichn + s.n_rot * ir] * (s2sc[ixy] * cx<T>{T(in.x), T(in.y)});
-          b = b + (ri ? add.y : add.x);
-        }
-      }
-      acc = M::fma(frag[64 * k + lane], b, acc);
-    }
+  constexpr int kPrefetchTasks = 8;
+  // One theta tile, one N tile, at most 16 J per block: every lane's addresses are a per-lane constant plus a
+  // wavefront-uniform offset, and all of a wavefront's B values are loaded before its first product.
+  const bool fast = NT == 1 && n_tt == 1 && max_jt == 1 && n_tasks_f <= kPrefetchTasks * WAVES;
+  const int cc0 = n / 2;
+  const bool col_live = cc0 < cw;
+  if (fast) {
+    const int lane_b = 2 * (izr0 + cc0 + s.n_zr * kk) + ri;  // B element (J = kk) of block offset 0
+    int dvr_off[4], dvr_swz[4];                               // phase-1 store positions of this lane's 4 rows
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
-      const int th = 16 * tt + M::row(lane, v);
-      if (th < n_theta) dvr_t[2 * ((cc * n_theta + th) * LD + slot) + ri] = acc[v];
+      const int row = M::row(lane, v) * C + cc0;
+      dvr_off[v] = 2 * row * Tile::ld + ri;
+      dvr_swz[v] = Tile::at(row, 0) - row * Tile::ld;  // the row's column swizzle (0 without swizzling)
+    }
+    T bv[kPrefetchTasks][4];
+#pragma unroll
+    for (int i = 0; i < kPrefetchTasks; ++i) {
+      const int bb = wave + i * WAVES;
+      const int f0 = bb < s.n_blocks ? at.block_first[bb] : 0;
+      const int nb = bb < s.n_blocks ? at.block_size[bb] : 0;
+      const T* src = site_t + 2L * s.n_zr * f0;
+#pragma unroll
+      for (int k = 0; k < 4; ++k)
+        bv[i][k] = (col_live && 4 * k + kk < nb) ? src[lane_b + 8 * s.n_zr * k] : T(0);
+    }
+#pragma unroll
+    for (int i = 0; i < kPrefetchTasks; ++i) {
+      const int bb = wave + i * WAVES;
+      if (bb >= s.n_blocks) break;
+      const int nb = at.block_size[bb];
+      const int m = bb - s.mj_max;
+      const int slot = m >= 0 ? m : NPHI + m;
+      const int k_steps = (nb + 3) / 4;
+      const T* frag = frag_fwd + 64L * mt.fwd_offset[bb] + lane;
+      acc_t acc = {};
+#pragma unroll
+      for (int k = 0; k < 4; ++k)
+        if (k < k_steps) acc = M::fma(frag[64 * k], bv[i][k], acc);
+#pragma unroll
+      for (int v = 0; v < 4; ++v)
+        if (M::row(lane, v) < n_theta) dvr_t[dvr_off[v] + 2 * (slot ^ dvr_swz[v])] = acc[v];
+    }
+  } else {
+    for (int task = wave; task < n_tasks_f; task += WAVES) {
+      const int nt = task % NT;
+      const int tt = (task / NT) % n_tt;
+      const int bb = task / (NT * n_tt);
+      const int f0 = at.block_first[bb], nb = at.block_size[bb];
+      const int m = bb - s.mj_max;
+      const int slot = m >= 0 ? m : NPHI + m;
+      const int cc = nt * 8 + n / 2;
+      const int k_steps = (nb + 3) / 4;
+      const T* frag = frag_fwd + 64L * (mt.fwd_offset[bb] + tt * k_steps);
+      acc_t acc = {};
+      for (int k = 0; k < k_steps; ++k) {
+        const int j = 4 * k + kk;
+        const T b = (cc < cw && j < nb) ? site_t[2 * (izr0 + cc + s.n_zr * (f0 + j)) + ri] : T(0);
+        acc = M::fma(frag[64 * k + lane], b, acc);
+      }
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int th = 16 * tt + M::row(lane, v);
+        if (th < n_theta) dvr_t[2 * Tile::at(th * C + cc, slot) + ri] = acc[v];
+      }
     }
   }
   __syncthreads();
@@ -218,20 +281,20 @@ __global__ __launch_bounds__(64 * WAVES) void interaction_mfma_kernel(
   const int rep = at.rep_of[ixy];
   const int n_tp = n_theta * NPHI;
   if constexpr (R1 > 0) {
-    const long stride = static_cast<long>(n_tp) * s.n_rep;
-    phi_step_fused<T, NPHI, R1, R2, C * 16, THREADS>(dvr, rows, n_theta, cw, table_phi, phi_map,
-                                                     exp_table + n_tp * (rep + static_cast<long>(s.n_rep) * izr0),
-                                                     stride);
+    const int stride = n_tp * s.n_rep;
+    phi_step_fused<T, NPHI, R1, R2, C, THREADS, Tile>(dvr, rows, cw, table_phi, phi_map,
+                                                      exp_table + n_tp * (rep + static_cast<long>(s.n_rep) * izr0),
+                                                      stride);
   } else {
-    using Rows = CtLayout<0, LD, 0, 1>;
+    using Rows = CtLayout<0, Tile::ld, 0, 1>;
     ct_fft<T, NPHI, THREADS, Rows>(dvr, rows, table_phi, +1);
     for (int e = tid; e < rows * NPHI; e += THREADS) {
       const int ip = e % NPHI;
-      const int row = e / NPHI;  // cc * n_theta + theta
-      const int th = row % n_theta;
-      const int cc = row / n_theta;
+      const int row = e / NPHI;  // theta * C + col
+      const int th = row / C;
+      const int cc = row % C;
       if (cc >= cw) continue;
-      cx<T>& v = dvr[row * LD + ip];
+      cx<T>& v = dvr[Tile::at(row, ip)];
       v = exp_table[phi_map[ip] + NPHI * th + static_cast<long>(n_tp) * (rep + static_cast<long>(s.n_rep) * (izr0 + cc))] * v;
     }
     __syncthreads();
@@ -240,42 +303,102 @@ __global__ __launch_bounds__(64 * WAVES) void interaction_mfma_kernel(
 
   // 3. DVR -> FBR, x rot(dt/2), to HBM. Task = (mJ block, J tile, N tile); max_jt = ceil(largest block / 16).
   const int k_bwd = (n_theta + 3) / 4;
-  for (int task = wave; task < s.n_blocks * max_jt * NT; task += WAVES) {
-    const int nt = task % NT;
-    const int jt = (task / NT) % max_jt;
-    const int bb = task / (NT * max_jt);
-    const int f0 = at.block_first[bb], nb = at.block_size[bb];
-    if (16 * jt >= nb) continue;  // wave-uniform
-    const int m = bb - s.mj_max;
-    const int slot = m >= 0 ? m : NPHI + m;
-    const int cc = nt * 8 + n / 2, ri = n % 2;
-    const T* frag = frag_bwd + 64L * (mt.bwd_offset[bb] + jt * k_bwd);
-    acc_t acc = {};
-    for (int k = 0; k < k_bwd; ++k) {
-      const int th = 4 * k + kk;
-      const T b = th < n_theta ? dvr_t[2 * ((cc * n_theta + th) * LD + slot) + ri] : T(0);
-      acc = M::fma(frag[64 * k + lane], b, acc);
-    }
-    const int izr = izr0 + cc;
+  if (fast) {
+    // k_bwd <= 4 (n_theta <= 16), one J tile per block.
+    const int izr = izr0 + cc0;
     const int ir = izr % s.nr;
+    int b_off[4], b_swz[4];
 #pragma unroll
-    for (int v = 0; v < 4; ++v) {
-      const T mine = acc[v];
-      const T other = __shfl_xor(mine, 1);
-      const int j = 16 * jt + M::row(lane, v);
-      const cx<T> c = ri ? cx<T>{other, mine} : cx<T>{mine, other};
-      if (cc < cw && j < nb) {
-        const int irot = f0 + j;
-        const cx<T> out = rot_half[irot + s.n_rot * ir] * c;
-        reinterpret_cast<T*>(psi)[2 * (izr + site + static_cast<long>(s.n_zr) * irot) + ri] = ri ? out.y : out.x;
+    for (int k = 0; k < 4; ++k) {
+      const int row = (4 * k + kk) * C + cc0;
+      b_off[k] = 2 * row * Tile::ld + ri;
+      b_swz[k] = Tile::at(row, 0) - row * Tile::ld;
+    }
+    for (int bb = wave; bb < s.n_blocks; bb += WAVES) {
+      const int f0 = at.block_first[bb], nb = at.block_size[bb];
+      const int m = bb - s.mj_max;
+      const int slot = m >= 0 ? m : NPHI + m;
+      const T* frag = frag_bwd + 64L * mt.bwd_offset[bb] + lane;
+      acc_t acc = {};
+#pragma unroll
+      for (int k = 0; k < 4; ++k)
+        if (k < k_bwd) {
+          const T b = 4 * k + kk < n_theta ? dvr_t[b_off[k] + 2 * (slot ^ b_swz[k])] : T(0);
+          acc = M::fma(frag[64 * k], b, acc);
+        }
+      const cx<T>* rot = rot_half + f0 + s.n_rot * ir;
+      T* dst = site_t + 2L * s.n_zr * f0;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const T mine = acc[v];
+        const T other = __shfl_xor(mine, 1);
+        const int j = M::row(lane, v);
+        const cx<T> c = ri ? cx<T>{other, mine} : cx<T>{mine, other};
+        if (col_live && j < nb) {
+          const cx<T> out = rot[j] * c;
+          dst[2 * (izr + s.n_zr * j) + ri] = ri ? out.y : out.x;
+        }
       }
     }
+  } else {
+    int ir_of[NT];
+#pragma unroll
+    for (int nt = 0; nt < NT; ++nt) ir_of[nt] = (izr0 + nt * 8 + n / 2) % s.nr;
+    for (int bt = wave; bt < s.n_blocks * NT; bt += WAVES)
+      for (int jt = 0; jt < max_jt; ++jt) {
+        const int nt = bt % NT;
+        const int bb = bt / NT;
+        const int f0 = at.block_first[bb], nb = at.block_size[bb];
+        if (16 * jt >= nb) break;  // wave-uniform
+        const int m = bb - s.mj_max;
+        const int slot = m >= 0 ? m : NPHI + m;
+        const int cc = nt * 8 + n / 2;
+        const T* frag = frag_bwd + 64L * (mt.bwd_offset[bb] + jt * k_bwd);
+        acc_t acc = {};
+        for (int k = 0; k < k_bwd; ++k) {
+          const int th = 4 * k + kk;
+          const T b = th < n_theta ? dvr_t[2 * Tile::at(th * C + cc, slot) + ri] : T(0);
+          acc = M::fma(frag[64 * k + lane], b, acc);
+        }
+        const int izr = izr0 + cc;
+        const int ir = ir_of[nt];
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const T mine = acc[v];
+          const T other = __shfl_xor(mine, 1);
+          const int j = 16 * jt + M::row(lane, v);
+          const cx<T> c = ri ? cx<T>{other, mine} : cx<T>{mine, other};
+          if (cc < cw && j < nb) {
+            const int irot = f0 + j;
+            const cx<T> out = rot_half[irot + s.n_rot * ir] * c;
+            site_t[2 * (izr + s.n_zr * irot) + ri] = ri ? out.y : out.x;
+          }
+        }
+      }
   }
 }
 
+// The initial channel's specular coupling term of the interaction region, added to psi before the MFMA interaction
+// kernel: psi[ichn](izr, ixy) += rot(dt/2)[ichn, ir] (s2sc[ixy] chi(ir, iz)) for izr < nz_potential * nr.
 template <typename T>
-inline std::size_t interaction_mfma_lds(int C, int n_theta, int nphi) {
-  return ((nphi * sizeof(int) + 15) / 16) * 16 + std::size_t(C) * n_theta * (nphi + 1) * sizeof(cx<T>);
+__global__ __launch_bounds__(256) void specular_coupling_kernel(cx<T>* psi, const cx<double>* spec, Shape s,
+                                                                 const cx<T>* rot_half, const cx<T>* s2sc) {
+  const long e = static_cast<long>(blockIdx.x) * 256 + threadIdx.x;
+  if (e >= static_cast<long>(s.n_zr_pot) * s.n_xy) return;
+  const int izr = static_cast<int>(e % s.n_zr_pot);
+  const int ixy = static_cast<int>(e / s.n_zr_pot);
+  const int ir = izr % s.nr, iz = izr / s.nr;
+  const cx<double> in = spec[ir + s.nr * iz];
+  cx<T>& f = psi[izr + static_cast<long>(s.n_zr) * (s.ichn + static_cast<long>(s.n_rot) * ixy)];
+  f = f + rot_half[s.ichn + s.n_rot * ir] * (s2sc[ixy] * cx<T>{T(in.x), T(in.y)});
+}
+
+// LDS bytes of the MFMA interaction kernel (phi map + DVR tile of C n_theta rows).
+template <typename T>
+inline std::size_t interaction_mfma_lds(int C, int n_theta, int nphi, bool fused) {
+  const bool swizzled = fused && nphi % 16 == 0;
+  return ((nphi * sizeof(int) + 15) / 16) * 16 +
+         std::size_t(C) * n_theta * (swizzled ? nphi : nphi + 1) * sizeof(cx<T>);
 }
 
 }  // namespace spo_dev

@@ -462,6 +462,10 @@ void State<T>::kinetic_and_potential(bool half, bool kinetic_only) {
     void* args[] = {&ps, const_cast<cx<double>**>(&sp), &s, const_cast<cx<T>**>(&tp), &at, &mt,
                     const_cast<T**>(&ff), const_cast<T**>(&fb), &mj, const_cast<cx<T>**>(&et),
                     const_cast<cx<T>**>(&rh), const_cast<cx<T>**>(&ssc)};
+    timed("specular_coupling", [&] {
+      hipLaunchKernelGGL(spo_dev::specular_coupling_kernel<T>, dim3(blocks_for(long(s.n_zr_pot) * s.n_xy, 256)),
+                         dim3(256), 0, stream, ps, sp, s, rh, ssc);
+    });
     timed("interaction", [&] { launch(v_mfma, tiles * s.n_xy, 64 * mfma_waves, lds, stream, args); });
   } else if (v_fn) {
     const cx<T>* tp = plan_phi->tw();
@@ -774,7 +778,7 @@ std::unique_ptr<spo_hip_backend> create_state(const spo_hip_shape* shape, const 
     int nt = 1, waves = 4;
     const bool fused = choice == "mfma" && s.n_theta <= 16;
     const void* fn = interaction_mfma_symbol<T>(s.n_phi, fused, &nt, &waves);
-    const std::size_t lds = spo_dev::interaction_mfma_lds<T>(8 * nt, s.n_theta, s.n_phi);
+    const std::size_t lds = spo_dev::interaction_mfma_lds<T>(8 * nt, s.n_theta, s.n_phi, fused);
     if (fn && lds <= lds_limit && choice != "lds") {
       const int n_tt = (s.n_theta + 15) / 16, k_bwd = (s.n_theta + 3) / 4;
       std::vector<int> fo(s.n_blocks), bo(s.n_blocks);
