@@ -14,10 +14,11 @@
 // loaded. Workgroups run tile-major over a site order that groups the sites of one symmetry representative on one
 // XCD (blockIdx mod 8), so the exp(i A) rows of a representative are fetched into one L2.
 //
-// DVR tile: row theta * C + col, phi along the row. With the fused phi step and n_phi a multiple of 16 the rows are
-// n_phi long and element phi of row w sits at column phi ^ (w mod 16): the 16 rows a wavefront touches at one phi
-// (phases 1 and 3), or at one column (phase 2, lanes along rows), land on 16 distinct bank groups without padding,
-// which keeps the FP64 tile of 8 x 13 x 48 values at two workgroups per CU. Otherwise rows are padded to n_phi + 1.
+// DVR tile: row theta * C + col, phi along the row. In FP64 with the fused phi step and n_phi a multiple of 16 the
+// rows are n_phi long and element phi of row w sits at column phi ^ (w mod 16): the 16 rows a wavefront touches at
+// one phi (phases 1 and 3), or at one column (phase 2, lanes along rows), land on 16 distinct bank groups without
+// padding, which keeps the FP64 tile of 8 x 13 x 48 values at two workgroups per CU. Otherwise rows are padded to
+// n_phi + 1 (an odd stride spreads the rows over the banks just as well).
 //
 // MFMA 16x16x4 operand layouts (lane l of 64): A[i][k] = lane i + 16 k, B[k][j] = lane j + 16 k; result
 // element v of lane l is D[l / 16 + 4 v][l % 16] for f64 and D[4 (l / 16) + v][l % 16] for f32.
@@ -69,9 +70,11 @@ struct DvrTile {
   static __device__ __forceinline__ int at(int row, int phi) { return row * ld + (SWZ ? (phi ^ (row & 15)) : phi); }
 };
 
-template <int NPHI, int R1>
+// FP64 tiles are swizzled (unpadded, to fit two workgroups per CU); complex64 tiles are padded, so that the fused
+// phi passes address their elements as a per-thread base plus compile-time offsets.
+template <typename T, int NPHI, int R1>
 constexpr bool dvr_swizzled() {
-  return R1 > 0 && NPHI % 16 == 0;
+  return sizeof(T) == 8 && R1 > 0 && NPHI % 16 == 0;
 }
 
 // The phi step in three register passes over the DVR rows (NPHI = R1 R2; rows = C n_theta <= C 16): radix R1
@@ -175,7 +178,7 @@ __global__ __launch_bounds__(64 * WAVES) void interaction_mfma_kernel(
     const cx<T>* s2sc) {
   using M = Mfma<T>;
   using acc_t = typename M::acc_t;
-  using Tile = DvrTile<NPHI, dvr_swizzled<NPHI, R1>()>;
+  using Tile = DvrTile<NPHI, dvr_swizzled<T, NPHI, R1>()>;
   constexpr int C = 8 * NT;
   constexpr int THREADS = 64 * WAVES;
   extern __shared__ unsigned char smem_raw[];
@@ -211,19 +214,30 @@ __global__ __launch_bounds__(64 * WAVES) void interaction_mfma_kernel(
   constexpr int kPrefetchTasks = 8;
   // One theta tile, one N tile, at most 16 J per block: every lane's addresses are a per-lane constant plus a
   // wavefront-uniform offset, and all of a wavefront's B values are loaded before its first product.
-  const bool fast = NT == 1 && n_tt == 1 && max_jt == 1 && n_tasks_f <= kPrefetchTasks * WAVES;
-  const int cc0 = n / 2;
-  const bool col_live = cc0 < cw;
-  if (fast) {
-    const int lane_b = 2 * (izr0 + cc0 + s.n_zr * kk) + ri;  // B element (J = kk) of block offset 0
-    int dvr_off[4], dvr_swz[4];                               // phase-1 store positions of this lane's 4 rows
+  const bool fast = n_tt == 1 && max_jt == 1 && s.n_blocks <= kPrefetchTasks * WAVES;
+  // Per N tile: this lane's complex column and whether it lies inside the tile.
+  int cc_of[NT];
+  bool live_of[NT];
 #pragma unroll
-    for (int v = 0; v < 4; ++v) {
-      const int row = M::row(lane, v) * C + cc0;
-      dvr_off[v] = 2 * row * Tile::ld + ri;
-      dvr_swz[v] = Tile::at(row, 0) - row * Tile::ld;  // the row's column swizzle (0 without swizzling)
+  for (int nt = 0; nt < NT; ++nt) {
+    cc_of[nt] = nt * 8 + n / 2;
+    live_of[nt] = cc_of[nt] < cw;
+  }
+  if (fast) {
+    // Task = mJ block (all N tiles); B element (J = kk) of block offset 0, and the phase-1 store positions of this
+    // lane's 4 theta rows, per N tile.
+    int lane_b[NT], dvr_off[NT][4], dvr_swz[NT][4];
+#pragma unroll
+    for (int nt = 0; nt < NT; ++nt) {
+      lane_b[nt] = 2 * (izr0 + cc_of[nt] + s.n_zr * kk) + ri;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int row = M::row(lane, v) * C + cc_of[nt];
+        dvr_off[nt][v] = 2 * row * Tile::ld + ri;
+        dvr_swz[nt][v] = Tile::at(row, 0) - row * Tile::ld;  // the row's column swizzle (0 without swizzling)
+      }
     }
-    T bv[kPrefetchTasks][4];
+    T bv[kPrefetchTasks][NT][4];
 #pragma unroll
     for (int i = 0; i < kPrefetchTasks; ++i) {
       const int bb = wave + i * WAVES;
@@ -231,8 +245,10 @@ __global__ __launch_bounds__(64 * WAVES) void interaction_mfma_kernel(
       const int nb = bb < s.n_blocks ? at.block_size[bb] : 0;
       const T* src = site_t + 2L * s.n_zr * f0;
 #pragma unroll
-      for (int k = 0; k < 4; ++k)
-        bv[i][k] = (col_live && 4 * k + kk < nb) ? src[lane_b + 8 * s.n_zr * k] : T(0);
+      for (int nt = 0; nt < NT; ++nt)
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+          bv[i][nt][k] = (live_of[nt] && 4 * k + kk < nb) ? src[lane_b[nt] + 8 * s.n_zr * k] : T(0);
     }
 #pragma unroll
     for (int i = 0; i < kPrefetchTasks; ++i) {
@@ -243,13 +259,19 @@ __global__ __launch_bounds__(64 * WAVES) void interaction_mfma_kernel(
       const int slot = m >= 0 ? m : NPHI + m;
       const int k_steps = (nb + 3) / 4;
       const T* frag = frag_fwd + 64L * mt.fwd_offset[bb] + lane;
-      acc_t acc = {};
+      acc_t acc[NT] = {};
 #pragma unroll
       for (int k = 0; k < 4; ++k)
-        if (k < k_steps) acc = M::fma(frag[64 * k], bv[i][k], acc);
+        if (k < k_steps) {
+          const T a = frag[64 * k];
 #pragma unroll
-      for (int v = 0; v < 4; ++v)
-        if (M::row(lane, v) < n_theta) dvr_t[dvr_off[v] + 2 * (slot ^ dvr_swz[v])] = acc[v];
+          for (int nt = 0; nt < NT; ++nt) acc[nt] = M::fma(a, bv[i][nt][k], acc[nt]);
+        }
+#pragma unroll
+      for (int nt = 0; nt < NT; ++nt)
+#pragma unroll
+        for (int v = 0; v < 4; ++v)
+          if (M::row(lane, v) < n_theta) dvr_t[dvr_off[nt][v] + 2 * (slot ^ dvr_swz[nt][v])] = acc[nt][v];
     }
   } else {
     for (int task = wave; task < n_tasks_f; task += WAVES) {
@@ -304,39 +326,49 @@ __global__ __launch_bounds__(64 * WAVES) void interaction_mfma_kernel(
   // 3. DVR -> FBR, x rot(dt/2), to HBM. Task = (mJ block, J tile, N tile); max_jt = ceil(largest block / 16).
   const int k_bwd = (n_theta + 3) / 4;
   if (fast) {
-    // k_bwd <= 4 (n_theta <= 16), one J tile per block.
-    const int izr = izr0 + cc0;
-    const int ir = izr % s.nr;
-    int b_off[4], b_swz[4];
+    // k_bwd <= 4 (n_theta <= 16), one J tile per block, all N tiles per task.
+    int izr_of[NT], ir_of[NT], b_off[NT][4], b_swz[NT][4];
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      const int row = (4 * k + kk) * C + cc0;
-      b_off[k] = 2 * row * Tile::ld + ri;
-      b_swz[k] = Tile::at(row, 0) - row * Tile::ld;
+    for (int nt = 0; nt < NT; ++nt) {
+      izr_of[nt] = izr0 + cc_of[nt];
+      ir_of[nt] = izr_of[nt] % s.nr;
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const int row = (4 * k + kk) * C + cc_of[nt];
+        b_off[nt][k] = 2 * row * Tile::ld + ri;
+        b_swz[nt][k] = Tile::at(row, 0) - row * Tile::ld;
+      }
     }
     for (int bb = wave; bb < s.n_blocks; bb += WAVES) {
       const int f0 = at.block_first[bb], nb = at.block_size[bb];
       const int m = bb - s.mj_max;
       const int slot = m >= 0 ? m : NPHI + m;
       const T* frag = frag_bwd + 64L * mt.bwd_offset[bb] + lane;
-      acc_t acc = {};
+      acc_t acc[NT] = {};
 #pragma unroll
       for (int k = 0; k < 4; ++k)
         if (k < k_bwd) {
-          const T b = 4 * k + kk < n_theta ? dvr_t[b_off[k] + 2 * (slot ^ b_swz[k])] : T(0);
-          acc = M::fma(frag[64 * k], b, acc);
+          const T a = frag[64 * k];
+#pragma unroll
+          for (int nt = 0; nt < NT; ++nt) {
+            const T b = 4 * k + kk < n_theta ? dvr_t[b_off[nt][k] + 2 * (slot ^ b_swz[nt][k])] : T(0);
+            acc[nt] = M::fma(a, b, acc[nt]);
+          }
         }
-      const cx<T>* rot = rot_half + f0 + s.n_rot * ir;
       T* dst = site_t + 2L * s.n_zr * f0;
 #pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const T mine = acc[v];
-        const T other = __shfl_xor(mine, 1);
-        const int j = M::row(lane, v);
-        const cx<T> c = ri ? cx<T>{other, mine} : cx<T>{mine, other};
-        if (col_live && j < nb) {
-          const cx<T> out = rot[j] * c;
-          dst[2 * (izr + s.n_zr * j) + ri] = ri ? out.y : out.x;
+      for (int nt = 0; nt < NT; ++nt) {
+        const cx<T>* rot = rot_half + f0 + s.n_rot * ir_of[nt];
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const T mine = acc[nt][v];
+          const T other = __shfl_xor(mine, 1);
+          const int j = M::row(lane, v);
+          const cx<T> c = ri ? cx<T>{other, mine} : cx<T>{mine, other};
+          if (live_of[nt] && j < nb) {
+            const cx<T> out = rot[j] * c;
+            dst[2 * (izr_of[nt] + s.n_zr * j) + ri] = ri ? out.y : out.x;
+          }
         }
       }
     }
@@ -396,7 +428,7 @@ __global__ __launch_bounds__(256) void specular_coupling_kernel(cx<T>* psi, cons
 // LDS bytes of the MFMA interaction kernel (phi map + DVR tile of C n_theta rows).
 template <typename T>
 inline std::size_t interaction_mfma_lds(int C, int n_theta, int nphi, bool fused) {
-  const bool swizzled = fused && nphi % 16 == 0;
+  const bool swizzled = sizeof(T) == 8 && fused && nphi % 16 == 0;
   return ((nphi * sizeof(int) + 15) / 16) * 16 +
          std::size_t(C) * n_theta * (swizzled ? nphi : nphi + 1) * sizeof(cx<T>);
 }

@@ -148,10 +148,10 @@ void set_lds(const void* kernel, std::size_t bytes) {
 #define SPO_ZR_REG_SHAPES(X)                                                                                 \
   X(96, 48, 3, 16, 8, 12, 192) X(128, 32, 4, 8, 16, 8, 256) X(64, 16, 4, 4, 8, 8, 128)                        \
   X(80, 16, 4, 4, 8, 10, 128) X(176, 56, 7, 8, 11, 16, 256)
-// Interaction with the theta contractions on MFMA (phi length, N tiles of 8 columns, wavefronts, phi radices R1 x R2
-// of the fused phi step).
+// Interaction with the theta contractions on MFMA (phi length, wavefronts, phi radices R1 x R2 of the fused phi
+// step).
 #define SPO_MFMA_SHAPES(X) \
-  X(48, 1, 8, 12, 4) X(22, 1, 8, 11, 2) X(12, 1, 8, 4, 3) X(10, 1, 8, 5, 2) X(8, 1, 8, 4, 2) X(56, 1, 8, 8, 7)
+  X(48, 8, 12, 4) X(22, 8, 11, 2) X(12, 8, 4, 3) X(10, 8, 5, 2) X(8, 8, 4, 2) X(56, 8, 8, 7)
 // T_xy tiles with whole-line DFTs in registers (NX = NY a single radix; C izr per tile, C * NX threads).
 #define SPO_XY_REG_SHAPES(X) X(16, 16, 16) X(8, 8, 32) X(4, 4, 64)
 
@@ -183,7 +183,8 @@ const void* xy_reg_symbol(int nx, int ny, int* c, int* threads) {
 // fused: the phi step in three register passes (phi_step_fused) rather than the staged LDS FFTs.
 template <typename T>
 const void* interaction_mfma_symbol(int nphi, bool fused, int* nt, int* waves) {
-#define X(A, NT, W, R1, R2)                                                                                  \
+  constexpr int NT = 1;  // 8 columns: one 128-byte line of an irot row in FP64 (two complex64 tiles ran slower)
+#define X(A, W, R1, R2)                                                                                      \
   if (nphi == A) {                                                                                           \
     *nt = NT;                                                                                                \
     *waves = W;                                                                                              \
