@@ -67,8 +67,8 @@ def measured_traffic(kernel: str, workload: str, precision: str) -> float | None
     if not os.path.exists(TRAFFIC_FILE):
         return None
     for line in open(TRAFFIC_FILE):
-        parts = line.split()
-        if len(parts) == 5 and not line.startswith("#") and parts[:3] == [kernel, workload, precision]:
+        parts = line.split("#", 1)[0].split()
+        if len(parts) == 5 and parts[:3] == [kernel, workload, precision]:
             return float(parts[3]) + float(parts[4])
     return None
 
@@ -123,6 +123,7 @@ def measure(path: str, workload: str, precision: str, steps: int, warmup: int) -
     Every seam call returns with the device idle (the analysis step ends in a synchronous copy of the BK
     coefficients), so the wall clock brackets exactly the timed work."""
     import spodvr_amd
+    os.environ.setdefault("SPO_HIP_TUNE_LOG", "1")  # the launch tuner's choices go to stderr
     run = spodvr_amd.Run(path, {"Precision": precision}, device="hip")
     info = run.info
     spa = info.steps_per_analysis

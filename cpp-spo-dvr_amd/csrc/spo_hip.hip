@@ -9,8 +9,10 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <map>
 #include <memory>
 #include <stdexcept>
 #include <string>
@@ -144,56 +146,76 @@ void set_lds(const void* kernel, std::size_t bytes) {
 // ---- instantiations of the compile-time-shaped kernels ---------------------------------------------------------
 // Phi lengths of the interaction kernel with the contractions on the vector units.
 #define SPO_PHI_LENGTHS(X) X(48) X(22) X(10) X(12) X(8) X(56)
-// T_Zr planes with register-resident passes (NZ, NR, A1, A2, B1, B2, threads): NR = A1 A2, NZ = B1 B2.
+// T_Zr planes with register-resident passes (NZ, NR, A1, A2, B1, B2, threads): NR = A1 A2, NZ = B1 B2. Entries of
+// one shape that differ only in the thread count compute identically; the launch tuner picks among them.
 #define SPO_ZR_REG_SHAPES(X)                                                                                 \
-  X(96, 48, 3, 16, 8, 12, 192) X(128, 32, 4, 8, 16, 8, 256) X(64, 16, 4, 4, 8, 8, 128)                        \
-  X(80, 16, 4, 4, 8, 10, 128) X(176, 56, 7, 8, 11, 16, 256)
+  X(96, 48, 3, 16, 8, 12, 192) X(96, 48, 3, 16, 8, 12, 384) X(128, 32, 4, 8, 16, 8, 256)                      \
+  X(128, 32, 4, 8, 16, 8, 512) X(64, 16, 4, 4, 8, 8, 128) X(80, 16, 4, 4, 8, 10, 128)                         \
+  X(176, 56, 7, 8, 11, 16, 256) X(176, 56, 7, 8, 11, 16, 512)
 // Interaction with the theta contractions on MFMA (phi length, wavefronts, phi radices R1 x R2 of the fused phi
-// step).
-#define SPO_MFMA_SHAPES(X) \
-  X(48, 8, 12, 4) X(22, 8, 11, 2) X(12, 8, 4, 3) X(10, 8, 5, 2) X(8, 8, 4, 2) X(56, 8, 8, 7)
+// step); the staged-FFT variant is instantiated with 8 wavefronts.
+#define SPO_MFMA_SHAPES(X)                                                                                   \
+  X(48, 8, 12, 4) X(48, 4, 12, 4) X(48, 16, 12, 4) X(22, 8, 11, 2) X(12, 8, 4, 3) X(10, 8, 5, 2)             \
+  X(8, 8, 4, 2) X(56, 8, 8, 7) X(56, 4, 8, 7)
 // T_xy tiles with whole-line DFTs in registers (NX = NY a single radix; C izr per tile, C * NX threads).
-#define SPO_XY_REG_SHAPES(X) X(16, 16, 16) X(8, 8, 32) X(4, 4, 64)
+#define SPO_XY_REG_SHAPES(X) X(16, 16, 16) X(16, 16, 8) X(8, 8, 32) X(8, 8, 16) X(4, 4, 64) X(4, 4, 32)
+
+// A launch variant of a kernel family: the kernel, its block size, its tile parameter (T_xy: izr per tile;
+// interaction: wavefronts) and its LDS bytes.
+struct Variant {
+  const void* fn;
+  int threads;
+  int param;
+  std::size_t lds;
+};
 
 template <typename T>
-const void* zr_reg_symbol(int nz, int nr, int* threads) {
-#define X(A, B, A1, A2, B1, B2, TH)                                                                   \
-  if (nz == A && nr == B) {                                                                           \
-    *threads = TH;                                                                                    \
-    return reinterpret_cast<const void*>(spo_dev::zr_reg_kernel<T, A, B, A1, A2, B1, B2, TH>);        \
-  }
+std::vector<Variant> zr_reg_variants(int nz, int nr) {
+  std::vector<Variant> out;
+#define X(A, B, A1, A2, B1, B2, TH)                                                                        \
+  if (nz == A && nr == B)                                                                                  \
+    out.push_back({reinterpret_cast<const void*>(spo_dev::zr_reg_kernel<T, A, B, A1, A2, B1, B2, TH>), TH, \
+                   TH, std::size_t(A) * (B + 1) * sizeof(cx<T>)});
   SPO_ZR_REG_SHAPES(X)
 #undef X
-  return nullptr;
+  // Default (first) variant, as measured on MI355X at NZ 96 x NR 48: FP64 planes run best with the larger thread
+  // count (3.9 vs 4.7 ms per time step), complex64 planes with the smaller one.
+  std::stable_sort(out.begin(), out.end(), [](const Variant& a, const Variant& b) {
+    return sizeof(T) == 8 ? a.threads > b.threads : a.threads < b.threads;
+  });
+  return out;
 }
 
 template <typename T>
-const void* xy_reg_symbol(int nx, int ny, int* c, int* threads) {
-#define X(A, B, CC)                                                                  \
-  if (nx == A && ny == B) {                                                          \
-    *c = CC;                                                                         \
-    *threads = CC * A;                                                               \
-    return reinterpret_cast<const void*>(spo_dev::xy_reg_kernel<T, A, B, CC>);       \
-  }
+std::vector<Variant> xy_reg_variants(int nx, int ny) {
+  std::vector<Variant> out;
+#define X(A, B, CC)                                                                                          \
+  if (nx == A && ny == B)                                                                                    \
+    out.push_back({reinterpret_cast<const void*>(spo_dev::xy_reg_kernel<T, A, B, CC>), CC * A, CC,           \
+                   std::size_t(A) * (B * (CC + 1) + 1) * sizeof(cx<T>)});
   SPO_XY_REG_SHAPES(X)
 #undef X
-  return nullptr;
+  // Default (first) variant, as measured on MI355X at 16 x 16: FP64 tiles of 8 izr, complex64 tiles of 16.
+  std::stable_sort(out.begin(), out.end(), [](const Variant& a, const Variant& b) {
+    return sizeof(T) == 8 ? a.param < b.param : a.param > b.param;
+  });
+  return out;
 }
 
-// fused: the phi step in three register passes (phi_step_fused) rather than the staged LDS FFTs.
+// fused: the phi step in three register passes (phi_step_fused) rather than the staged LDS FFTs (8 wavefronts).
 template <typename T>
-const void* interaction_mfma_symbol(int nphi, bool fused, int* nt, int* waves) {
+std::vector<Variant> interaction_mfma_variants(int nphi, int n_theta, bool fused) {
   constexpr int NT = 1;  // 8 columns: one 128-byte line of an irot row in FP64 (two complex64 tiles ran slower)
+  std::vector<Variant> out;
+  const std::size_t lds = spo_dev::interaction_mfma_lds<T>(8 * NT, n_theta, nphi, fused);
 #define X(A, W, R1, R2)                                                                                      \
-  if (nphi == A) {                                                                                           \
-    *nt = NT;                                                                                                \
-    *waves = W;                                                                                              \
-    return fused ? reinterpret_cast<const void*>(spo_dev::interaction_mfma_kernel<T, A, NT, W, R1, R2>)      \
-                 : reinterpret_cast<const void*>(spo_dev::interaction_mfma_kernel<T, A, NT, W>);             \
-  }
+  if (nphi == A && (fused || W == 8))                                                                        \
+    out.push_back({fused ? reinterpret_cast<const void*>(spo_dev::interaction_mfma_kernel<T, A, NT, W, R1, R2>) \
+                         : reinterpret_cast<const void*>(spo_dev::interaction_mfma_kernel<T, A, NT, W>),     \
+                   64 * W, W, lds});
   SPO_MFMA_SHAPES(X)
 #undef X
-  return nullptr;
+  return out;
 }
 
 template <typename T>
@@ -358,19 +380,21 @@ struct State final : spo_hip_backend {
   Buffer shift_f, shift_b, s2sc, legendre, block_first, block_size, block_offset, rot_block, rep_of, op_of, phi_index;
   Buffer exp_table, legendre_t;
   Buffer frag_fwd, frag_bwd, fwd_offset, bwd_offset, site_order;
-  const void* v_mfma = nullptr;
-  int mfma_nt = 1, mfma_waves = 4, max_jt = 1;
+  int max_jt = 1;
   int legendre_size = 0;
   std::size_t zr_lds = 0, xy_lds = 0, v_lds = 0;
   std::unique_ptr<DevicePlan<T>> plan_z, plan_r, plan_x, plan_y, plan_phi;
   int xy_c = 1, v_c = 1;
-  const void* zr_fn = nullptr;
-  int zr_threads = kThreads;
-  const void* xy_fn = nullptr;
-  bool xy_reg = false;
-  int xy_threads = kThreads;
+  // Launch variants of the register-pass and MFMA kernels (identical arithmetic) and the one in use.
+  std::vector<Variant> zr_variants, xy_variants, v_variants;
+  int zr_pick = 0, xy_pick = 0, v_pick = 0;
   const void* v_fn = nullptr;
   bool zr_fused = false;
+
+  void run_zr(const Variant& v, bool half);
+  void run_xy(const Variant& v, bool half, bool kinetic_only);
+  void run_interaction_mfma(const Variant& v);
+  void tune();
 
   void kinetic_and_potential(bool half, bool kinetic_only) override;
   void upload_wave(const double* scattering) override;
@@ -396,17 +420,139 @@ void State<T>::lines(cx<T>* data, spo_dev::Lines L, const DevicePlan<T>& p, int 
 }
 
 template <typename T>
+void State<T>::run_zr(const Variant& v, bool half) {
+  cx<T>* ps = psi.as<cx<T>>();
+  const cx<T>* tz = plan_z->tw();
+  const cx<T>* tr = plan_r->tw();
+  const cx<T>* zr = (half ? zr_half : zr_full).as<cx<T>>();
+  void* args[] = {&ps, const_cast<cx<T>**>(&tz), const_cast<cx<T>**>(&tr), const_cast<cx<T>**>(&zr)};
+  launch(v.fn, s.n_rot * s.n_xy, v.threads, v.lds, stream, args);
+}
+
+template <typename T>
+void State<T>::run_xy(const Variant& v, bool half, bool kinetic_only) {
+  cx<T>* ps = psi.as<cx<T>>();
+  const int tiles = (s.n_zr + v.param - 1) / v.param;
+  const cx<T>* factor = (half ? xy_half : xy_full).as<cx<T>>();
+  const cx<T>* sf = shift_f.as<cx<T>>();
+  const cx<T>* sb = shift_b.as<cx<T>>();
+  int needs_shift = shape.needs_shift;
+  int apply_potential = kinetic_only ? 0 : 1;
+  const T* zo = z_opt.as<T>();
+  const cx<T>* bf = before.as<cx<T>>();
+  const cx<T>* gs = gas_step.as<cx<T>>();
+  void* args[] = {&ps, &s, const_cast<cx<T>**>(&factor), const_cast<cx<T>**>(&sf), const_cast<cx<T>**>(&sb),
+                  &needs_shift, &apply_potential, const_cast<T**>(&zo), const_cast<cx<T>**>(&bf),
+                  const_cast<cx<T>**>(&gs)};
+  launch(v.fn, tiles * s.n_rot, v.threads, v.lds, stream, args);
+}
+
+template <typename T>
+void State<T>::run_interaction_mfma(const Variant& v) {
+  cx<T>* ps = psi.as<cx<T>>();
+  spo_dev::AngularTables at{legendre.as<double>(),  block_first.as<int>(), block_size.as<int>(),
+                            block_offset.as<int>(), rot_block.as<int>(),   rep_of.as<int>(),
+                            op_of.as<int>(),        phi_index.as<int>()};
+  const int tiles = (s.n_zr_pot + 7) / 8;
+  const cx<double>* sp = spec.as<cx<double>>();
+  const cx<T>* et = exp_table.as<cx<T>>();
+  const cx<T>* rh = rot_half.as<cx<T>>();
+  const cx<T>* ssc = s2sc.as<cx<T>>();
+  const cx<T>* tp = plan_phi->tw();
+  spo_dev::MfmaTables mt{fwd_offset.as<int>(), bwd_offset.as<int>(), site_order.as<int>()};
+  const T* ff = frag_fwd.as<T>();
+  const T* fb = frag_bwd.as<T>();
+  int mj = max_jt;
+  void* args[] = {&ps, const_cast<cx<double>**>(&sp), &s, const_cast<cx<T>**>(&tp), &at, &mt,
+                  const_cast<T**>(&ff), const_cast<T**>(&fb), &mj, const_cast<cx<T>**>(&et),
+                  const_cast<cx<T>**>(&rh), const_cast<cx<T>**>(&ssc)};
+  launch(v.fn, tiles * s.n_xy, v.threads, v.lds, stream, args);
+}
+
+// The launch tuner. The variants of one family differ only in block sizes and tiles; the compiler may contract
+// their multiply-adds differently, so their results can differ in the last bits, and the default is therefore
+// fixed: the first variant of each family (ordered by what measured fastest on MI355X for the shape), which keeps
+// repeated runs bitwise identical. SPO_HIP_TUNE=auto makes the first backend of a process with a given shape and
+// precision time every variant on the (still zero) scattering wave, one warm-up and two timed launches each, and
+// keep the fastest for the process; SPO_HIP_TUNE="zr=384,xy=8,interaction=16" forces variants by thread count
+// (zr), izr per tile (xy) or wavefronts (interaction).
+std::map<std::string, int>& tune_cache() {
+  static std::map<std::string, int> cache;
+  return cache;
+}
+
+// The variant SPO_HIP_TUNE selects for a family, or -1 to time them (SPO_HIP_TUNE=auto).
+int forced_variant(const std::vector<Variant>& v, const char* family, bool by_threads) {
+  const char* env = std::getenv("SPO_HIP_TUNE");
+  if (!env) return 0;
+  const std::string spec = env;
+  if (spec == "auto") return -1;
+  const std::string key = std::string(family) + "=";
+  const std::size_t at = spec.find(key);
+  if (at == std::string::npos) return 0;
+  const int want = std::atoi(spec.c_str() + at + key.size());
+  for (std::size_t i = 0; i < v.size(); ++i)
+    if ((by_threads ? v[i].threads : v[i].param) == want) return static_cast<int>(i);
+  throw Status(SPO_HIP_E_ARGUMENT, std::string("SPO_HIP_TUNE: no ") + family + " variant " + std::to_string(want));
+}
+
+template <typename T>
+void State<T>::tune() {
+  int dev = 0;
+  HIP_CHECK(hipGetDevice(&dev));
+  auto pick = [&](const std::vector<Variant>& v, const char* family, bool by_threads, auto&& run) -> int {
+    if (v.size() < 2) return 0;
+    const int forced = forced_variant(v, family, by_threads);
+    if (forced >= 0) return forced;
+    const std::string key = std::string(family) + ":" + std::to_string(s.nz) + "x" + std::to_string(s.nr) + "x" +
+                            std::to_string(s.nx) + "x" + std::to_string(s.ny) + "x" + std::to_string(s.n_rot) +
+                            "x" + std::to_string(s.n_phi) + ":" + std::to_string(sizeof(T)) + ":" +
+                            std::to_string(dev);
+    const auto hit = tune_cache().find(key);
+    if (hit != tune_cache().end()) return hit->second;
+    hipEvent_t e0, e1;
+    HIP_CHECK(hipEventCreate(&e0));
+    HIP_CHECK(hipEventCreate(&e1));
+    int best = 0;
+    float best_ms = 0.0f;
+    for (std::size_t i = 0; i < v.size(); ++i) {
+      run(v[i]);
+      HIP_CHECK(hipEventRecord(e0, stream));
+      run(v[i]);
+      run(v[i]);
+      HIP_CHECK(hipEventRecord(e1, stream));
+      HIP_CHECK(hipEventSynchronize(e1));
+      float ms = 0.0f;
+      HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+      if (i == 0 || ms < best_ms) {
+        best = static_cast<int>(i);
+        best_ms = ms;
+      }
+    }
+    HIP_CHECK(hipEventDestroy(e0));
+    HIP_CHECK(hipEventDestroy(e1));
+    tune_cache()[key] = best;
+    if (std::getenv("SPO_HIP_TUNE_LOG"))
+      std::fprintf(stderr, "spo_hip tune %s: %s=%d (%.3f ms for two launches)\n", key.c_str(), family,
+                   by_threads ? v[best].threads : v[best].param, best_ms);
+    return best;
+  };
+  // The scattering wave is zero here, and every kernel maps zero to zero.
+  zr_pick = pick(zr_variants, "zr", true, [&](const Variant& v) { run_zr(v, false); });
+  xy_pick = pick(xy_variants, "xy", false, [&](const Variant& v) { run_xy(v, false, false); });
+  v_pick = pick(v_variants, "interaction", false, [&](const Variant& v) { run_interaction_mfma(v); });
+  HIP_CHECK(hipStreamSynchronize(stream));
+}
+
+template <typename T>
 void State<T>::kinetic_and_potential(bool half, bool kinetic_only) {
   specular_step(half, kinetic_only);
   cx<T>* ps = psi.as<cx<T>>();
   // T_Zr.
   const cx<T>* zr = (half ? zr_half : zr_full).as<cx<T>>();
   const int n_planes = s.n_rot * s.n_xy;
-  if (zr_fn) {
-    const cx<T>* tz = plan_z->tw();
-    const cx<T>* tr = plan_r->tw();
-    void* args[] = {&ps, const_cast<cx<T>**>(&tz), const_cast<cx<T>**>(&tr), const_cast<cx<T>**>(&zr)};
-    timed("zr_plane", [&] { launch(zr_fn, n_planes, zr_threads, zr_lds, stream, args); });
+  if (!zr_variants.empty()) {
+    timed("zr_plane", [&] { run_zr(zr_variants[zr_pick], half); });
   } else if (zr_fused) {
     timed("zr_plane_generic", [&] {
       hipLaunchKernelGGL(spo_dev::zr_plane_kernel<T>, dim3(n_planes), dim3(kThreads), std::size_t(s.n_zr) * sizeof(cx<T>),
@@ -420,28 +566,17 @@ void State<T>::kinetic_and_potential(bool half, bool kinetic_only) {
     lines(ps, r_lines, *plan_r, +1, 0, nullptr, 0, 0);
   }
   // T_xy and the diagonal part of V.
-  {
+  if (!xy_variants.empty()) {
+    timed("xy_tile", [&] { run_xy(xy_variants[xy_pick], half, kinetic_only); });
+  } else {
     const int tiles = (s.n_zr + xy_c - 1) / xy_c;
-    const std::size_t lds = xy_lds;
     const cx<T>* factor = (half ? xy_half : xy_full).as<cx<T>>();
-    const cx<T>* sf = shift_f.as<cx<T>>();
-    const cx<T>* sb = shift_b.as<cx<T>>();
-    int needs_shift = shape.needs_shift;
-    int apply_potential = kinetic_only ? 0 : 1;
     const T* zo = z_opt.as<T>();
-    const cx<T>* bf = before.as<cx<T>>();
-    const cx<T>* gs = gas_step.as<cx<T>>();
-    if (xy_reg) {
-      void* args[] = {&ps, &s, const_cast<cx<T>**>(&factor), const_cast<cx<T>**>(&sf), const_cast<cx<T>**>(&sb),
-                      &needs_shift, &apply_potential, const_cast<T**>(&zo), const_cast<cx<T>**>(&bf),
-                      const_cast<cx<T>**>(&gs)};
-      timed("xy_tile", [&] { launch(xy_fn, tiles * s.n_rot, xy_threads, lds, stream, args); });
-    } else {
-      timed("xy_tile_generic", [&] {
-        hipLaunchKernelGGL(spo_dev::xy_kernel<T>, dim3(tiles * s.n_rot), dim3(kThreads), lds, stream, ps, s, xy_c,
-                           plan_x->plan, plan_y->plan, factor, sf, sb, needs_shift, apply_potential, zo, bf, gs);
-      });
-    }
+    timed("xy_tile_generic", [&] {
+      hipLaunchKernelGGL(spo_dev::xy_kernel<T>, dim3(tiles * s.n_rot), dim3(kThreads), xy_lds, stream, ps, s, xy_c,
+                         plan_x->plan, plan_y->plan, factor, shift_f.as<cx<T>>(), shift_b.as<cx<T>>(),
+                         shape.needs_shift, kinetic_only ? 0 : 1, zo, before.as<cx<T>>(), gas_step.as<cx<T>>());
+    });
   }
   if (kinetic_only || s.n_zr_pot == 0) return;
   // V in the interaction region, then the projection onto the specular grid.
@@ -454,20 +589,12 @@ void State<T>::kinetic_and_potential(bool half, bool kinetic_only) {
   const cx<T>* et = exp_table.as<cx<T>>();
   const cx<T>* rh = rot_half.as<cx<T>>();
   const cx<T>* ssc = s2sc.as<cx<T>>();
-  if (v_mfma) {
-    const cx<T>* tp = plan_phi->tw();
-    spo_dev::MfmaTables mt{fwd_offset.as<int>(), bwd_offset.as<int>(), site_order.as<int>()};
-    const T* ff = frag_fwd.as<T>();
-    const T* fb = frag_bwd.as<T>();
-    int mj = max_jt;
-    void* args[] = {&ps, const_cast<cx<double>**>(&sp), &s, const_cast<cx<T>**>(&tp), &at, &mt,
-                    const_cast<T**>(&ff), const_cast<T**>(&fb), &mj, const_cast<cx<T>**>(&et),
-                    const_cast<cx<T>**>(&rh), const_cast<cx<T>**>(&ssc)};
+  if (!v_variants.empty()) {
     timed("specular_coupling", [&] {
       hipLaunchKernelGGL(spo_dev::specular_coupling_kernel<T>, dim3(blocks_for(long(s.n_zr_pot) * s.n_xy, 256)),
                          dim3(256), 0, stream, ps, sp, s, rh, ssc);
     });
-    timed("interaction", [&] { launch(v_mfma, tiles * s.n_xy, 64 * mfma_waves, lds, stream, args); });
+    timed("interaction", [&] { run_interaction_mfma(v_variants[v_pick]); });
   } else if (v_fn) {
     const cx<T>* tp = plan_phi->tw();
     const T* lt = legendre_t.as<T>();
@@ -734,28 +861,25 @@ std::unique_ptr<spo_hip_backend> create_state(const spo_hip_shape* shape, const 
   const std::size_t lds_limit = static_cast<std::size_t>(b->max_lds);
   // T_Zr: the register-pass kernel (padded rows) if instantiated and it fits, else the run-time one, else lines.
   const std::size_t zr_reg_lds = std::size_t(s.nz) * (s.nr + 1) * cT;
-  b->zr_fn = zr_reg_lds <= lds_limit ? zr_reg_symbol<T>(s.nz, s.nr, &b->zr_threads) : nullptr;
-  if (b->zr_fn) {
-    b->zr_lds = zr_reg_lds;
-    set_lds(b->zr_fn, b->zr_lds);
+  if (zr_reg_lds <= lds_limit) b->zr_variants = zr_reg_variants<T>(s.nz, s.nr);
+  if (!b->zr_variants.empty()) {
+    for (const Variant& v : b->zr_variants) set_lds(v.fn, v.lds);
   } else if (std::size_t(s.n_zr) * cT <= lds_limit) {
     b->zr_fused = true;
     b->zr_lds = std::size_t(s.n_zr) * cT;
     set_lds(reinterpret_cast<const void*>(spo_dev::zr_plane_kernel<T>), b->zr_lds);
   }
   // T_xy.
-  int xy_c = 0;
-  b->xy_fn = xy_reg_symbol<T>(s.nx, s.ny, &xy_c, &b->xy_threads);
-  b->xy_reg = b->xy_fn != nullptr;
-  if (b->xy_fn) {
-    b->xy_c = xy_c;
-    b->xy_lds = std::size_t(s.nx) * (s.ny * (xy_c + 1) + 1) * cT;
+  for (const Variant& v : xy_reg_variants<T>(s.nx, s.ny))
+    if (v.lds <= lds_limit) b->xy_variants.push_back(v);
+  if (!b->xy_variants.empty()) {
+    for (const Variant& v : b->xy_variants) set_lds(v.fn, v.lds);
   } else {
     b->xy_c = std::max(1, std::min(64, 4096 / s.n_xy));
     b->xy_lds = std::size_t(b->xy_c) * s.n_xy * cT;
+    if (b->xy_lds > lds_limit) throw Status(SPO_HIP_E_NOT_SUPPORTED, "the (x, y) tile does not fit in LDS");
+    set_lds(reinterpret_cast<const void*>(spo_dev::xy_kernel<T>), b->xy_lds);
   }
-  if (b->xy_lds > lds_limit) throw Status(SPO_HIP_E_NOT_SUPPORTED, "the (x, y) tile does not fit in LDS");
-  set_lds(b->xy_fn ? b->xy_fn : reinterpret_cast<const void*>(spo_dev::xy_kernel<T>), b->xy_lds);
   // Interaction tile: the largest power of two of columns (at most 32) whose LDS fits in 80 KiB (two per CU).
   b->v_fn = s.n_theta <= spo_dev::kMaxTheta ? interaction_ct_symbol<T>(s.n_phi) : nullptr;
   auto v_bytes = [&](int c) {
@@ -776,11 +900,11 @@ std::unique_ptr<spo_hip_backend> create_state(const spo_hip_shape* shape, const 
     const std::string choice = env ? env : "mfma";
     if (choice != "mfma" && choice != "mfma_staged" && choice != "lds")
       throw Status(SPO_HIP_E_ARGUMENT, "SPO_HIP_INTERACTION must be mfma, mfma_staged or lds");
-    int nt = 1, waves = 4;
     const bool fused = choice == "mfma" && s.n_theta <= 16;
-    const void* fn = interaction_mfma_symbol<T>(s.n_phi, fused, &nt, &waves);
-    const std::size_t lds = spo_dev::interaction_mfma_lds<T>(8 * nt, s.n_theta, s.n_phi, fused);
-    if (fn && lds <= lds_limit && choice != "lds") {
+    std::vector<Variant> variants;
+    for (const Variant& v : interaction_mfma_variants<T>(s.n_phi, s.n_theta, fused))
+      if (v.lds <= lds_limit) variants.push_back(v);
+    if (!variants.empty() && choice != "lds") {
       const int n_tt = (s.n_theta + 15) / 16, k_bwd = (s.n_theta + 3) / 4;
       std::vector<int> fo(s.n_blocks), bo(s.n_blocks);
       int nf = 0, nbw = 0, max_nb = 0;
@@ -827,18 +951,16 @@ std::unique_ptr<spo_hip_backend> create_state(const spo_hip_shape* shape, const 
       b->bwd_offset = upload(bo.data(), bo.size() * sizeof(int));
       b->site_order = upload(order.data(), order.size() * sizeof(int));
       b->max_jt = (max_nb + 15) / 16;
-      b->v_mfma = fn;
-      b->mfma_nt = nt;
-      b->mfma_waves = waves;
-      b->v_c = 8 * nt;
-      b->v_lds = lds;
-      set_lds(fn, lds);
+      b->v_variants = variants;
+      b->v_c = 8;
+      for (const Variant& v : variants) set_lds(v.fn, v.lds);
     }
   }
   set_lds(reinterpret_cast<const void*>(spo_dev::lines_fft_kernel<T>), 65536);
   set_lds(reinterpret_cast<const void*>(spo_dev::lines_fft_kernel<double>), 65536);
   set_lds(reinterpret_cast<const void*>(spo_dev::bk_scattering_kernel<T>), std::size_t(s.n_xy) * c16);
   HIP_CHECK(hipStreamSynchronize(b->stream));
+  b->tune();
   return b;
 }
 

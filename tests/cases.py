@@ -3,6 +3,7 @@
 GOLDEN_DIR holds the inputs of the reference's Fortran-gated cases, copied as data together with the extracted
 Fortran outputs (tests/golden/make_fixtures.py). The dictionaries below are this repository's own workloads:
   SMOKE_DAILIGHT  a small Dai-Light H2/Cu(111) grid with full angular coupling (seconds on the CPU oracle);
+  DZ_ANGULAR      the same grid with the DZ workload's angular basis (JMax 24, MJMax 23, C6v);
   HARDWALL_6D     the HardWallPES workload: NZ 128 on Z in [0.5, 20], NR 32 on r in [0.6, 4.5], 8 x 8 (x, y),
                   JMax 10, 2000 steps of 2 a.u.;
   DZ_6D           the Dai-Light H2/Cu(111) LEPS workload: NZ 96 on Z in [-1, 12], NR 48 on r in [0.8, 6],
@@ -48,6 +49,13 @@ SMOKE_DAILIGHT = dict(_COMMON, **{
 SMOKE_GENERIC = dict(SMOKE_DAILIGHT, **{
     "Title": "small Dai-Light grid on shapes served by the run-time-shaped kernels",
     "dZ": 0.14, "nZ": 72, "nX": 6, "nY": 6, "JMax": 6, "MJMax": 6, "ZOptEnd": 8.94,
+})
+
+# The angular basis of the DZ workload (JMax 24, MJMax 23: 323 rotational states, 13 theta x 48 phi, C6v symmetry)
+# on a small spatial grid: the instantiated MFMA interaction shapes at a size the CPU oracle runs in seconds.
+DZ_ANGULAR = dict(SMOKE_DAILIGHT, **{
+    "Title": "small Dai-Light grid with the DZ workload's angular basis",
+    "JMax": 24, "MJMax": 23, "UseSymmetry": True,
 })
 
 HARDWALL_6D = dict(_COMMON, **{

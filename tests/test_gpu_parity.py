@@ -18,6 +18,7 @@ TOL = 1e-12
 CASE_PARAMS = {
     "smoke": lambda: cases.write_case("smoke_dailight", cases.SMOKE_DAILIGHT),
     "generic": lambda: cases.write_case("smoke_generic", cases.SMOKE_GENERIC),
+    "dz_angular": lambda: cases.write_case("dz_angular", cases.DZ_ANGULAR),
     "hardwall_short": lambda: cases.golden_input("hardwall_short"),
     "cu111_dailight_short": lambda: cases.golden_input("cu111_dailight_short"),
     "cu111_dailight_short_sym": lambda: cases.golden_input("cu111_dailight_short_sym"),
@@ -70,7 +71,8 @@ def test_one_t_block_matches_oracle(case, half, kinetic_only):
     assert relative_difference(s0, sc) > 1e-3 and relative_difference(p0, pc) > 1e-3
 
 
-@pytest.mark.parametrize("case", ["smoke", "generic", "cu111_dailight_short_sym", "cu111_dailight_oblique"])
+@pytest.mark.parametrize("case", ["smoke", "generic", "dz_angular", "cu111_dailight_short_sym",
+                                  "cu111_dailight_oblique"])
 def test_analysis_step_matches_oracle(case):
     print(f"case {case} seed 11 analysis step", flush=True)
     gpu, cpu = _pair(case)
@@ -88,7 +90,8 @@ def test_analysis_step_matches_oracle(case):
     assert relative_difference(cpu.site_column(3), gpu.site_column(3)) < TOL
 
 
-@pytest.mark.parametrize("case", ["smoke", "generic", "cu111_dailight_short_sym", "cu111_dailight_oblique"])
+@pytest.mark.parametrize("case", ["smoke", "generic", "dz_angular", "cu111_dailight_short_sym",
+                                  "cu111_dailight_oblique"])
 def test_mixed_precision_t_block_against_fp64_oracle(case):
     """The complex64 storage mode against the FP64 oracle on one full T block from the same wave packet:
     single-precision rounding only (max |a - b| / max |a| < 1e-5)."""
@@ -105,7 +108,7 @@ def test_mixed_precision_t_block_against_fp64_oracle(case):
     assert ds > 1e-9  # the mode really stores complex64
 
 
-@pytest.mark.parametrize("case", ["smoke", "cu111_dailight_short_sym", "cu111_dailight_oblique"])
+@pytest.mark.parametrize("case", ["smoke", "dz_angular", "cu111_dailight_short_sym", "cu111_dailight_oblique"])
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 @pytest.mark.parametrize("interaction", ["lds", "mfma_staged"])
 def test_interaction_variants_match_oracle(case, precision, interaction):
@@ -141,3 +144,34 @@ def test_two_runs_are_bitwise_identical(precision):
         run.close()
     for a, b in zip(*out):
         assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("case,settings", [("dz_angular", ["interaction=4", "interaction=16", "xy=32", "xy=64"]),
+                                           ("hardwall_short", ["zr=256", "zr=512", "xy=32", "xy=64"])])
+def test_launch_variants_match_the_default(case, settings, precision):
+    """The launch variants (block sizes, tile widths) against the default variant after a full T block: the same
+    arithmetic up to the compiler's multiply-add contraction, so within the parity bar (FP64 1e-13, complex64 1e-6
+    relative)."""
+    import os
+    import spodvr_amd
+    path = CASE_PARAMS[case]()
+    out = {}
+    for setting in ["default"] + settings:
+        print(f"case {case} seed 51 {precision} SPO_HIP_TUNE={setting}", flush=True)
+        if setting != "default":
+            os.environ["SPO_HIP_TUNE"] = setting
+        try:
+            run = spodvr_amd.Run(path, {"Precision": precision}, device="hip")
+        finally:
+            os.environ.pop("SPO_HIP_TUNE", None)
+        run.set_wave(pseudo_random(run.scattering_size, 51), pseudo_random(run.specular_size, 52))
+        run.kinetic_and_potential(False, False)
+        out[setting] = run.get_wave()
+        run.close()
+    tol = 1e-13 if precision == "fp64" else 1e-6
+    for setting in settings:
+        ds = relative_difference(out["default"][0], out[setting][0])
+        dp = relative_difference(out["default"][1], out[setting][1])
+        print(f"  {setting}: scattering {ds:.3e} specular {dp:.3e}", flush=True)
+        assert ds < tol and dp < tol, setting
