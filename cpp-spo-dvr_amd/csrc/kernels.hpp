@@ -385,4 +385,35 @@ __global__ void device_to_host_kernel(const cx<T>* psi, Shape s, cx<double>* chu
   chunk[k] = cx<double>{double(v.x), double(v.y)};
 }
 
+// Energy-resolved sums (src/analysis/energy_resolved.cpp:62-91): sums[site][ien][e] += (psi(site)[e] phase[ien])
+// dtAna over the site's column e = izr + n_zr irot (the device layout of one ixy), in FP64.
+template <typename T>
+__global__ __launch_bounds__(256) void energy_resolved_kernel(const cx<T>* psi, long block, const int* site_ixy,
+                                                              int n_energies, const cx<double>* phase, double dt,
+                                                              cx<double>* sums) {
+  const long e = static_cast<long>(blockIdx.x) * 256 + threadIdx.x;
+  if (e >= block) return;
+  const int site = blockIdx.y;
+  const cx<T> p = psi[e + block * site_ixy[site]];
+  const cx<double> v = {double(p.x), double(p.y)};
+  cx<double>* out = sums + static_cast<long>(site) * n_energies * block + e;
+  for (int ien = 0; ien < n_energies; ++ien) {
+    cx<double>& acc = out[static_cast<long>(ien) * block];
+    acc = acc + scale(v * phase[ien], dt);
+  }
+}
+
+// The same for the specular wave: sums[ien][k] += (spec[k] phase[ien]) dtAna over its n values (ir, iz).
+__global__ __launch_bounds__(256) void energy_resolved_specular_kernel(const cx<double>* spec, int n, int n_energies,
+                                                                       const cx<double>* phase, double dt,
+                                                                       cx<double>* sums) {
+  const int k = blockIdx.x * 256 + threadIdx.x;
+  if (k >= n) return;
+  const cx<double> v = spec[k];
+  for (int ien = 0; ien < n_energies; ++ien) {
+    cx<double>& acc = sums[static_cast<long>(ien) * n + k];
+    acc = acc + scale(v * phase[ien], dt);
+  }
+}
+
 }  // namespace spo_dev

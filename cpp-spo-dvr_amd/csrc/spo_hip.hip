@@ -308,6 +308,13 @@ struct spo_hip_backend {
   virtual void block_densities() = 0;
   virtual void site_column(int ixy, double* out) = 0;
   virtual void bk_scattering() = 0;
+  // One analysis step of the energy-resolved sums of the scattering wave (er_phase holds exp(i E t)).
+  virtual void energy_resolved_scattering() = 0;
+
+  // Energy-resolved sums: device layout [site][energy][irot][izr] and [energy][iz][ir].
+  Buffer er_sums, er_specular, er_sites, er_phase;
+  int er_n_sites = 0, er_n_energies = 0;
+  double er_dt = 0.0;
 
   void specular_step(bool half, bool kinetic_only);
   void lines_d(cx<double>* data, spo_dev::Lines L, const DevicePlan<double>& p, int sign, int mode,
@@ -403,6 +410,7 @@ struct State final : spo_hip_backend {
   void block_densities() override;
   void site_column(int ixy, double* out) override;
   void bk_scattering() override;
+  void energy_resolved_scattering() override;
   void lines(cx<T>* data, spo_dev::Lines L, const DevicePlan<T>& p, int sign, int mode, const cx<T>* factor,
              int f_line, int f_elem);
 };
@@ -694,6 +702,16 @@ void State<T>::bk_scattering() {
                        a.n_rotation, a.n_diffraction, a.n_states, rot_index.as<int>(), dif_slot.as<int>(),
                        radial_weight.as<double>(), shift_f_d.as<cx<double>>(), a.xy_scale, plan_x_d->plan,
                        plan_y_d->plan, bk.as<cx<double>>());
+  });
+}
+
+template <typename T>
+void State<T>::energy_resolved_scattering() {
+  const long block = long(s.n_zr) * s.n_rot;
+  timed("energy_resolved", [&] {
+    hipLaunchKernelGGL(spo_dev::energy_resolved_kernel<T>, dim3(blocks_for(block, 256), er_n_sites), dim3(256), 0,
+                       stream, psi.as<cx<T>>(), block, er_sites.as<int>(), er_n_energies,
+                       er_phase.as<cx<double>>(), er_dt, er_sums.as<cx<double>>());
   });
 }
 
@@ -1082,6 +1100,67 @@ int spo_hip_get_s_sums(spo_hip_backend* b, double* out) {
     const std::size_t n = std::size_t(a.n_states) * a.n_v * a.n_energies;
     HIP_CHECK(hipMemcpyAsync(out, b->sums.as<void>(), n * sizeof(cx<double>), hipMemcpyDeviceToHost, b->stream));
     HIP_CHECK(hipStreamSynchronize(b->stream));
+  });
+}
+
+int spo_hip_energy_resolved_start(spo_hip_backend* b, int n_sites, const int* site_ixy, int n_energies,
+                                  double analysis_time_step) {
+  return guarded([&] {
+    if (n_sites < 1 || n_energies < 1 || !site_ixy)
+      throw Status(SPO_HIP_E_ARGUMENT, "spo_hip_energy_resolved_start: no sites or no energies");
+    for (int i = 0; i < n_sites; ++i)
+      if (site_ixy[i] < 0 || site_ixy[i] >= b->s.n_xy) throw Status(SPO_HIP_E_ARGUMENT, "site outside the (x, y) grid");
+    const std::size_t block = std::size_t(b->s.n_zr) * b->s.n_rot;
+    const std::size_t spec_n = std::size_t(b->s.nzsp) * b->s.nr;
+    b->er_sums = Buffer(std::size_t(n_sites) * n_energies * block * sizeof(cx<double>));
+    b->er_specular = Buffer(std::size_t(n_energies) * spec_n * sizeof(cx<double>));
+    b->er_sites = upload(site_ixy, std::size_t(n_sites) * sizeof(int));
+    b->er_phase = Buffer(std::size_t(n_energies) * sizeof(cx<double>));
+    b->er_n_sites = n_sites;
+    b->er_n_energies = n_energies;
+    b->er_dt = analysis_time_step;
+    HIP_CHECK(hipMemsetAsync(b->er_sums.as<void>(), 0, std::size_t(n_sites) * n_energies * block * sizeof(cx<double>),
+                             b->stream));
+    HIP_CHECK(hipMemsetAsync(b->er_specular.as<void>(), 0, std::size_t(n_energies) * spec_n * sizeof(cx<double>),
+                             b->stream));
+    HIP_CHECK(hipStreamSynchronize(b->stream));
+  });
+}
+
+int spo_hip_energy_resolved_add(spo_hip_backend* b, const double* phase_factors) {
+  return guarded([&] {
+    if (b->er_n_sites == 0) throw Status(SPO_HIP_E_ARGUMENT, "spo_hip_energy_resolved_add before _start");
+    HIP_CHECK(hipMemcpyAsync(b->er_phase.as<void>(), phase_factors, std::size_t(b->er_n_energies) * sizeof(cx<double>),
+                             hipMemcpyHostToDevice, b->stream));
+    b->energy_resolved_scattering();
+    const int spec_n = b->s.nzsp * b->s.nr;
+    b->timed("energy_resolved_specular", [&] {
+      hipLaunchKernelGGL(spo_dev::energy_resolved_specular_kernel, dim3(blocks_for(spec_n, 256)), dim3(256), 0,
+                         b->stream, b->spec.as<cx<double>>(), spec_n, b->er_n_energies, b->er_phase.as<cx<double>>(),
+                         b->er_dt, b->er_specular.as<cx<double>>());
+    });
+    HIP_CHECK(hipStreamSynchronize(b->stream));
+  });
+}
+
+int spo_hip_energy_resolved_read(spo_hip_backend* b, double* sums, double* specular_sums) {
+  return guarded([&] {
+    if (b->er_n_sites == 0) throw Status(SPO_HIP_E_ARGUMENT, "spo_hip_energy_resolved_read before _start");
+    const std::size_t n_zr = b->s.n_zr, n_rot = b->s.n_rot, block = n_zr * n_rot;
+    const std::size_t pairs = std::size_t(b->er_n_sites) * b->er_n_energies;
+    std::vector<cx<double>> dev(pairs * block);
+    HIP_CHECK(hipMemcpyAsync(dev.data(), b->er_sums.as<void>(), dev.size() * sizeof(cx<double>), hipMemcpyDeviceToHost,
+                             b->stream));
+    HIP_CHECK(hipMemcpyAsync(specular_sums, b->er_specular.as<void>(),
+                             std::size_t(b->er_n_energies) * b->s.nzsp * b->s.nr * sizeof(cx<double>),
+                             hipMemcpyDeviceToHost, b->stream));
+    HIP_CHECK(hipStreamSynchronize(b->stream));
+    // [pair][irot][izr] -> [pair][izr][irot]
+    cx<double>* out = reinterpret_cast<cx<double>*>(sums);
+    for (std::size_t pr = 0; pr < pairs; ++pr)
+      for (std::size_t irot = 0; irot < n_rot; ++irot)
+        for (std::size_t izr = 0; izr < n_zr; ++izr)
+          out[pr * block + irot + n_rot * izr] = dev[pr * block + izr + n_zr * irot];
   });
 }
 

@@ -157,6 +157,25 @@ AnalysisEnergies make_analysis_energies(const Config& c, const GasPhase& gas) {
   return e;
 }
 
+EnergyResolvedSetup make_energy_resolved_setup(const Config& c, const GasPhase& gas) {
+  EnergyResolvedSetup e;
+  const InputParameters& p = c.input;
+  for (const std::string& name : p.energy_resolved_sites)
+    for (const PlotSite& site : p.plot_sites)
+      if (site.name == name) {
+        e.site_names.push_back(name);
+        e.site_ixy.push_back(plot_site_ixy(site, p.nx, p.ny));
+        break;
+      }
+  for (const double collision : p.energy_resolved_energies) {
+    const double total = collision + Units::hartree_in_ev * gas.initial_energy;
+    e.collision_ev.push_back(collision);
+    e.total_hartree.push_back(total / Units::hartree_in_ev);
+  }
+  e.analysis_time_step = c.derived.steps_per_analysis * p.time_step;
+  return e;
+}
+
 EnergyResult s_matrix_at(const Config& c, const AnalysisBasis& a, const GasPhase& gas, const AnalysisEnergies& energies,
                          const std::vector<cplx>& sums, int ien, std::optional<double> specular_line) {
   const InputParameters& p = c.input;

@@ -65,6 +65,19 @@ struct AnalysisEnergies {
 };
 AnalysisEnergies make_analysis_energies(const Config& config, const GasPhase& gas);
 
+// The energy-resolved sums' sites and energies (EnergyResolvedSites / EnergyResolvedEnergies,
+// src/analysis/energy_resolved.cpp:36-58): each site's grid point, and E_tot = E_coll + E(vin, jin) formed as the
+// S matrix's energies are.
+struct EnergyResolvedSetup {
+  std::vector<std::string> site_names;
+  std::vector<int> site_ixy;
+  std::vector<double> collision_ev;
+  std::vector<double> total_hartree;
+  double analysis_time_step = 0.0;
+  bool enabled() const { return !site_ixy.empty() && !total_hartree.empty(); }
+};
+EnergyResolvedSetup make_energy_resolved_setup(const Config& config, const GasPhase& gas);
+
 struct EnergyResult {
   double total_energy = 0.0, collision_energy = 0.0;
   std::vector<cplx> s_matrix;       // (is, v)

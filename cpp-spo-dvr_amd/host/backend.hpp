@@ -65,6 +65,23 @@ class PropagationBackend {
   // The FP64 sums (is, v, ien).
   virtual std::vector<cplx> s_matrix_sums() = 0;
 
+  // Energy-resolved sums at chosen (x, y) sites (src/analysis/energy_resolved.cpp:62-91): start() zeroes the sums of
+  // the sites and n_energies energies; add() sums one analysis step, s += psi(site) exp(i E_tot t) dtAna and
+  // sigma += chi exp(i E_tot t) dtAna, given exp(i E_tot t) per energy; read() returns them in the reference's
+  // layouts, [site, energy, Z, r, rot] (rot fastest) and [energy, Z_specular, r] (r fastest), in FP64.
+  virtual void energy_resolved_start(const std::vector<int>& site_ixy, int n_energies, double analysis_time_step) {
+    (void)site_ixy, (void)n_energies, (void)analysis_time_step;
+    throw NotSupportedError("this backend has no energy-resolved sums");
+  }
+  virtual void energy_resolved_add(const std::vector<cplx>& phase_factors) {
+    (void)phase_factors;
+    throw NotSupportedError("this backend has no energy-resolved sums");
+  }
+  virtual void energy_resolved_read(std::vector<cplx>* sums, std::vector<cplx>* specular_sums) {
+    (void)sums, (void)specular_sums;
+    throw NotSupportedError("this backend has no energy-resolved sums");
+  }
+
   // Per-kernel device timing (a backend without kernels reports nothing).
   struct KernelTime {
     std::string name;

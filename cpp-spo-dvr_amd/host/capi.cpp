@@ -228,6 +228,23 @@ int spo_run_s_sums(const spo_run* run, double* out) {
   return guarded([&] { copy_complex(run->propagation->backend().s_matrix_sums(), out); });
 }
 
+int spo_run_energy_resolved_shape(const spo_run* run, int* n_sites, int* n_energies) {
+  return guarded([&] {
+    const spo::EnergyResolvedSetup& e = run->propagation->energy_resolved_setup();
+    *n_sites = static_cast<int>(e.site_ixy.size());
+    *n_energies = static_cast<int>(e.total_hartree.size());
+  });
+}
+
+int spo_run_energy_resolved(spo_run* run, double* sums, double* specular_sums) {
+  return guarded([&] {
+    std::vector<spo::cplx> a, b;
+    run->propagation->energy_resolved(&a, &b);
+    copy_complex(a, sums);
+    copy_complex(b, specular_sums);
+  });
+}
+
 int spo_run_array(const spo_run* run, const char* name, double* out, size_t capacity, size_t* count) {
   return guarded([&] {
     const spo::RunSetup& s = *run->setup;

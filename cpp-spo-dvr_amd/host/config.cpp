@@ -5,6 +5,7 @@
 #include <array>
 #include <cmath>
 #include <numeric>
+#include <tuple>
 #include <utility>
 
 namespace spo {
@@ -189,7 +190,46 @@ InputParameters read_input(const json::Value& d) {
   get(d, "ZSpOptLowFactor", p.zsp_opt_low.factor);
   get(d, "TolSp", p.tol_specular);
   get(d, "Precision", p.precision);
+  if (const auto* v = d.find("PlotSites")) {
+    if (v->kind != json::Value::Kind::Array) throw InputError("PlotSites", "expected a list of sites");
+    for (const json::Value& item : v->items) {
+      if (item.kind != json::Value::Kind::Object) throw InputError("PlotSites", "each site must be an object");
+      for (const auto& [key, value] : item.members)
+        if (key != "name" && key != "x" && key != "y") throw InputError("PlotSites", "unknown site field '" + key + "'");
+      PlotSite site;
+      const json::Value* name = item.find("name");
+      const json::Value* x = item.find("x");
+      const json::Value* y = item.find("y");
+      if (!name || !x || !y) throw InputError("PlotSites", "a site needs a name, x and y");
+      if (name->kind != json::Value::Kind::String || x->kind != json::Value::Kind::Number ||
+          y->kind != json::Value::Kind::Number)
+        throw InputError("PlotSites", "a site's name is a string, x and y are numbers");
+      site.name = name->text;
+      site.x = x->number;
+      site.y = y->number;
+      p.plot_sites.push_back(site);
+    }
+  }
+  if (const auto* v = d.find("EnergyResolvedSites")) {
+    if (v->kind != json::Value::Kind::Array) throw InputError("EnergyResolvedSites", "expected a list of site names");
+    for (const json::Value& item : v->items) {
+      if (item.kind != json::Value::Kind::String) throw InputError("EnergyResolvedSites", "expected site names");
+      p.energy_resolved_sites.push_back(item.text);
+    }
+  }
+  if (const auto* v = d.find("EnergyResolvedEnergies")) {
+    if (v->kind != json::Value::Kind::Array) throw InputError("EnergyResolvedEnergies", "expected a list of energies");
+    for (const json::Value& item : v->items) {
+      if (item.kind != json::Value::Kind::Number) throw InputError("EnergyResolvedEnergies", "expected numbers (eV)");
+      p.energy_resolved_energies.push_back(item.number);
+    }
+  }
   return p;
+}
+
+int plot_site_ixy(const PlotSite& site, int nx, int ny) {
+  auto nearest = [](double fraction, int n) { return static_cast<int>(std::lround(fraction * n)) % n; };
+  return nearest(site.y, ny) + ny * nearest(site.x, nx);
 }
 
 InputParameters read_input_file(const std::string& path) { return read_input(json::parse_file(path)); }
@@ -206,6 +246,31 @@ Config build_config(const InputParameters& input) {
   if (p.nr < 2) throw InputError("nR", "nR = 1 is not supported");
   if (p.nz_specular < p.nz) throw InputError("nZSpecular", "nZSpecular < nZ");
   if (p.nx < 2 || p.ny < 2) throw NotSupportedError("nX = 1 or nY = 1 is not supported");
+  // src/input/validate_outputs.cpp:28-41, :87-110: sites on the grid, energy-resolved keys together and named.
+  for (const PlotSite& site : p.plot_sites)
+    for (const auto& [axis, fraction, n] : {std::tuple<const char*, double, int>{"x", site.x, p.nx},
+                                            std::tuple<const char*, double, int>{"y", site.y, p.ny}}) {
+      if (!(fraction >= 0.0 && fraction < 1.0))
+        throw InputError("PlotSites", "site '" + site.name + "': " + axis + " is not in [0, 1)");
+      const double scaled = fraction * n, nearest = std::round(scaled);
+      if (!(std::abs(scaled - nearest) <= 1e-6)) throw InputError("PlotSites", "site '" + site.name + "' is off the grid");
+      if (nearest == n) throw InputError("PlotSites", "site '" + site.name + "' rounds to 1: give 0");
+    }
+  if (p.energy_resolved_sites.empty() != p.energy_resolved_energies.empty())
+    throw InputError(p.energy_resolved_sites.empty() ? "EnergyResolvedEnergies" : "EnergyResolvedSites",
+                     "EnergyResolvedSites and EnergyResolvedEnergies go together");
+  for (std::size_t i = 0; i < p.energy_resolved_sites.size(); ++i) {
+    const std::string& name = p.energy_resolved_sites[i];
+    bool found = false;
+    for (const PlotSite& site : p.plot_sites) found = found || site.name == name;
+    if (!found) throw InputError("EnergyResolvedSites", "'" + name + "' is not a site of PlotSites");
+    for (std::size_t k = 0; k < i; ++k)
+      if (p.energy_resolved_sites[k] == name) throw InputError("EnergyResolvedSites", "'" + name + "' is listed twice");
+  }
+  for (const double e : p.energy_resolved_energies)
+    if (tol::lt(e, p.e_min_wp + p.e_parallel) || tol::gt(e, p.e_max_wp + p.e_parallel))
+      throw InputError("EnergyResolvedEnergies", "an energy lies outside the packet's collision energies "
+                                                 "(EMinWP, EMaxWP + EParallel)");
   if (p.precision != "fp64" && p.precision != "mixed")
     throw InputError("Precision", "Precision must be \"fp64\" or \"mixed\", not \"" + p.precision + "\"");
   // src/input/validate.cpp:48-66: every FFT grid must have a length 2^h 3^i 5^j 7^k 11^m (h >= 1, i <= 2).

@@ -4,6 +4,7 @@
 
 #include <optional>
 #include <string>
+#include <vector>
 
 #include "common.hpp"
 #include "json.hpp"
@@ -18,6 +19,13 @@ struct SystemParameters {
   double z_asymptotic = 0.0;
   double skew = 0.0;        // rad, angle between the lattice vectors
   std::string point_group;  // "C2v", "C6v", ...
+};
+
+// A named (x, y) point of the surface cell for the per-site outputs, as fractions of the lattice vectors
+// (src/input/config.hpp:53-57).
+struct PlotSite {
+  std::string name;
+  double x = 0.0, y = 0.0;
 };
 
 // A low-end absorber, on when both ends are given.
@@ -62,6 +70,10 @@ struct InputParameters {
   std::optional<double> zsp_opt_energy, zsp_opt_factor;
   LowAbsorber z_opt_low, zsp_opt_low;
   double tol_specular = 0.0;
+  // Energy-resolved sums at chosen sites (PlotSites, EnergyResolvedSites, EnergyResolvedEnergies in eV).
+  std::vector<PlotSite> plot_sites;
+  std::vector<std::string> energy_resolved_sites;
+  std::vector<double> energy_resolved_energies;
   // Storage precision of the device backend's scattering wave ("fp64", or "mixed": complex64 with FP64 phases,
   // specular grid and analysis). Not a key of the reference program.
   std::string precision = "fp64";
@@ -107,6 +119,10 @@ InputParameters read_input_file(const std::string& path);
 
 // Derived parameters (src/input/process.cpp:37-139) and the checks that keep the propagation well defined.
 Config build_config(const InputParameters& input);
+
+// The (x, y) grid point of a site, ixy = iy + ny * ix, the nearest grid point to each fraction
+// (src/input/validate_outputs.cpp:25, :134-136).
+int plot_site_ixy(const PlotSite& site, int nx, int ny);
 
 // Optimal strength of a quadratic absorber of the given length for kinetic energy ekin (hartree), from the
 // Vibok / Balint-Kurti table (src/input/wave_packet_params.cpp:14-57).

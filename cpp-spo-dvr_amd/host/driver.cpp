@@ -36,6 +36,19 @@ Propagation::Propagation(std::shared_ptr<const RunSetup> setup, const BackendFac
       surface_(setup_->config, setup_->grids, setup_->initial_specular) {
   const InputParameters& p = setup_->config.input;
   record_averages(p.t_end - p.n_steps * p.time_step);
+  energy_resolved_ = make_energy_resolved_setup(setup_->config, setup_->gas);
+  if (energy_resolved_.enabled())
+    backend_->energy_resolved_start(energy_resolved_.site_ixy, static_cast<int>(energy_resolved_.total_hartree.size()),
+                                    energy_resolved_.analysis_time_step);
+}
+
+void Propagation::energy_resolved(std::vector<cplx>* sums, std::vector<cplx>* specular_sums) {
+  if (!energy_resolved_.enabled()) {
+    sums->clear();
+    specular_sums->clear();
+    return;
+  }
+  backend_->energy_resolved_read(sums, specular_sums);
 }
 
 int Propagation::analysis_steps_total() const {
@@ -61,6 +74,13 @@ void Propagation::step() {
   std::vector<cplx> bk;
   backend_->analyse(surface_, t, keep_bk_ ? &bk : nullptr);
   if (keep_bk_) bk_.push_back(std::move(bk));
+  if (energy_resolved_.enabled()) {
+    // exp(i E_tot t) per energy, formed as the reference forms it (src/analysis/energy_resolved.cpp:67-68).
+    std::vector<cplx> phase(energy_resolved_.total_hartree.size());
+    for (std::size_t ien = 0; ien < phase.size(); ++ien)
+      phase[ien] = std::exp(cplx(0.0, 1.0) * energy_resolved_.total_hartree[ien] * t);
+    backend_->energy_resolved_add(phase);
+  }
   if (done_ % setup_->config.derived.analyses_per_average == 0) record_averages(t);
 }
 

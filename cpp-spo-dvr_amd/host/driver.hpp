@@ -53,6 +53,11 @@ class Propagation {
   std::vector<EnergyResult> results(std::optional<double> specular_line) const;
   std::vector<EnergyResult> results() const { return results(surface_.fixed_line()); }
 
+  // The energy-resolved sums so far (empty when EnergyResolvedSites is not set): [site, energy, Z, r, rot] and
+  // [energy, Z_specular, r].
+  const EnergyResolvedSetup& energy_resolved_setup() const { return energy_resolved_; }
+  void energy_resolved(std::vector<cplx>* sums, std::vector<cplx>* specular_sums);
+
  private:
   void record_averages(double time);
 
@@ -64,6 +69,7 @@ class Propagation {
   std::vector<Averages> averages_;
   std::vector<std::vector<cplx>> bk_;
   std::optional<double> fixed_time_;
+  EnergyResolvedSetup energy_resolved_;
 };
 
 }  // namespace spo

@@ -56,6 +56,21 @@ class HipBackend final : public PropagationBackend {
   void site_column(int ixy, cplx* out) const override {
     check(spo_hip_site_column(handle_, ixy, reinterpret_cast<double*>(out)));
   }
+  void energy_resolved_start(const std::vector<int>& site_ixy, int n_energies, double analysis_time_step) override {
+    check(spo_hip_energy_resolved_start(handle_, static_cast<int>(site_ixy.size()), site_ixy.data(), n_energies,
+                                        analysis_time_step));
+    er_sites_ = static_cast<int>(site_ixy.size());
+    er_energies_ = n_energies;
+  }
+  void energy_resolved_add(const std::vector<cplx>& phase_factors) override {
+    check(spo_hip_energy_resolved_add(handle_, reinterpret_cast<const double*>(phase_factors.data())));
+  }
+  void energy_resolved_read(std::vector<cplx>* sums, std::vector<cplx>* specular_sums) override {
+    sums->assign(std::size_t(er_sites_) * er_energies_ * host_.n_zr * host_.n_rot, cplx(0.0, 0.0));
+    specular_sums->assign(std::size_t(er_energies_) * specular_.size(), cplx(0.0, 0.0));
+    check(spo_hip_energy_resolved_read(handle_, reinterpret_cast<double*>(sums->data()),
+                                       reinterpret_cast<double*>(specular_sums->data())));
+  }
   void analyse(const AnalysisSurface& surface, double time, std::vector<cplx>* coefficients) override {
     double* out = nullptr;
     if (coefficients) {
@@ -86,6 +101,7 @@ class HipBackend final : public PropagationBackend {
   std::vector<cplx> specular_;
   mutable std::vector<cplx> plane_;
   std::size_t n_coefficients_ = 0, n_sums_ = 0;
+  int er_sites_ = 0, er_energies_ = 0;
 };
 
 HipBackend::HipBackend(const BackendInput& in) {

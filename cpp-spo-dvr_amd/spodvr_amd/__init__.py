@@ -85,6 +85,9 @@ def host_library() -> ctypes.CDLL:
         lib.spo_run_results.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
                                         ctypes.c_void_p, ctypes.c_void_p]
         lib.spo_run_s_sums.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+        lib.spo_run_energy_resolved_shape.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int),
+                                                      ctypes.POINTER(ctypes.c_int)]
+        lib.spo_run_energy_resolved.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
         lib.spo_run_array.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_void_p, ctypes.c_size_t,
                                       ctypes.POINTER(ctypes.c_size_t)]
         lib.spo_pes_value.argtypes = [ctypes.c_char_p] + [ctypes.c_double] * 6 + [ctypes.POINTER(ctypes.c_double)]
@@ -250,6 +253,19 @@ class Run:
         te = np.zeros(ne, dtype=np.float64)
         _check(host_library().spo_run_results(self._h, int(zana_convention), _ptr(s), _ptr(p), _ptr(pr), _ptr(te)))
         return {"s": s, "p": p, "p_reaction": pr, "total_energy": te}
+
+    def energy_resolved(self):
+        """(sums[site, energy, Z, r, rot], specular[energy, Z_specular, r]) of the energy-resolved sites, or None
+        when EnergyResolvedSites is not set."""
+        ns, ne = ctypes.c_int(0), ctypes.c_int(0)
+        _check(host_library().spo_run_energy_resolved_shape(self._h, ctypes.byref(ns), ctypes.byref(ne)))
+        if ns.value == 0:
+            return None
+        i = self.info
+        sums = np.zeros((ns.value, ne.value, i.nz, i.nr, i.n_rot), dtype=np.complex128)
+        spec = np.zeros((ne.value, i.nz_specular, i.nr), dtype=np.complex128)
+        _check(host_library().spo_run_energy_resolved(self._h, _ptr(sums), _ptr(spec)))
+        return sums, spec
 
     def s_sums(self) -> np.ndarray:
         out = np.zeros(self.info.n_energies * self.info.n_states * self.info.n_v, dtype=np.complex128)

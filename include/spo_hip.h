@@ -138,6 +138,17 @@ int spo_hip_analyse(spo_hip_backend* backend, int surface_fixed, int iz_specular
  * (src/analysis/s_matrix.cpp:70-176), which stay on the host in FP64. */
 int spo_hip_get_s_sums(spo_hip_backend* backend, double* out);
 
+/* Energy-resolved sums at chosen (x, y) sites, replacing EnergyResolvedAccumulator
+ * (src/analysis/energy_resolved.hpp:52-68, energy_resolved.cpp:62-91) with sums kept on the device: start zeroes
+ * the FP64 sums of n_sites sites (site_ixy = iy + ny * ix) and n_energies energies; add sums one analysis step,
+ * s += psi(site) exp(i E_tot t) dtAna and sigma += chi exp(i E_tot t) dtAna, given exp(i E_tot t) per energy
+ * (n_energies complex); read returns them in the reference's layouts: sums [site, energy, Z, r, rot] (rot
+ * fastest, n_sites * n_energies * nz * nr * n_rot complex) and specular_sums [energy, Z_specular, r] (r fastest). */
+int spo_hip_energy_resolved_start(spo_hip_backend* backend, int n_sites, const int* site_ixy, int n_energies,
+                                  double analysis_time_step);
+int spo_hip_energy_resolved_add(spo_hip_backend* backend, const double* phase_factors);
+int spo_hip_energy_resolved_read(spo_hip_backend* backend, double* sums, double* specular_sums);
+
 /* Kernel timing with HIP events recorded on the backend's stream around every launch (no counterpart in the
  * reference; for the roofline measurement). Enabling clears the record. */
 typedef struct spo_hip_kernel_time {

@@ -87,6 +87,12 @@ int spo_run_results(const spo_run* run, int zana_convention, double* s, double* 
 /* The raw FP64 sums (ien, v, is), n_energies*n_states*n_v complex. */
 int spo_run_s_sums(const spo_run* run, double* out);
 
+/* Energy-resolved sums (EnergyResolvedSites / EnergyResolvedEnergies, src/analysis/energy_resolved.hpp:16-43): the
+ * number of sites and energies (0 when the keys are not set), and the sums so far in the reference's layouts,
+ * [site, energy, Z, r, rot] (rot fastest) and [energy, Z_specular, r] (r fastest). */
+int spo_run_energy_resolved_shape(const spo_run* run, int* n_sites, int* n_energies);
+int spo_run_energy_resolved(spo_run* run, double* sums, double* specular_sums);
+
 /* Set-up arrays by name, for checks of the set-up: "gas.energies" (j + (jmax+1) v), "gas.wave_functions"
  * (ir, j, v), "potential.values" (itp, irep, izr), "potential.asymptotic", "angular.thetas", "angular.weights",
  * "angular.phis", "basis.j", "basis.mj", "state.j", "state.mj", "state.n", "state.m", "state.order",

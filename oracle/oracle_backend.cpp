@@ -66,6 +66,15 @@ class OracleBackend final : public PropagationBackend {
   void analyse(const spo::AnalysisSurface& surface, double time, std::vector<cplx>* coefficients) override;
   std::vector<cplx> s_matrix_sums() override { return sums_; }
 
+  // Energy-resolved sums as EnergyResolvedAccumulator (src/analysis/energy_resolved.cpp:59-91): each site's column
+  // read through site_column, [site, energy, izr, irot] and [energy, iz_specular, ir].
+  void energy_resolved_start(const std::vector<int>& site_ixy, int n_energies, double analysis_time_step) override;
+  void energy_resolved_add(const std::vector<cplx>& phase_factors) override;
+  void energy_resolved_read(std::vector<cplx>* sums, std::vector<cplx>* specular_sums) override {
+    *sums = er_sums_;
+    *specular_sums = er_specular_;
+  }
+
   // One T block (specular, T_Zr, T_xy, V unless kinetic_only) with full or half kinetic factors.
   void kinetic_and_potential(bool half, bool kinetic_only) override;
 
@@ -88,6 +97,10 @@ class OracleBackend final : public PropagationBackend {
   std::vector<cplx> spec_gas_;
   std::vector<cplx> sums_;
   mutable std::vector<cplx> plane_;
+  std::vector<int> er_sites_;
+  int er_energies_ = 0;
+  double er_dt_ = 0.0;
+  std::vector<cplx> er_sums_, er_specular_, er_column_;
 };
 
 OracleBackend::OracleBackend(const BackendInput& in)
@@ -349,6 +362,37 @@ std::vector<double> OracleBackend::block_densities() const {
     d[izr] = s;
   }
   return d;
+}
+
+void OracleBackend::energy_resolved_start(const std::vector<int>& site_ixy, int n_energies,
+                                          double analysis_time_step) {
+  er_sites_ = site_ixy;
+  er_energies_ = n_energies;
+  er_dt_ = analysis_time_step;
+  const std::size_t block = std::size_t(n_zr_) * n_rot_;
+  er_sums_.assign(site_ixy.size() * n_energies * block, cplx(0.0, 0.0));
+  er_specular_.assign(std::size_t(n_energies) * nzsp_ * nr_, cplx(0.0, 0.0));
+  er_column_.assign(block, cplx(0.0, 0.0));
+}
+
+void OracleBackend::energy_resolved_add(const std::vector<cplx>& phase) {
+  const std::size_t block = std::size_t(n_zr_) * n_rot_;
+  for (std::size_t isite = 0; isite < er_sites_.size(); ++isite) {
+    site_column(er_sites_[isite], er_column_.data());  // (izr, irot), izr fastest
+    cplx* site = er_sums_.data() + isite * er_energies_ * block;
+#pragma omp parallel for schedule(static)
+    for (int izr = 0; izr < n_zr_; ++izr)
+      for (int ien = 0; ien < er_energies_; ++ien) {
+        cplx* b = site + ien * block;
+        for (int irot = 0; irot < n_rot_; ++irot)
+          b[irot + std::size_t(n_rot_) * izr] =
+              b[irot + std::size_t(n_rot_) * izr] + er_column_[izr + std::size_t(n_zr_) * irot] * phase[ien] * er_dt_;
+      }
+  }
+  const std::size_t spec_n = std::size_t(nzsp_) * nr_;
+  for (int ien = 0; ien < er_energies_; ++ien)
+    for (std::size_t k = 0; k < spec_n; ++k)
+      er_specular_[ien * spec_n + k] = er_specular_[ien * spec_n + k] + psi_.specular[k] * phase[ien] * er_dt_;
 }
 
 void OracleBackend::site_column(int ixy, cplx* out) const {

@@ -175,3 +175,21 @@ def test_launch_variants_match_the_default(case, settings, precision):
         dp = relative_difference(out["default"][1], out[setting][1])
         print(f"  {setting}: scattering {ds:.3e} specular {dp:.3e}", flush=True)
         assert ds < tol and dp < tol, setting
+
+
+@pytest.mark.parametrize("case", ["smoke", "dz_angular"])
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_energy_resolved_sums_match_oracle(case, precision):
+    """The device-side energy-resolved sums (two sites, two energies) against the oracle's accumulator after three
+    analysis steps from the same wave packet."""
+    from tests.test_energy_resolved import SITES
+    print(f"case {case} seed 71 energy-resolved {precision}", flush=True)
+    gpu, cpu = _pair(case, dict(SITES, Precision=precision))
+    _randomise(gpu, cpu, 71)
+    gpu.steps(3)
+    cpu.steps(3)
+    (sg, pg), (sc, pc) = gpu.energy_resolved(), cpu.energy_resolved()
+    ds, dp = relative_difference(sc, sg), relative_difference(pc, pg)
+    print(f"  sums {ds:.3e} specular {dp:.3e}", flush=True)
+    tol = TOL if precision == "fp64" else 1e-5
+    assert ds < tol and dp < tol
