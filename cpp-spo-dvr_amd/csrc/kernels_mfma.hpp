@@ -60,8 +60,25 @@ struct Mfma<float> {
 struct MfmaTables {
   const int* fwd_offset;
   const int* bwd_offset;
-  const int* site_order;  // blockIdx % n_xy -> ixy
+  const int* site_order;  // position -> ixy, the sites of one symmetry representative at positions of one XCD
+  int pair_tiles;         // see block_tile_site
 };
+
+// The (izr tile, site) of a workgroup. Without pairing: tile-major over the site order (blockIdx = tile * n_xy +
+// position). With pairing (n_xy a multiple of 8): the workgroups of tiles 2k and 2k + 1 of a site are 8 apart in
+// blockIdx, so they run on the same XCD at about the same time and their half-line (64-byte) complex64 rows
+// combine in its L2 before they reach HBM.
+__device__ __forceinline__ void block_tile_site(int b, int n_xy, const MfmaTables& mt, int& tile, int& ixy) {
+  if (mt.pair_tiles) {
+    const int x = b % 8, q = b / 8, per = n_xy / 8;
+    const int rest = q / 2;
+    tile = 2 * (rest / per) + q % 2;
+    ixy = mt.site_order[8 * (rest % per) + x];
+  } else {
+    tile = b / n_xy;
+    ixy = mt.site_order[b % n_xy];
+  }
+}
 
 // Position of DVR element (row, phi) in the tile.
 template <int NPHI, bool SWZ>
@@ -187,9 +204,10 @@ __global__ __launch_bounds__(64 * WAVES) void interaction_mfma_kernel(
   T* dvr_t = reinterpret_cast<T*>(dvr);
 
   const int n_theta = s.n_theta;
-  const int tile = blockIdx.x / s.n_xy;
-  const int ixy = mt.site_order[blockIdx.x % s.n_xy];
+  int tile, ixy;
+  block_tile_site(blockIdx.x, s.n_xy, mt, tile, ixy);
   const int izr0 = tile * C;
+  if (izr0 >= s.n_zr_pot) return;  // the padding workgroup of an odd tile count (whole workgroup, no barrier yet)
   const int cw = min(C, s.n_zr_pot - izr0);
   // This site's column of psi (n_zr * n_rot values, so 32-bit offsets within it) as components.
   T* site_t = reinterpret_cast<T*>(psi + static_cast<long>(s.n_zr) * s.n_rot * ixy);

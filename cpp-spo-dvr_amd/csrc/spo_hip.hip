@@ -462,19 +462,21 @@ void State<T>::run_interaction_mfma(const Variant& v) {
                             block_offset.as<int>(), rot_block.as<int>(),   rep_of.as<int>(),
                             op_of.as<int>(),        phi_index.as<int>()};
   const int tiles = (s.n_zr_pot + 7) / 8;
+  const int pair = s.n_xy % 8 == 0 ? 1 : 0;
   const cx<double>* sp = spec.as<cx<double>>();
   const cx<T>* et = exp_table.as<cx<T>>();
   const cx<T>* rh = rot_half.as<cx<T>>();
   const cx<T>* ssc = s2sc.as<cx<T>>();
   const cx<T>* tp = plan_phi->tw();
-  spo_dev::MfmaTables mt{fwd_offset.as<int>(), bwd_offset.as<int>(), site_order.as<int>()};
+  spo_dev::MfmaTables mt{fwd_offset.as<int>(), bwd_offset.as<int>(), site_order.as<int>(), pair};
   const T* ff = frag_fwd.as<T>();
   const T* fb = frag_bwd.as<T>();
   int mj = max_jt;
   void* args[] = {&ps, const_cast<cx<double>**>(&sp), &s, const_cast<cx<T>**>(&tp), &at, &mt,
                   const_cast<T**>(&ff), const_cast<T**>(&fb), &mj, const_cast<cx<T>**>(&et),
                   const_cast<cx<T>**>(&rh), const_cast<cx<T>**>(&ssc)};
-  launch(v.fn, tiles * s.n_xy, v.threads, v.lds, stream, args);
+  const int grid = pair ? 2 * ((tiles + 1) / 2) * s.n_xy : tiles * s.n_xy;
+  launch(v.fn, grid, v.threads, v.lds, stream, args);
 }
 
 // The launch tuner. The variants of one family differ only in block sizes and tiles; the compiler may contract
