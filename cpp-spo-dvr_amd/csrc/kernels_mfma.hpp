@@ -53,6 +53,18 @@ struct Mfma<float> {
   static __device__ __forceinline__ int row(int lane, int v) { return 4 * (lane / 16) + v; }
 };
 
+// The value of the neighbouring lane (lane ^ 1) through a DPP quad permutation [1, 0, 3, 2] (a VALU operand
+// modifier, no LDS traffic); the real and imaginary parts of a column sit in adjacent lanes.
+__device__ __forceinline__ float swap_adjacent_lanes(float v) {
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0xB1, 0xF, 0xF, false));
+}
+__device__ __forceinline__ double swap_adjacent_lanes(double v) {
+  const long long bits = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_mov_dpp(static_cast<int>(bits), 0xB1, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_mov_dpp(static_cast<int>(bits >> 32), 0xB1, 0xF, 0xF, false);
+  return __longlong_as_double((static_cast<long long>(hi) << 32) | static_cast<unsigned int>(lo));
+}
+
 // Pre-arranged Legendre fragments (built on the host), 64 values per fragment in lane order:
 //   forward  [fwd_offset[b] + tt * k_fwd(b) + k]: lane (i, kk) holds L_m[theta = 16 tt + i][J = 4 k + kk]
 //   backward [bwd_offset[b] + jt * k_bwd + k]:    lane (i, kk) holds L_m[theta = 4 k + kk][J = 16 jt + i]
@@ -268,28 +280,39 @@ __global__ __launch_bounds__(64 * WAVES) void interaction_mfma_kernel(
         for (int k = 0; k < 4; ++k)
           bv[i][nt][k] = (live_of[nt] && 4 * k + kk < nb) ? src[lane_b[nt] + 8 * s.n_zr * k] : T(0);
     }
+    // The A fragments of task i + 1 are loaded while the products of task i run.
+    auto load_frags = [&](int bb, T (&a)[4]) {
+      const int k_steps = bb < s.n_blocks ? (at.block_size[bb] + 3) / 4 : 0;
+      const T* frag = frag_fwd + 64L * (bb < s.n_blocks ? mt.fwd_offset[bb] : 0) + lane;
+#pragma unroll
+      for (int k = 0; k < 4; ++k) a[k] = k < k_steps ? frag[64 * k] : T(0);
+    };
+    T fa[4];
+    load_frags(wave, fa);
 #pragma unroll
     for (int i = 0; i < kPrefetchTasks; ++i) {
       const int bb = wave + i * WAVES;
       if (bb >= s.n_blocks) break;
+      T next[4];
+      load_frags(bb + WAVES, next);
       const int nb = at.block_size[bb];
       const int m = bb - s.mj_max;
       const int slot = m >= 0 ? m : NPHI + m;
       const int k_steps = (nb + 3) / 4;
-      const T* frag = frag_fwd + 64L * mt.fwd_offset[bb] + lane;
       acc_t acc[NT] = {};
 #pragma unroll
       for (int k = 0; k < 4; ++k)
         if (k < k_steps) {
-          const T a = frag[64 * k];
 #pragma unroll
-          for (int nt = 0; nt < NT; ++nt) acc[nt] = M::fma(a, bv[i][nt][k], acc[nt]);
+          for (int nt = 0; nt < NT; ++nt) acc[nt] = M::fma(fa[k], bv[i][nt][k], acc[nt]);
         }
 #pragma unroll
       for (int nt = 0; nt < NT; ++nt)
 #pragma unroll
         for (int v = 0; v < 4; ++v)
           if (M::row(lane, v) < n_theta) dvr_t[dvr_off[nt][v] + 2 * (slot ^ dvr_swz[nt][v])] = acc[nt][v];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) fa[k] = next[k];
     }
   } else {
     for (int task = wave; task < n_tasks_f; task += WAVES) {
@@ -361,30 +384,43 @@ __global__ __launch_bounds__(64 * WAVES) void interaction_mfma_kernel(
       const int f0 = at.block_first[bb], nb = at.block_size[bb];
       const int m = bb - s.mj_max;
       const int slot = m >= 0 ? m : NPHI + m;
+      // Every HBM / L2 operand of the task first (A fragments, rotation factors), then the LDS operands.
       const T* frag = frag_bwd + 64L * mt.bwd_offset[bb] + lane;
+      T a[4];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) a[k] = k < k_bwd ? frag[64 * k] : T(0);
+      cx<T> rot[NT][4];
+#pragma unroll
+      for (int nt = 0; nt < NT; ++nt)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int j = M::row(lane, v);
+          rot[nt][v] = j < nb ? rot_half[f0 + j + s.n_rot * ir_of[nt]] : cx<T>{T(0), T(0)};
+        }
+      T b[NT][4];
+#pragma unroll
+      for (int nt = 0; nt < NT; ++nt)
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+          b[nt][k] = (k < k_bwd && 4 * k + kk < n_theta) ? dvr_t[b_off[nt][k] + 2 * (slot ^ b_swz[nt][k])] : T(0);
       acc_t acc[NT] = {};
 #pragma unroll
       for (int k = 0; k < 4; ++k)
         if (k < k_bwd) {
-          const T a = frag[64 * k];
 #pragma unroll
-          for (int nt = 0; nt < NT; ++nt) {
-            const T b = 4 * k + kk < n_theta ? dvr_t[b_off[nt][k] + 2 * (slot ^ b_swz[nt][k])] : T(0);
-            acc[nt] = M::fma(a, b, acc[nt]);
-          }
+          for (int nt = 0; nt < NT; ++nt) acc[nt] = M::fma(a[k], b[nt][k], acc[nt]);
         }
       T* dst = site_t + 2L * s.n_zr * f0;
 #pragma unroll
       for (int nt = 0; nt < NT; ++nt) {
-        const cx<T>* rot = rot_half + f0 + s.n_rot * ir_of[nt];
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
           const T mine = acc[nt][v];
-          const T other = __shfl_xor(mine, 1);
+          const T other = swap_adjacent_lanes(mine);
           const int j = M::row(lane, v);
           const cx<T> c = ri ? cx<T>{other, mine} : cx<T>{mine, other};
           if (live_of[nt] && j < nb) {
-            const cx<T> out = rot[j] * c;
+            const cx<T> out = rot[nt][v] * c;
             dst[2 * (izr_of[nt] + s.n_zr * j) + ri] = ri ? out.y : out.x;
           }
         }
