@@ -109,10 +109,37 @@ constexpr bool dvr_swizzled() {
 // The phi step in three register passes over the DVR rows (NPHI = R1 R2; rows = C n_theta <= C 16): radix R1
 // forward; radix R2 forward, x exp(i A) in natural phi order, radix R2 backward on the same registers; radix R1
 // backward. Butterfly b: row b % (16 C), lanes along rows. Sign +1 towards the DVR, -1 back; W = exp(-2 pi i m/NPHI).
-template <typename T, int NPHI, int R1, int R2, int C, int THREADS, class Tile>
-__device__ __forceinline__ void phi_step_fused(cx<T>* dvr, int rows, int cw, const cx<T>* table, const int* phi_map,
-                                               const cx<T>* exp_rows, int exp_stride) {
+template <int NPHI, int R1, int C, int THREADS>
+constexpr int phi_exp_per_thread() {
+  return per_thread<16 * C * R1, THREADS>();
+}
+
+// The exp(i A) values a thread multiplies in the second pass of phi_step_fused (R2 per butterfly, natural phi
+// order), from the exp rows of the tile's first izr (exp_rows; row stride exp_stride per izr).
+template <typename T, int NPHI, int R1, int R2, int C, int THREADS, int Q>
+__device__ __forceinline__ void phi_exp_load(cx<T> (&ev)[Q][R2], int rows, int cw, const int* phi_map,
+                                             const cx<T>* exp_rows, int exp_stride) {
+  static_assert(Q == phi_exp_per_thread<NPHI, R1, C, THREADS>(), "exp values per thread");
+  constexpr int ROWS_MAX = 16 * C;
+#pragma unroll
+  for (int q = 0; q < Q; ++q) {
+    const int b = threadIdx.x + q * THREADS;
+    const int row = b % ROWS_MAX, j = b / ROWS_MAX;
+    const int th = row / C, cc = row % C;
+    const bool on = row < rows && j < R1 && cc < cw;
+    const cx<T>* e = exp_rows + exp_stride * cc + NPHI * th;
+#pragma unroll
+    for (int r = 0; r < R2; ++r) ev[q][r] = on ? e[phi_map[j + r * R1]] : cx<T>{T(1), T(0)};
+  }
+}
+
+// ev holds the exp(i A) values of phi_exp_load once load_ev() has run; load_ev is called at the start of the second
+// pass (a no-op when the caller loaded them earlier).
+template <typename T, int NPHI, int R1, int R2, int C, int THREADS, class Tile, int QE, class LoadEv>
+__device__ __forceinline__ void phi_step_fused(cx<T>* dvr, int rows, const cx<T>* table, cx<T> (&ev)[QE][R2],
+                                               LoadEv load_ev) {
   static_assert(R1 * R2 == NPHI, "phi radices");
+  static_assert(QE == phi_exp_per_thread<NPHI, R1, C, THREADS>(), "exp values per thread");
   constexpr int ROWS_MAX = 16 * C;
   const int tid = threadIdx.x;
   {
@@ -143,17 +170,7 @@ __device__ __forceinline__ void phi_step_fused(cx<T>* dvr, int rows, int cw, con
   {
     constexpr int Q = per_thread<ROWS_MAX * R1, THREADS>();
     cx<T> v[Q][R2];
-    cx<T> ev[Q][R2];  // the exp(i A) values, loaded first so that their latency overlaps the LDS work
-#pragma unroll
-    for (int q = 0; q < Q; ++q) {
-      const int b = tid + q * THREADS;
-      const int row = b % ROWS_MAX, j = b / ROWS_MAX;
-      const int th = row / C, cc = row % C;
-      const bool on = row < rows && j < R1 && cc < cw;
-      const cx<T>* e = exp_rows + exp_stride * cc + NPHI * th;
-#pragma unroll
-      for (int r = 0; r < R2; ++r) ev[q][r] = on ? e[phi_map[j + r * R1]] : cx<T>{T(1), T(0)};
-    }
+    load_ev();
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
       const int b = tid + q * THREADS;
@@ -211,8 +228,10 @@ __global__ __launch_bounds__(64 * WAVES) void interaction_mfma_kernel(
   constexpr int C = 8 * NT;
   constexpr int THREADS = 64 * WAVES;
   extern __shared__ unsigned char smem_raw[];
+  // LDS: phi map, phi twiddles, DVR tile.
   int* phi_map = reinterpret_cast<int*>(smem_raw);
-  cx<T>* dvr = reinterpret_cast<cx<T>*>(smem_raw + ((NPHI * sizeof(int) + 15) / 16) * 16);
+  cx<T>* tw_phi = reinterpret_cast<cx<T>*>(smem_raw + ((NPHI * sizeof(int) + 15) / 16) * 16);
+  cx<T>* dvr = tw_phi + NPHI;
   T* dvr_t = reinterpret_cast<T*>(dvr);
 
   const int n_theta = s.n_theta;
@@ -232,7 +251,10 @@ __global__ __launch_bounds__(64 * WAVES) void interaction_mfma_kernel(
   const int rows = C * n_theta;
 
   const int op = at.op_of[ixy];
-  for (int e = tid; e < NPHI; e += THREADS) phi_map[e] = at.phi_index[e + NPHI * op];
+  for (int e = tid; e < NPHI; e += THREADS) {
+    phi_map[e] = at.phi_index[e + NPHI * op];
+    tw_phi[e] = table_phi[e];
+  }
   // Slots of |mJ| > MJMax carry no basis function.
   const int n_unused = NPHI - s.n_blocks;
   for (int e = tid; e < rows * n_unused; e += THREADS) dvr[Tile::at(e / n_unused, s.mj_max + 1 + e % n_unused)] = {};
@@ -340,17 +362,19 @@ __global__ __launch_bounds__(64 * WAVES) void interaction_mfma_kernel(
   }
   __syncthreads();
 
-  // 2. phi FFTs, the potential phase, phi FFTs back.
+  // 2. phi FFTs, the potential phase, phi FFTs back. The exp(i A) values are loaded at the start of the second
+  // pass, so that their latency overlaps its LDS reads and twiddles.
   const int rep = at.rep_of[ixy];
   const int n_tp = n_theta * NPHI;
   if constexpr (R1 > 0) {
-    const int stride = n_tp * s.n_rep;
-    phi_step_fused<T, NPHI, R1, R2, C, THREADS, Tile>(dvr, rows, cw, table_phi, phi_map,
-                                                      exp_table + n_tp * (rep + static_cast<long>(s.n_rep) * izr0),
-                                                      stride);
+    cx<T> ev[phi_exp_per_thread<NPHI, R1, C, THREADS>()][R2];
+    const cx<T>* exp_rows = exp_table + n_tp * (rep + static_cast<long>(s.n_rep) * izr0);
+    phi_step_fused<T, NPHI, R1, R2, C, THREADS, Tile>(dvr, rows, tw_phi, ev, [&] {
+      phi_exp_load<T, NPHI, R1, R2, C, THREADS>(ev, rows, cw, phi_map, exp_rows, n_tp * s.n_rep);
+    });
   } else {
     using Rows = CtLayout<0, Tile::ld, 0, 1>;
-    ct_fft<T, NPHI, THREADS, Rows>(dvr, rows, table_phi, +1);
+    ct_fft<T, NPHI, THREADS, Rows>(dvr, rows, tw_phi, +1);
     for (int e = tid; e < rows * NPHI; e += THREADS) {
       const int ip = e % NPHI;
       const int row = e / NPHI;  // theta * C + col
@@ -361,7 +385,7 @@ __global__ __launch_bounds__(64 * WAVES) void interaction_mfma_kernel(
       v = exp_table[phi_map[ip] + NPHI * th + static_cast<long>(n_tp) * (rep + static_cast<long>(s.n_rep) * (izr0 + cc))] * v;
     }
     __syncthreads();
-    ct_fft<T, NPHI, THREADS, Rows>(dvr, rows, table_phi, -1);
+    ct_fft<T, NPHI, THREADS, Rows>(dvr, rows, tw_phi, -1);
   }
 
   // 3. DVR -> FBR, x rot(dt/2), to HBM. Task = (mJ block, J tile, N tile); max_jt = ceil(largest block / 16).
@@ -479,11 +503,11 @@ __global__ __launch_bounds__(256) void specular_coupling_kernel(cx<T>* psi, cons
   f = f + rot_half[s.ichn + s.n_rot * ir] * (s2sc[ixy] * cx<T>{T(in.x), T(in.y)});
 }
 
-// LDS bytes of the MFMA interaction kernel (phi map + DVR tile of C n_theta rows).
+// LDS bytes of the MFMA interaction kernel (phi map, phi twiddles, DVR tile of C n_theta rows).
 template <typename T>
 inline std::size_t interaction_mfma_lds(int C, int n_theta, int nphi, bool fused) {
   const bool swizzled = sizeof(T) == 8 && fused && nphi % 16 == 0;
-  return ((nphi * sizeof(int) + 15) / 16) * 16 +
+  return ((nphi * sizeof(int) + 15) / 16) * 16 + std::size_t(nphi) * sizeof(cx<T>) +
          std::size_t(C) * n_theta * (swizzled ? nphi : nphi + 1) * sizeof(cx<T>);
 }
 

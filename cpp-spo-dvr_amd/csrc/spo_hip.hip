@@ -175,7 +175,7 @@ std::vector<Variant> zr_reg_variants(int nz, int nr) {
 #define X(A, B, A1, A2, B1, B2, TH)                                                                        \
   if (nz == A && nr == B)                                                                                  \
     out.push_back({reinterpret_cast<const void*>(spo_dev::zr_reg_kernel<T, A, B, A1, A2, B1, B2, TH>), TH, \
-                   TH, std::size_t(A) * (B + 1) * sizeof(cx<T>)});
+                   TH, spo_dev::zr_reg_lds<T>(A, B)});
   SPO_ZR_REG_SHAPES(X)
 #undef X
   // Default (first) variant, as measured on MI355X at NZ 96 x NR 48: FP64 planes run best with the larger thread
@@ -880,8 +880,7 @@ std::unique_ptr<spo_hip_backend> create_state(const spo_hip_shape* shape, const 
   b->plan_y_d = std::make_unique<DevicePlan<double>>(s.ny);
   const std::size_t lds_limit = static_cast<std::size_t>(b->max_lds);
   // T_Zr: the register-pass kernel (padded rows) if instantiated and it fits, else the run-time one, else lines.
-  const std::size_t zr_reg_lds = std::size_t(s.nz) * (s.nr + 1) * cT;
-  if (zr_reg_lds <= lds_limit) b->zr_variants = zr_reg_variants<T>(s.nz, s.nr);
+  if (spo_dev::zr_reg_lds<T>(s.nz, s.nr) <= lds_limit) b->zr_variants = zr_reg_variants<T>(s.nz, s.nr);
   if (!b->zr_variants.empty()) {
     for (const Variant& v : b->zr_variants) set_lds(v.fn, v.lds);
   } else if (std::size_t(s.n_zr) * cT <= lds_limit) {
