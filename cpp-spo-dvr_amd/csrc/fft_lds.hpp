@@ -13,8 +13,10 @@
 
 namespace spo_dev {
 
+// Aligned to its size, so that a complex value moves as one 16-byte (FP64) or 8-byte (FP32) access: in LDS one
+// ds_read_b128 / ds_write_b128 instead of a pair of 8-byte accesses, which use only half of the banks per pass.
 template <typename T>
-struct cx {
+struct alignas(2 * sizeof(T)) cx {
   T x, y;
 };
 

@@ -1157,11 +1157,15 @@ int spo_hip_energy_resolved_read(spo_hip_backend* b, double* sums, double* specu
                              hipMemcpyDeviceToHost, b->stream));
     HIP_CHECK(hipStreamSynchronize(b->stream));
     // [pair][irot][izr] -> [pair][izr][irot]
-    cx<double>* out = reinterpret_cast<cx<double>*>(sums);
+    // (the caller's buffer is only guaranteed 8-byte aligned: copy components)
     for (std::size_t pr = 0; pr < pairs; ++pr)
       for (std::size_t irot = 0; irot < n_rot; ++irot)
-        for (std::size_t izr = 0; izr < n_zr; ++izr)
-          out[pr * block + irot + n_rot * izr] = dev[pr * block + izr + n_zr * irot];
+        for (std::size_t izr = 0; izr < n_zr; ++izr) {
+          const cx<double> v = dev[pr * block + izr + n_zr * irot];
+          double* o = sums + 2 * (pr * block + irot + n_rot * izr);
+          o[0] = v.x;
+          o[1] = v.y;
+        }
   });
 }
 
