@@ -124,9 +124,9 @@ __device__ __forceinline__ void phi_exp_load(cx<T> (&ev)[Q][R2], int rows, int c
 #pragma unroll
   for (int q = 0; q < Q; ++q) {
     const int b = threadIdx.x + q * THREADS;
-    const int row = b % ROWS_MAX, j = b / ROWS_MAX;
+    const int row = b / R1, j = b % R1;
     const int th = row / C, cc = row % C;
-    const bool on = row < rows && j < R1 && cc < cw;
+    const bool on = row < rows && cc < cw;
     const cx<T>* e = exp_rows + exp_stride * cc + NPHI * th;
 #pragma unroll
     for (int r = 0; r < R2; ++r) ev[q][r] = on ? e[phi_map[j + r * R1]] : cx<T>{T(1), T(0)};
@@ -174,8 +174,8 @@ __device__ __forceinline__ void phi_step_fused(cx<T>* dvr, int rows, const cx<T>
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
       const int b = tid + q * THREADS;
-      const int row = b % ROWS_MAX, j = b / ROWS_MAX;
-      if (row < rows && j < R1) {
+      const int row = b / R1, j = b % R1;  // lanes along j: the exp(i A) loads of a row share cache lines
+      if (row < rows) {
 #pragma unroll
         for (int r = 0; r < R2; ++r) v[q][r] = dvr[Tile::at(row, j + r * R1)];
         twiddle<T, R2>(v[q], table, j, +1);
@@ -189,8 +189,8 @@ __device__ __forceinline__ void phi_step_fused(cx<T>* dvr, int rows, const cx<T>
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
       const int b = tid + q * THREADS;
-      const int row = b % ROWS_MAX, j = b / ROWS_MAX;
-      if (row < rows && j < R1) {
+      const int row = b / R1, j = b % R1;
+      if (row < rows) {
 #pragma unroll
         for (int r = 0; r < R2; ++r) dvr[Tile::at(row, j * R2 + r)] = v[q][r];
       }
