@@ -9,14 +9,16 @@
 //   2. phi FFTs (sign +1) over every (theta, col) row, x exp(i A[rep(ixy), theta, phi_map(op(ixy), phi)]), phi
 //      FFTs back (sign -1): three register passes (phi_step_fused), or staged LDS FFTs.
 //   3. DVR -> FBR: C'[J, col] = sum_theta L_m[theta, J] D'[theta, col] (A = L_m transposed, B read from the tile),
-//      x rot(dt/2), stored straight to HBM.
+//      x rot(dt/2) (read from a copy of the table with ir fastest, so that the 8 columns of a lane group take
+//      consecutive factors), stored straight to HBM.
 // The initial channel's specular coupling term rot(dt/2) s2sc(ixy) chi(Z, r) is added to its FBR value as it is
 // loaded. Workgroups run tile-major over a site order that groups the sites of one symmetry representative on one
 // XCD (blockIdx mod 8), so the exp(i A) rows of a representative are fetched into one L2.
 //
 // DVR tile: row theta * C + col, phi along the row. In FP64 with the fused phi step and n_phi a multiple of 16 the
 // rows are n_phi long and element phi of row w sits at column phi ^ (w mod 16): the 16 rows a wavefront touches at
-// one phi (phases 1 and 3), or at one column (phase 2, lanes along rows), land on 16 distinct bank groups without
+// one phi (phases 1 and 3), or at one column (the first and last phi passes, lanes along rows), land on 16
+// distinct bank groups without
 // padding, which keeps the FP64 tile of 8 x 13 x 48 values at two workgroups per CU. Otherwise rows are padded to
 // n_phi + 1 (an odd stride spreads the rows over the banks just as well).
 //
@@ -220,7 +222,7 @@ __device__ __forceinline__ void phi_step_fused(cx<T>* dvr, int rows, const cx<T>
 template <typename T, int NPHI, int NT, int WAVES, int R1 = 0, int R2 = 0>
 __global__ __launch_bounds__(64 * WAVES) void interaction_mfma_kernel(
     cx<T>* psi, const cx<double>* spec, Shape s, const cx<T>* table_phi, AngularTables at, MfmaTables mt,
-    const T* frag_fwd, const T* frag_bwd, int max_jt, const cx<T>* exp_table, const cx<T>* rot_half,
+    const T* frag_fwd, const T* frag_bwd, int max_jt, const cx<T>* exp_table, const cx<T>* rot_t,
     const cx<T>* s2sc) {
   using M = Mfma<T>;
   using acc_t = typename M::acc_t;
@@ -419,7 +421,7 @@ __global__ __launch_bounds__(64 * WAVES) void interaction_mfma_kernel(
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
           const int j = M::row(lane, v);
-          rot[nt][v] = j < nb ? rot_half[f0 + j + s.n_rot * ir_of[nt]] : cx<T>{T(0), T(0)};
+          rot[nt][v] = j < nb ? rot_t[ir_of[nt] + s.nr * (f0 + j)] : cx<T>{T(0), T(0)};
         }
       T b[NT][4];
 #pragma unroll
@@ -480,7 +482,7 @@ __global__ __launch_bounds__(64 * WAVES) void interaction_mfma_kernel(
           const cx<T> c = ri ? cx<T>{other, mine} : cx<T>{mine, other};
           if (cc < cw && j < nb) {
             const int irot = f0 + j;
-            const cx<T> out = rot_half[irot + s.n_rot * ir] * c;
+            const cx<T> out = rot_t[ir + s.nr * irot] * c;
             site_t[2 * (izr + s.n_zr * irot) + ri] = ri ? out.y : out.x;
           }
         }

@@ -383,7 +383,7 @@ namespace {
 
 template <typename T>
 struct State final : spo_hip_backend {
-  Buffer psi, zr_full, zr_half, xy_full, xy_half, rot_half, before, gas_step, z_opt;
+  Buffer psi, zr_full, zr_half, xy_full, xy_half, rot_half, rot_half_t, before, gas_step, z_opt;
   Buffer shift_f, shift_b, s2sc, legendre, block_first, block_size, block_offset, rot_block, rep_of, op_of, phi_index;
   Buffer exp_table, legendre_t;
   Buffer frag_fwd, frag_bwd, fwd_offset, bwd_offset, site_order;
@@ -465,7 +465,7 @@ void State<T>::run_interaction_mfma(const Variant& v) {
   const int pair = s.n_xy % 8 == 0 ? 1 : 0;
   const cx<double>* sp = spec.as<cx<double>>();
   const cx<T>* et = exp_table.as<cx<T>>();
-  const cx<T>* rh = rot_half.as<cx<T>>();
+  const cx<T>* rh = rot_half_t.as<cx<T>>();
   const cx<T>* ssc = s2sc.as<cx<T>>();
   const cx<T>* tp = plan_phi->tw();
   spo_dev::MfmaTables mt{fwd_offset.as<int>(), bwd_offset.as<int>(), site_order.as<int>(), pair};
@@ -800,6 +800,15 @@ std::unique_ptr<spo_hip_backend> create_state(const spo_hip_shape* shape, const 
     b->xy_half = upload_complex<T>(xh.data(), s.n_xy);
   }
   b->rot_half = upload_complex<T>(t->rotation_half, std::size_t(s.n_rot) * s.nr);
+  {
+    // (irot, ir) -> (ir, irot), ir fastest, for the MFMA interaction kernel
+    std::vector<double> rt(2 * std::size_t(s.n_rot) * s.nr);
+    for (int ir = 0; ir < s.nr; ++ir)
+      for (int irot = 0; irot < s.n_rot; ++irot)
+        for (int c = 0; c < 2; ++c)
+          rt[2 * (ir + std::size_t(s.nr) * irot) + c] = t->rotation_half[2 * (irot + std::size_t(s.n_rot) * ir) + c];
+    b->rot_half_t = upload_complex<T>(rt.data(), rt.size() / 2);
+  }
   b->before = upload_complex<T>(t->before_interaction, std::size_t(s.n_rot) * s.nr);
   b->gas_step = upload_complex<T>(t->gas_phase_step, std::size_t(s.n_rot) * s.nr);
   b->spec_gas = upload(t->specular_gas_phase, std::size_t(s.nr) * c16);
