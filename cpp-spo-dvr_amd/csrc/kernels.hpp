@@ -248,24 +248,37 @@ __global__ __launch_bounds__(256) void interaction_kernel(cx<T>* psi, const cx<d
   }
 }
 
-// Projection of the initial channel onto the specular grid and its removal from the scattering grid, one thread
-// per interaction-region (Z, r) point (src/propagation/potential_operator.cpp:133-140).
+// Projection of the initial channel onto the specular grid and its removal from the scattering grid
+// (src/propagation/potential_operator.cpp:133-140). One workgroup per 64 interaction-region (Z, r) points, lanes
+// along izr (each load is 64 consecutive values of one site); wavefront w sums the sites w, w + W, w + 2 W, ... and
+// the W partial sums are added in wavefront order, a fixed order, so repeated runs agree bitwise.
 // The projection is summed in FP64 and the specular wave kept in FP64 whatever the storage type of psi.
+constexpr int kProjectionWaves = 16;
+
 template <typename T>
-__global__ void projection_kernel(cx<T>* psi, cx<double>* spec, Shape s, const cx<double>* sc2s,
-                                  const cx<double>* s2sc) {
-  const int izr = blockIdx.x * blockDim.x + threadIdx.x;
-  if (izr >= s.n_zr_pot) return;
+__global__ __launch_bounds__(64 * kProjectionWaves) void projection_kernel(cx<T>* psi, cx<double>* spec, Shape s,
+                                                                          const cx<double>* sc2s,
+                                                                          const cx<double>* s2sc) {
+  constexpr int W = kProjectionWaves;
+  __shared__ cx<double> part[W][64];
+  const int lane = threadIdx.x % 64, w = threadIdx.x / 64;
+  const int izr = blockIdx.x * 64 + lane;
+  const bool on = izr < s.n_zr_pot;
   const long stride = (long)s.n_zr * s.n_rot;
   cx<T>* p = psi + izr + (long)s.n_zr * s.ichn;
+  cx<double> acc = {0.0, 0.0};
+  if (on)
+    for (int ixy = w; ixy < s.n_xy; ixy += W) {
+      const cx<T> v = p[stride * ixy];
+      acc = acc + sc2s[ixy] * cx<double>{double(v.x), double(v.y)};
+    }
+  part[w][lane] = acc;
+  __syncthreads();
   cx<double> chi = {0.0, 0.0};
-  for (int ixy = 0; ixy < s.n_xy; ++ixy) {
-    const cx<T> v = p[stride * ixy];
-    chi = chi + sc2s[ixy] * cx<double>{double(v.x), double(v.y)};
-  }
-  const int ir = izr % s.nr, iz = izr / s.nr;
-  spec[ir + s.nr * iz] = chi;
-  for (int ixy = 0; ixy < s.n_xy; ++ixy) {
+  for (int k = 0; k < W; ++k) chi = chi + part[k][lane];
+  if (!on) return;
+  if (w == 0) spec[izr % s.nr + s.nr * (izr / s.nr)] = chi;
+  for (int ixy = w; ixy < s.n_xy; ixy += W) {
     const cx<T> v = p[stride * ixy];
     const cx<double> r = cx<double>{double(v.x), double(v.y)} - s2sc[ixy] * chi;
     p[stride * ixy] = cx<T>{T(r.x), T(r.y)};

@@ -621,7 +621,8 @@ void State<T>::kinetic_and_potential(bool half, bool kinetic_only) {
     });
   }
   timed("projection", [&] {
-    hipLaunchKernelGGL(spo_dev::projection_kernel<T>, dim3(blocks_for(s.n_zr_pot, kThreads)), dim3(kThreads), 0,
+    hipLaunchKernelGGL(spo_dev::projection_kernel<T>, dim3(blocks_for(s.n_zr_pot, 64)),
+                       dim3(64 * spo_dev::kProjectionWaves), 0,
                        stream, ps, spec.as<cx<double>>(), s, sc2s_d.as<cx<double>>(), s2sc_d.as<cx<double>>());
   });
 }
