@@ -121,35 +121,89 @@ __device__ __forceinline__ void static_for(F&& f) {
 template <int R, typename T>
 __device__ __forceinline__ void rdft(cx<T>* v, int sign);
 
-// y[k1 + R1 k2] = sum_{n2} w_R^{n2 k1} (sum_{n1} x[R2 n1 + n2] w_R1^{n1 k1}) w_R2^{n2 k2}, R = R1 R2.
+// a * w_R^m (w_R = exp(sign 2 pi i / R)) for a compile-time m: the quarter turns are sign swaps, the odd eighth
+// turns (1 +- i) / sqrt 2 two adds and two multiplications, the rest a full complex product.
+template <int M, int R, typename T>
+__device__ __forceinline__ cx<T> mul_unit_root(cx<T> a, int sign) {
+  constexpr int m = ((M % R) + R) % R;
+  if constexpr (m == 0) {
+    return a;
+  } else if constexpr (4 * m == R) {
+    return times_i(a, sign);
+  } else if constexpr (2 * m == R) {
+    return {-a.x, -a.y};
+  } else if constexpr (4 * m == 3 * R) {
+    return times_i(a, -sign);
+  } else if constexpr ((8 * m) % R == 0) {
+    // odd eighth turn: w = (cx + i sign sx) with |cx| = |sx| = sqrt(1/2)
+    constexpr CS w = unit_root(m, R);
+    constexpr double h = 0.70710678118654752440;
+    const T sx = T(sign) * T(w.s > 0 ? 1 : -1);
+    const T cxs = T(w.c > 0 ? 1 : -1);
+    // (a.x + i a.y)(cxs + i sx) h
+    return {T(h) * (cxs * a.x - sx * a.y), T(h) * (cxs * a.y + sx * a.x)};
+  } else {
+    constexpr CS w = unit_root(m, R);
+    return a * cx<T>{T(w.c), T(sign) * T(w.s)};
+  }
+}
+
+// Length R = R1 R2 from DFTs of lengths R1 and R2.
+// Coprime R1, R2 (Good-Thomas): x[(R2 n1 + R1 n2) mod R] -> DFT over n1 -> DFT over n2 -> X[k], k = k1 mod R1 and
+// k2 mod R2 (Chinese remainder), with no twiddles at all.
+// Otherwise (Cooley-Tukey): y[k1 + R1 k2] = sum_{n2} w_R^{n2 k1} (sum_{n1} x[R2 n1 + n2] w_R1^{n1 k1}) w_R2^{n2 k2}.
+constexpr int ct_gcd(int a, int b) { return b == 0 ? a : ct_gcd(b, a % b); }
+constexpr int ct_crt(int k1, int r1, int k2, int r2) {
+  for (int k = 0; k < r1 * r2; ++k)
+    if (k % r1 == k1 && k % r2 == k2) return k;
+  return -1;
+}
+
 template <int R1, int R2, typename T>
 __device__ __forceinline__ void rdft_composite(cx<T>* v, int sign) {
   constexpr int R = R1 * R2;
   cx<T> a[R2][R1];
+  if constexpr (ct_gcd(R1, R2) == 1) {
+    static_for<R2 * R1>([&](auto idx) {
+      constexpr int n2 = decltype(idx)::value / R1;
+      constexpr int n1 = decltype(idx)::value % R1;
+      a[n2][n1] = v[(R2 * n1 + R1 * n2) % R];
+    });
 #pragma unroll
-  for (int n2 = 0; n2 < R2; ++n2) {
+    for (int n2 = 0; n2 < R2; ++n2) rdft<R1, T>(a[n2], sign);
+    static_for<R1>([&](auto k1c) {
+      constexpr int k1 = decltype(k1c)::value;
+      cx<T> b[R2];
 #pragma unroll
-    for (int n1 = 0; n1 < R1; ++n1) a[n2][n1] = v[R2 * n1 + n2];
-    rdft<R1, T>(a[n2], sign);
-  }
-  // Internal twiddles w_R^{n2 k1}, n2, k1 >= 1, as compile-time constants.
-  static_for<R2 * R1>([&](auto idx) {
-    constexpr int n2 = decltype(idx)::value / R1;
-    constexpr int k1 = decltype(idx)::value % R1;
-    if constexpr (n2 > 0 && k1 > 0) {
-      constexpr CS w = unit_root(n2 * k1, R);
-      const cx<T> t = {T(w.c), T(sign) * T(w.s)};
-      a[n2][k1] = a[n2][k1] * t;
+      for (int n2 = 0; n2 < R2; ++n2) b[n2] = a[n2][k1];
+      rdft<R2, T>(b, sign);
+      static_for<R2>([&](auto k2c) {
+        constexpr int k2 = decltype(k2c)::value;
+        v[ct_crt(k1, R1, k2, R2)] = b[k2];
+      });
+    });
+  } else {
+#pragma unroll
+    for (int n2 = 0; n2 < R2; ++n2) {
+#pragma unroll
+      for (int n1 = 0; n1 < R1; ++n1) a[n2][n1] = v[R2 * n1 + n2];
+      rdft<R1, T>(a[n2], sign);
     }
-  });
+    // Internal twiddles w_R^{n2 k1}, n2, k1 >= 1, as compile-time constants.
+    static_for<R2 * R1>([&](auto idx) {
+      constexpr int n2 = decltype(idx)::value / R1;
+      constexpr int k1 = decltype(idx)::value % R1;
+      if constexpr (n2 > 0 && k1 > 0) a[n2][k1] = mul_unit_root<n2 * k1, R, T>(a[n2][k1], sign);
+    });
 #pragma unroll
-  for (int k1 = 0; k1 < R1; ++k1) {
-    cx<T> b[R2];
+    for (int k1 = 0; k1 < R1; ++k1) {
+      cx<T> b[R2];
 #pragma unroll
-    for (int n2 = 0; n2 < R2; ++n2) b[n2] = a[n2][k1];
-    rdft<R2, T>(b, sign);
+      for (int n2 = 0; n2 < R2; ++n2) b[n2] = a[n2][k1];
+      rdft<R2, T>(b, sign);
 #pragma unroll
-    for (int k2 = 0; k2 < R2; ++k2) v[k1 + R1 * k2] = b[k2];
+      for (int k2 = 0; k2 < R2; ++k2) v[k1 + R1 * k2] = b[k2];
+    }
   }
 }
 

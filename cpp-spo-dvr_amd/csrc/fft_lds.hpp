@@ -125,11 +125,11 @@ __device__ __forceinline__ void dft(cx<T>* v, int sign) {
     dft<4, T>(e, sign);
     dft<4, T>(o, sign);
     const T h = T(0.7071067811865476);
-    // w^k = exp(sign i pi k / 4)
-    const cx<T> w1 = {h, T(sign) * h}, w3 = {-h, T(sign) * h};
-    o[1] = o[1] * w1;
+    const T sg = T(sign);
+    // w^k = exp(sign i pi k / 4): w^1 = h (1 + sign i), w^3 = h (-1 + sign i)
+    o[1] = {h * (o[1].x - sg * o[1].y), h * (o[1].y + sg * o[1].x)};
     o[2] = times_i(o[2], sign);
-    o[3] = o[3] * w3;
+    o[3] = {h * (-o[3].x - sg * o[3].y), h * (-o[3].y + sg * o[3].x)};
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
       v[k] = e[k] + o[k];

@@ -28,9 +28,10 @@ constexpr int per_thread() {
   return (NBUT + THREADS - 1) / THREADS;
 }
 
-template <typename T, int R>
+// SKIP_ZERO: skip the (exact) multiplications by 1 of j == 0, worth a branch when j is uniform over a wavefront.
+template <typename T, int R, bool SKIP_ZERO = true>
 __device__ __forceinline__ void twiddle(cx<T>* v, const cx<T>* table, int j, int sign) {
-  if (j == 0) return;
+  if (SKIP_ZERO && j == 0) return;
 #pragma unroll
   for (int r = 1; r < R; ++r) {
     cx<T> w = table[r * j];

@@ -117,21 +117,25 @@ constexpr int phi_exp_per_thread() {
 }
 
 // The exp(i A) values a thread multiplies in the second pass of phi_step_fused (R2 per butterfly, natural phi
-// order), from the exp rows of the tile's first izr (exp_rows; row stride exp_stride per izr).
+// order), from the exp rows of the tile's first izr (exp_rows, wavefront-uniform; row stride exp_stride per izr).
+// Rows past the tile (row >= rows, or a column cc >= cw past the interaction region) read the last valid row
+// instead: their products are never stored, and every load stays inside the table without a branch. The offsets
+// are 32-bit byte offsets from the uniform base.
 template <typename T, int NPHI, int R1, int R2, int C, int THREADS, int Q>
 __device__ __forceinline__ void phi_exp_load(cx<T> (&ev)[Q][R2], int rows, int cw, const int* phi_map,
                                              const cx<T>* exp_rows, int exp_stride) {
   static_assert(Q == phi_exp_per_thread<NPHI, R1, C, THREADS>(), "exp values per thread");
-  constexpr int ROWS_MAX = 16 * C;
+  const unsigned char* base = reinterpret_cast<const unsigned char*>(exp_rows);
 #pragma unroll
   for (int q = 0; q < Q; ++q) {
-    const int b = threadIdx.x + q * THREADS;
-    const int row = b / R1, j = b % R1;
-    const int th = row / C, cc = row % C;
-    const bool on = row < rows && cc < cw;
-    const cx<T>* e = exp_rows + exp_stride * cc + NPHI * th;
+    const unsigned b = threadIdx.x + q * THREADS;
+    const unsigned row = min(b / R1, static_cast<unsigned>(rows - 1)), j = b % R1;
+    const int th = row / C, cc = min(static_cast<int>(row % C), cw - 1);
+    const unsigned off = static_cast<unsigned>(exp_stride * cc + NPHI * th);
 #pragma unroll
-    for (int r = 0; r < R2; ++r) ev[q][r] = on ? e[phi_map[j + r * R1]] : cx<T>{T(1), T(0)};
+    for (int r = 0; r < R2; ++r)
+      ev[q][r] = *reinterpret_cast<const cx<T>*>(base + (off + static_cast<unsigned>(phi_map[j + r * R1])) *
+                                                            static_cast<unsigned>(sizeof(cx<T>)));
   }
 }
 
@@ -175,12 +179,12 @@ __device__ __forceinline__ void phi_step_fused(cx<T>* dvr, int rows, const cx<T>
     load_ev();
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
-      const int b = tid + q * THREADS;
+      const unsigned b = tid + q * THREADS;
       const int row = b / R1, j = b % R1;  // lanes along j: the exp(i A) loads of a row share cache lines
       if (row < rows) {
 #pragma unroll
         for (int r = 0; r < R2; ++r) v[q][r] = dvr[Tile::at(row, j + r * R1)];
-        twiddle<T, R2>(v[q], table, j, +1);
+        twiddle<T, R2, false>(v[q], table, j, +1);
         rdft<R2, T>(v[q], +1);
 #pragma unroll
         for (int r = 0; r < R2; ++r) v[q][r] = ev[q][r] * v[q][r];
@@ -190,7 +194,7 @@ __device__ __forceinline__ void phi_step_fused(cx<T>* dvr, int rows, const cx<T>
     __syncthreads();
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
-      const int b = tid + q * THREADS;
+      const unsigned b = tid + q * THREADS;
       const int row = b / R1, j = b % R1;
       if (row < rows) {
 #pragma unroll
