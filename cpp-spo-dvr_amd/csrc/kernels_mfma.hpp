@@ -256,10 +256,20 @@ __global__ __launch_bounds__(64 * WAVES) void interaction_mfma_kernel(
   const int ri = n % 2;
   const int rows = C * n_theta;
 
+  // The phi map (and for complex64 the twiddles) are loaded here and written to LDS only after phase 1, so that
+  // their latency does not hold up phase 1's first loads. (FP64 writes its twiddles at once: holding them through
+  // phase 1 would spill.)
+  static_assert(NPHI <= THREADS, "one phi entry per thread");
+  constexpr bool kLateTwiddles = sizeof(T) == 4;
   const int op = at.op_of[ixy];
-  for (int e = tid; e < NPHI; e += THREADS) {
-    phi_map[e] = at.phi_index[e + NPHI * op];
-    tw_phi[e] = table_phi[e];
+  int map_entry = 0;
+  cx<T> tw_entry = {};
+  if (tid < NPHI) {
+    map_entry = at.phi_index[tid + NPHI * op];
+    if constexpr (kLateTwiddles)
+      tw_entry = table_phi[tid];
+    else
+      tw_phi[tid] = table_phi[tid];
   }
   // Slots of |mJ| > MJMax carry no basis function.
   const int n_unused = NPHI - s.n_blocks;
@@ -295,6 +305,9 @@ __global__ __launch_bounds__(64 * WAVES) void interaction_mfma_kernel(
         dvr_swz[nt][v] = Tile::at(row, 0) - row * Tile::ld;  // the row's column swizzle (0 without swizzling)
       }
     }
+    // B values of every k step of a task, wavefront-uniform; the rows past the block (J >= nb) and the columns past
+    // the interaction region read neighbouring finite values (or the zero slack behind psi): the former meet zero
+    // fragment entries, the latter are never stored.
     T bv[kPrefetchTasks][NT][4];
 #pragma unroll
     for (int i = 0; i < kPrefetchTasks; ++i) {
@@ -306,7 +319,7 @@ __global__ __launch_bounds__(64 * WAVES) void interaction_mfma_kernel(
       for (int nt = 0; nt < NT; ++nt)
 #pragma unroll
         for (int k = 0; k < 4; ++k)
-          bv[i][nt][k] = (live_of[nt] && 4 * k + kk < nb) ? src[lane_b[nt] + 8 * s.n_zr * k] : T(0);
+          bv[i][nt][k] = k < (nb + 3) / 4 ? src[lane_b[nt] + 8 * s.n_zr * k] : T(0);
     }
     // The A fragments of task i + 1 are loaded while the products of task i run.
     auto load_frags = [&](int bb, T (&a)[4]) {
@@ -365,6 +378,10 @@ __global__ __launch_bounds__(64 * WAVES) void interaction_mfma_kernel(
         if (th < n_theta) dvr_t[2 * Tile::at(th * C + cc, slot) + ri] = acc[v];
       }
     }
+  }
+  if (tid < NPHI) {
+    phi_map[tid] = map_entry;
+    if constexpr (kLateTwiddles) tw_phi[tid] = tw_entry;
   }
   __syncthreads();
 

@@ -768,8 +768,12 @@ std::unique_ptr<spo_hip_backend> create_state(const spo_hip_shape* shape, const 
   HIP_CHECK(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
 
   const std::size_t cT = sizeof(cx<T>), c16 = sizeof(cx<double>);
-  b->psi = Buffer(psi_values * cT);
-  HIP_CHECK(hipMemsetAsync(b->psi.template as<void>(), 0, psi_values * cT, b->stream));
+  // Zero slack behind the last site: the interaction kernel's phase-1 loads of a partial k step or of a column past
+  // the interaction region read a few values past a site column without a branch (their products vanish: the
+  // matching Legendre fragments are zero, or the column is never stored), at most 3 n_zr + 6 values past the end.
+  const std::size_t psi_slack = 4 * std::size_t(s.n_zr) + 64;
+  b->psi = Buffer((psi_values + psi_slack) * cT);
+  HIP_CHECK(hipMemsetAsync(b->psi.template as<void>(), 0, (psi_values + psi_slack) * cT, b->stream));
   b->spec = upload(initial_specular, std::size_t(s.nr) * s.nzsp * c16);
   b->densities = Buffer(std::size_t(s.n_zr) * sizeof(double));
   b->partial = Buffer(std::size_t(s.n_zr) * 16 * sizeof(double));
