@@ -154,7 +154,10 @@ def measure(path: str, workload: str, precision: str, steps: int, warmup: int) -
                         "traffic": measured_traffic(kname, workload, precision), "kernel": kname,
                         "algorithmic_bytes_per_launch": alg, "mean_launch_ms": round(total_ms / launches, 4),
                         "launches": launches}
-    step_bytes = sum(algorithmic_bytes(k, info, complex_bytes) or 0.0 for k in ("zr_plane", "xy_tile", "interaction"))
+    # algorithmic bytes per time step: every launch of the three passes in the timed region (T_Zr and T_xy run
+    # (n + 1) / n times per time step: the schedule's n + 1 kinetic blocks per analysis step of n time steps)
+    step_bytes = sum((algorithmic_bytes(k, info, complex_bytes) or 0.0) * kernels[k][0]
+                     for k in ("zr_plane", "xy_tile", "interaction") if k in kernels) / n_steps
     return {"info": info, "wave": wave, "steps": n_steps, "warmup": warm_analysis * spa,
             "value": dvr_points(info) * n_steps / elapsed, "ms_per_step": elapsed / n_steps * 1e3,
             "roofline": roofline, "step_roofline_frac": round(step_bytes / (elapsed / n_steps) / 1e9 / HBM_PEAK_GBS, 4),
