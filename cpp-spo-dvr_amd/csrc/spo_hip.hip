@@ -21,6 +21,7 @@
 #include "../../include/spo_hip.h"
 #include "kernels_ct.hpp"
 #include "kernels_mfma.hpp"
+#include "kernels_pes.hpp"
 
 using spo_dev::cx;
 
@@ -80,6 +81,12 @@ class Buffer {
   void* ptr_ = nullptr;
   std::size_t bytes_ = 0;
 };
+
+void require_device() {
+  int devices = 0;
+  if (hipGetDeviceCount(&devices) != hipSuccess || devices < 1)
+    throw Status(SPO_HIP_E_NOT_SUPPORTED, "no HIP device is visible");
+}
 
 Buffer upload(const void* host, std::size_t bytes) {
   Buffer b(bytes);
@@ -1015,9 +1022,7 @@ int spo_hip_create(const spo_hip_shape* shape, const spo_hip_tables* t, const sp
   return guarded([&] {
     if (!shape || !t || !a || !initial_specular || !out) throw Status(SPO_HIP_E_ARGUMENT, "spo_hip_create: NULL argument");
     *out = nullptr;
-    int devices = 0;
-    if (hipGetDeviceCount(&devices) != hipSuccess || devices < 1)
-      throw Status(SPO_HIP_E_NOT_SUPPORTED, "no HIP device is visible");
+    require_device();
     std::unique_ptr<spo_hip_backend> b;
     if (shape->precision == SPO_HIP_FP64)
       b = create_state<double>(shape, t, a, initial_specular);
@@ -1211,6 +1216,60 @@ int spo_hip_profile_read(spo_hip_backend* b, spo_hip_kernel_time* out, int capac
     }
     *count = static_cast<int>(agg.size());
     for (int k = 0; k < *count && k < capacity; ++k) out[k] = agg[k];
+  });
+}
+
+int spo_hip_pes_values(int pes, long n, const double* points, double* values) {
+  return guarded([&] {
+    if (pes != SPO_HIP_PES_HARDWALL && pes != SPO_HIP_PES_DAILIGHT)
+      throw Status(SPO_HIP_E_ARGUMENT, "spo_hip_pes_values: unknown surface " + std::to_string(pes));
+    if (n < 0 || (n > 0 && (!points || !values))) throw Status(SPO_HIP_E_ARGUMENT, "spo_hip_pes_values: bad arrays");
+    if (n == 0) return;
+    require_device();
+    Buffer p = upload(points, std::size_t(n) * 6 * sizeof(double));
+    Buffer out(std::size_t(n) * sizeof(double));
+    hipLaunchKernelGGL(spo_dev::pes_points_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, nullptr, pes, n,
+                       p.as<double>(), out.as<double>());
+    HIP_CHECK(hipGetLastError());
+    HIP_CHECK(hipMemcpy(values, out.as<void>(), std::size_t(n) * sizeof(double), hipMemcpyDeviceToHost));
+  });
+}
+
+int spo_hip_potential_grid(const spo_hip_pes_grid* grid, double* values) {
+  return guarded([&] {
+    if (!grid || !values) throw Status(SPO_HIP_E_ARGUMENT, "spo_hip_potential_grid: NULL argument");
+    const spo_hip_pes_grid& g = *grid;
+    if (g.pes != SPO_HIP_PES_HARDWALL && g.pes != SPO_HIP_PES_DAILIGHT)
+      throw Status(SPO_HIP_E_ARGUMENT, "spo_hip_potential_grid: unknown surface " + std::to_string(g.pes));
+    if (g.nx <= 0 || g.ny <= 0 || g.nr <= 0 || g.nz_potential < 0 || g.n_theta <= 0 || g.n_phi <= 0 ||
+        g.n_representatives <= 0 || g.n_representatives > g.nx * g.ny)
+      throw Status(SPO_HIP_E_ARGUMENT, "spo_hip_potential_grid: bad extents");
+    if (!g.x || !g.y || !g.r || !g.theta || !g.phi || !g.representatives || (g.nz_potential > 0 && !g.z) ||
+        (g.use_switching && !g.gas_phase))
+      throw Status(SPO_HIP_E_ARGUMENT, "spo_hip_potential_grid: NULL table");
+    for (int k = 0; k < g.n_representatives; ++k)
+      if (g.representatives[k] < 0 || g.representatives[k] >= g.nx * g.ny)
+        throw Status(SPO_HIP_E_ARGUMENT, "spo_hip_potential_grid: representative outside the (x, y) grid");
+    const long n_zr = long(g.nz_potential) * g.nr;
+    const long n = long(g.n_theta) * g.n_phi * g.n_representatives * n_zr;
+    if (n == 0) return;
+    require_device();
+    auto up = [](const void* h, std::size_t bytes) { return upload(h, bytes); };
+    Buffer x = up(g.x, g.nx * sizeof(double)), y = up(g.y, g.ny * sizeof(double));
+    Buffer z = up(g.z, g.nz_potential * sizeof(double)), r = up(g.r, g.nr * sizeof(double));
+    Buffer th = up(g.theta, g.n_theta * sizeof(double)), ph = up(g.phi, g.n_phi * sizeof(double));
+    Buffer reps = up(g.representatives, g.n_representatives * sizeof(int));
+    Buffer gas = g.use_switching ? up(g.gas_phase, g.nr * sizeof(double)) : Buffer();
+    Buffer out(std::size_t(n) * sizeof(double));
+    spo_dev::PesGrid d{g.pes, g.ny, g.nr, g.n_theta, g.n_phi, g.n_representatives, static_cast<int>(n_zr),
+                       x.as<double>(), y.as<double>(), z.as<double>(), r.as<double>(), th.as<double>(),
+                       ph.as<double>(), reps.as<int>(), g.use_switching, g.z_min_switch, g.z_max_switch,
+                       g.use_switching ? gas.as<double>() : nullptr, g.impose_v_min, g.impose_v_max, g.v_min,
+                       g.v_max};
+    hipLaunchKernelGGL(spo_dev::potential_grid_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, nullptr, d,
+                       out.as<double>());
+    HIP_CHECK(hipGetLastError());
+    HIP_CHECK(hipMemcpy(values, out.as<void>(), std::size_t(n) * sizeof(double), hipMemcpyDeviceToHost));
   });
 }
 

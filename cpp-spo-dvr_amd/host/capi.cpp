@@ -1,4 +1,5 @@
 // C ABI of the host driver (include/spo_host.h).
+#include <cstdlib>
 #include <cstring>
 #include <exception>
 #include <memory>
@@ -63,7 +64,12 @@ int spo_run_create(const char* input_path, const char* overrides_json, int devic
       }
     }
     auto run = std::make_unique<spo_run>();
-    run->setup = spo::make_run_setup(spo::read_input(doc));
+    // On the GPU the potential grid is evaluated there too, unless SPO_HIP_POTENTIAL=host asks for the host build.
+    const char* where = std::getenv("SPO_HIP_POTENTIAL");
+    if (where && std::string(where) != "host" && std::string(where) != "device")
+      throw std::invalid_argument("SPO_HIP_POTENTIAL must be host or device");
+    const bool device_potential = device == SPO_DEVICE_HIP && !(where && std::string(where) == "host");
+    run->setup = spo::make_run_setup(spo::read_input(doc), device_potential);
     spo::BackendFactory make;
     if (device == SPO_DEVICE_HIP) {
       make = spo::make_hip_backend;

@@ -14,14 +14,15 @@ std::vector<cplx> exp_potential_table(const PotentialGrid& grid, double dt) {
   return table;
 }
 
-std::shared_ptr<RunSetup> make_run_setup(const InputParameters& input) {
+std::shared_ptr<RunSetup> make_run_setup(const InputParameters& input, bool device_potential) {
   auto s = std::make_shared<RunSetup>();
   s->config = build_config(input);
   s->grids = make_spatial_grids(s->config);
   s->angular = make_angular_grid(s->config);
   s->basis = make_rotational_basis(s->angular, s->config.input.j_in, s->config.input.mj_in);
   const auto pes = make_pes(s->config.input.system_id);
-  s->potential = compute_potential_grid(*pes, s->config, s->grids, s->angular);
+  s->potential = device_potential ? compute_potential_grid_on_device(*pes, s->config, s->grids, s->angular)
+                                  : compute_potential_grid(*pes, s->config, s->grids, s->angular);
   s->gas = compute_gas_phase(s->config, s->angular, s->potential.asymptotic);
   s->tables = make_propagation_tables(s->config, s->grids, s->basis, s->potential.asymptotic);
   s->analysis = make_analysis_basis(s->config, s->angular, s->basis, s->gas, s->tables.xy_kinetic_energies);

@@ -27,7 +27,8 @@ struct RunSetup {
   }
 };
 
-std::shared_ptr<RunSetup> make_run_setup(const InputParameters& input);
+// device_potential: evaluate the potential grid on the GPU (compute_potential_grid_on_device) instead of the host.
+std::shared_ptr<RunSetup> make_run_setup(const InputParameters& input, bool device_potential = false);
 
 class Propagation {
  public:

@@ -1,6 +1,10 @@
 #include "potential.hpp"
 
 #include <cmath>
+#include <stdexcept>
+#include <string>
+
+#include "../../include/spo_hip.h"
 
 namespace spo {
 
@@ -106,6 +110,53 @@ void clamp(std::vector<double>& v, const Config& c) {
 }
 
 }  // namespace
+
+PotentialGrid compute_potential_grid_on_device(const Pes& pes, const Config& c, const SpatialGrids& g,
+                                               const AngularGrid& a) {
+  int id = 0;
+  if (c.input.system_id == "HardWallPES") id = SPO_HIP_PES_HARDWALL;
+  if (c.input.system_id == "H2Cu111DaiLight") id = SPO_HIP_PES_DAILIGHT;
+  if (id == 0) throw NotSupportedError("SystemID " + c.input.system_id + " has no device surface");
+  PotentialGrid grid;
+  grid.symmetry = make_grid_symmetry(c, a.n_phi());
+  const int nr = g.r.size();
+  grid.n_tp = a.n_theta() * a.n_phi();
+  grid.n_rep = grid.symmetry.n_representatives();
+  grid.n_zr = g.nz_potential * nr;
+  grid.values.resize(std::size_t(grid.n_tp) * grid.n_rep * grid.n_zr);
+  const std::vector<double> v_gas = c.input.use_switching
+                                        ? gas_phase_potential(pes, c.derived.z_asymptotic_unswitched, g)
+                                        : std::vector<double>(nr, 0.0);
+  spo_hip_pes_grid d{};
+  d.pes = id;
+  d.nx = g.x.size();
+  d.ny = g.y.size();
+  d.nr = nr;
+  d.nz_potential = g.nz_potential;
+  d.n_theta = a.n_theta();
+  d.n_phi = a.n_phi();
+  d.n_representatives = grid.n_rep;
+  d.x = g.x.points.data();
+  d.y = g.y.points.data();
+  d.z = g.z.points.data();
+  d.r = g.r.points.data();
+  d.theta = a.thetas.data();
+  d.phi = a.phis.data();
+  d.representatives = grid.symmetry.representatives.data();
+  d.use_switching = c.input.use_switching ? 1 : 0;
+  d.z_min_switch = c.input.z_min_switch;
+  d.z_max_switch = c.input.z_max_switch;
+  d.gas_phase = v_gas.data();
+  d.impose_v_min = c.input.impose_v_min ? 1 : 0;
+  d.impose_v_max = c.input.impose_v_max ? 1 : 0;
+  d.v_min = c.derived.v_min;
+  d.v_max = c.derived.v_max;
+  if (spo_hip_potential_grid(&d, grid.values.data()) != SPO_HIP_OK)
+    throw std::runtime_error(std::string("device potential grid: ") + spo_hip_last_error());
+  grid.asymptotic = gas_phase_potential(pes, c.derived.z_asymptotic, g);
+  clamp(grid.asymptotic, c);
+  return grid;
+}
 
 PotentialGrid compute_potential_grid(const Pes& pes, const Config& c, const SpatialGrids& g, const AngularGrid& a) {
   PotentialGrid grid;

@@ -49,4 +49,9 @@ struct PotentialGrid {
 PotentialGrid compute_potential_grid(const Pes& pes, const Config& config, const SpatialGrids& grids,
                                      const AngularGrid& angular);
 
+// The same grid with the surface evaluated on the GPU (spo_hip_potential_grid); the gas-phase rows of the switch
+// and the asymptote come from `pes` on the host. Throws NotSupportedError for a surface the device does not have.
+PotentialGrid compute_potential_grid_on_device(const Pes& pes, const Config& config, const SpatialGrids& grids,
+                                               const AngularGrid& angular);
+
 }  // namespace spo

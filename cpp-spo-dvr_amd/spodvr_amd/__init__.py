@@ -113,6 +113,25 @@ def pes_value(system_id: str, x, y, z, r, theta, phi) -> float:
     return out.value
 
 
+_PES_IDS = {"HardWallPES": 1, "H2Cu111DaiLight": 2}  # SPO_HIP_PES_* of include/spo_hip.h
+
+
+def device_pes_values(system_id: str, points: np.ndarray) -> np.ndarray:
+    """The surface in eV at points (n, 6) = (x, y, z, r, theta, phi), evaluated on the GPU (spo_hip_pes_values)."""
+    host_library()  # loads libspo_hip.so with it
+    lib = ctypes.CDLL(library_path("libspo_hip.so"))
+    lib.spo_hip_pes_values.argtypes = [ctypes.c_int, ctypes.c_long, ctypes.c_void_p, ctypes.c_void_p]
+    lib.spo_hip_last_error.restype = ctypes.c_char_p
+    if system_id not in _PES_IDS:
+        raise SpoError(3, f"SystemID {system_id} has no device surface")
+    pts = np.ascontiguousarray(points, dtype=np.float64).reshape(-1, 6)
+    out = np.empty(len(pts))
+    status = lib.spo_hip_pes_values(_PES_IDS[system_id], len(pts), _ptr(pts), _ptr(out))
+    if status != 0:
+        raise SpoError(status, lib.spo_hip_last_error().decode(errors="replace"))
+    return out
+
+
 class Run:
     """A propagation run: the set-up from a JSON input (plus overrides) and a backend.
 

@@ -160,6 +160,34 @@ int spo_hip_profile_enable(spo_hip_backend* backend, int on);
 /* Per kernel: launches and summed event time since profiling was enabled; count receives the number of kernels. */
 int spo_hip_profile_read(spo_hip_backend* backend, spo_hip_kernel_time* out, int capacity, int* count);
 
+/* Potential energy surfaces evaluated on the device (the SystemIDs "HardWallPES" and "H2Cu111DaiLight";
+ * src/potential/hard_wall_potential.cpp, src/potential/dai_light_cu111_potential.cpp). */
+enum { SPO_HIP_PES_HARDWALL = 1, SPO_HIP_PES_DAILIGHT = 2 };
+
+/* The PES in eV at n points, points = (x, y, z, r, theta, phi) per point in bohr and radians; replaces
+ * Pes::value (src/potential/pes.hpp) for these surfaces. Needs no backend. */
+int spo_hip_pes_values(int pes, long n, const double* points, double* values);
+
+/* The potential grid the propagation stores (src/potential/potential_grid.cpp:65-127): V in hartree at the
+ * symmetry representatives, values[iphi + n_phi (itheta + n_theta (irep + n_representatives izr))] for
+ * izr = ir + nr iz < nz_potential nr, with the cosine switch into gas_phase between z_min_switch and
+ * z_max_switch (:35-56) and the VMin / VMax limits (:139-159). gas_phase (nr values, hartree) is the surface at
+ * the switch's asymptote, which the caller evaluates. Replaces compute_potential_grid for the surfaces above;
+ * needs no backend. */
+typedef struct spo_hip_pes_grid {
+  int pes; /* SPO_HIP_PES_* */
+  int nx, ny, nr, nz_potential, n_theta, n_phi, n_representatives;
+  const double* x; const double* y; const double* z; const double* r; /* z: nz_potential values */
+  const double* theta; const double* phi;
+  const int* representatives; /* full-grid ixy = iy + ny ix of each representative */
+  int use_switching;
+  double z_min_switch, z_max_switch;
+  const double* gas_phase;
+  int impose_v_min, impose_v_max;
+  double v_min, v_max; /* hartree */
+} spo_hip_pes_grid;
+int spo_hip_potential_grid(const spo_hip_pes_grid* grid, double* values);
+
 #ifdef __cplusplus
 }
 #endif
