@@ -1219,6 +1219,22 @@ int spo_hip_profile_read(spo_hip_backend* b, spo_hip_kernel_time* out, int capac
   });
 }
 
+int spo_hip_accumulate(spo_hip_backend* b, double time, const double* coefficients) {
+  return guarded([&] {
+    if (!b || !coefficients) throw Status(SPO_HIP_E_ARGUMENT, "spo_hip_accumulate: NULL argument");
+    const spo_hip_analysis& a = b->analysis;
+    const std::size_t n = std::size_t(a.n_states) * a.n_v;
+    HIP_CHECK(hipMemcpyAsync(b->bk.as<void>(), coefficients, n * sizeof(cx<double>), hipMemcpyHostToDevice,
+                             b->stream));
+    const long total = long(n) * a.n_energies;
+    hipLaunchKernelGGL(spo_dev::s_accumulate_kernel, dim3(blocks_for(total, kThreads)), dim3(kThreads), 0, b->stream,
+                       b->sums.as<cx<double>>(), b->bk.as<cx<double>>(), static_cast<int>(n), a.n_energies,
+                       b->energies.as<double>(), time, a.analysis_time_step);
+    HIP_CHECK(hipGetLastError());
+    HIP_CHECK(hipStreamSynchronize(b->stream));
+  });
+}
+
 int spo_hip_pes_values(int pes, long n, const double* points, double* values) {
   return guarded([&] {
     if (pes != SPO_HIP_PES_HARDWALL && pes != SPO_HIP_PES_DAILIGHT)

@@ -38,9 +38,24 @@ struct AnalysisBasis {
 AnalysisBasis make_analysis_basis(const Config& config, const AngularGrid& angular, const RotationalBasis& basis,
                                   const GasPhase& gas, const std::vector<double>& xy_kinetic_energies);
 
+// What a checkpoint keeps of the analysis surface (src/propagation/analysis_surface.hpp).
+struct AnalysisSurfaceState {
+  int index = 0, previous_index = 0;
+  bool fixed = false;
+  double norm_tolerance = 0.0;
+};
+
 // Dynamic analysis surface on the specular grid (src/propagation/analysis_surface.cpp:24-83).
 class AnalysisSurface {
  public:
+  AnalysisSurfaceState state() const { return {index_, previous_index_, fixed_, norm_tolerance_}; }
+  // The state of a checkpoint (src/propagation/analysis_surface.cpp:85-91).
+  void restore(const AnalysisSurfaceState& s) {
+    index_ = s.index;
+    previous_index_ = s.previous_index;
+    fixed_ = s.fixed;
+    norm_tolerance_ = s.norm_tolerance;
+  }
   AnalysisSurface(const Config& config, const SpatialGrids& grids, const std::vector<cplx>& initial_specular);
   int index() const { return index_; }
   bool fixed() const { return fixed_; }

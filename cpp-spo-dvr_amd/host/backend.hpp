@@ -64,6 +64,9 @@ class PropagationBackend {
   virtual void analyse(const AnalysisSurface& surface, double time, std::vector<cplx>* coefficients) = 0;
   // The FP64 sums (is, v, ien).
   virtual std::vector<cplx> s_matrix_sums() = 0;
+  // Adds the coefficients of an analysis step of an earlier run (a restart) to the sums at its time
+  // (src/propagation/propagation.hpp:88, SMatrixAccumulator::add).
+  virtual void add_earlier_analysis_step(double time, const std::vector<cplx>& coefficients) = 0;
 
   // Energy-resolved sums at chosen (x, y) sites (src/analysis/energy_resolved.cpp:62-91): start() zeroes the sums of
   // the sites and n_energies energies; add() sums one analysis step, s += psi(site) exp(i E_tot t) dtAna and

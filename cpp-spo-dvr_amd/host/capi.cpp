@@ -6,6 +6,7 @@
 #include <string>
 
 #include "../../include/spo_host.h"
+#include "checkpoint.hpp"
 #include "driver.hpp"
 #include "json.hpp"
 
@@ -32,6 +33,8 @@ int guarded(F&& f) {
     return fail(SPO_E_INPUT, e.what());
   } catch (const spo::NotSupportedError& e) {
     return fail(SPO_E_NOT_SUPPORTED, e.what());
+  } catch (const spo::CheckpointError& e) {
+    return fail(SPO_E_CHECKPOINT, e.what());
   } catch (const std::exception& e) {
     return fail(SPO_E_OTHER, e.what());
   }
@@ -185,6 +188,13 @@ int spo_run_bk_record(const spo_run* run, int k, double* out) {
     const auto& r = run->propagation->bk_records();
     if (k < 0 || k >= static_cast<int>(r.size())) throw std::out_of_range("spo_run_bk_record: no such record");
     copy_complex(r[k], out);
+  });
+}
+
+int spo_run_write_checkpoint(spo_run* run, const char* directory) {
+  return guarded([&] {
+    if (!directory || !*directory) throw std::invalid_argument("spo_run_write_checkpoint: no directory");
+    run->propagation->write_checkpoint(directory);
   });
 }
 

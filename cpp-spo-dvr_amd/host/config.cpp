@@ -139,6 +139,11 @@ InputParameters read_input(const json::Value& d) {
   get(d, "MJMax", p.mj_max);
   if (p.mj_max < 0) p.mj_max = p.j_max;
   get(d, "ProjOp", p.projection_operator);
+  get(d, "NewCalc", p.new_calculation);
+  get(d, "tStart", p.t_start);
+  get(d, "RestartDir", p.restart_dir);
+  get(d, "RestartFormat", p.restart_format);
+  get(d, "CheckpointFormat", p.checkpoint_format);
   get(d, "tEnd", p.t_end);
   get(d, "dtStep", p.time_step);
   get(d, "nStep", p.n_steps);
@@ -320,6 +325,24 @@ Config build_config(const InputParameters& input) {
   d.fixed_z_analysis_specular = !p.dynamical_zsp_analysis;
   if (d.steps_per_analysis < 1) throw InputError("dtAna", "dtAna < dtStep");
   if (p.n_steps % d.steps_per_analysis != 0) throw InputError("nStep", "nStep is not a multiple of dtAna/dtStep");
+  // src/input/validate.cpp:90-103 (the restart and time checks).
+  if (tol::lt(p.t_start, 0.0)) throw InputError("tStart", "tStart < 0.");
+  if (p.new_calculation && !tol::eq(p.t_start, 0.0))
+    throw InputError("tStart", "This is a new calculation but tStart /= 0.");
+  if (!p.new_calculation && tol::eq(p.t_start, 0.0)) throw InputError("tStart", "This is a restart but tStart = 0.");
+  if (!p.new_calculation && p.restart_dir.empty())
+    throw InputError("RestartDir", "This is a restart but RestartDir is not given.");
+  if (p.restart_format != "native" && p.restart_format != "fortran")
+    throw InputError("RestartFormat", "RestartFormat must be \"native\" or \"fortran\".");
+  if (p.checkpoint_format != "native" && p.checkpoint_format != "fortran" && p.checkpoint_format != "both")
+    throw InputError("CheckpointFormat", "CheckpointFormat must be \"native\", \"fortran\" or \"both\".");
+  if (!p.new_calculation && p.restart_format != "native")
+    throw NotSupportedError("RestartFormat \"fortran\" (fort.8 / fort.22) is not read by this build");
+  if (p.checkpoint_format != "native")
+    throw NotSupportedError("CheckpointFormat \"" + p.checkpoint_format + "\" (Fortran files) is not written by this build");
+  if (tol::gt(p.t_start, p.t_end)) throw InputError("tStart", "tStart > tEnd.");
+  if (!tol::eq(p.t_end, p.t_start + p.n_steps * p.time_step))
+    throw InputError("tEnd", "tEnd /= tStart + nStep*dtStep.");
 
   // src/input/process.cpp:97-139 (MiscellaneousSetup, scalar part).
   const double skew = d.system.skew;

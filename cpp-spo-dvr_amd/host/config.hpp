@@ -46,6 +46,12 @@ struct InputParameters {
   int nr = 32, nx = 4, ny = 4;
   int j_max = 6, mj_max = -1;
   int projection_operator = 2;
+  // Restart (src/input/config.hpp:67, :96-99, :179-182): NewCalc = false continues the run whose checkpoint is in
+  // RestartDir, from tStart (native files; the Fortran-layout ones are not read or written by this build).
+  bool new_calculation = true;
+  double t_start = 0.0;
+  std::string restart_dir;
+  std::string restart_format = "native", checkpoint_format = "native";
   double t_end = 1000.0, time_step = 10.0;
   int n_steps = 100;
   double dt_analysis = 50.0, dt_averages = 50.0;

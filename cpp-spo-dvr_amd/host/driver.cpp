@@ -1,6 +1,9 @@
 #include "driver.hpp"
 
 #include <cmath>
+#include <filesystem>
+
+#include "checkpoint.hpp"
 
 namespace spo {
 
@@ -36,11 +39,57 @@ Propagation::Propagation(std::shared_ptr<const RunSetup> setup, const BackendFac
       backend_(factory(setup_->backend_input())),
       surface_(setup_->config, setup_->grids, setup_->initial_specular) {
   const InputParameters& p = setup_->config.input;
+  if (!p.new_calculation) restore(p.restart_dir);
   record_averages(p.t_end - p.n_steps * p.time_step);
   energy_resolved_ = make_energy_resolved_setup(setup_->config, setup_->gas);
   if (energy_resolved_.enabled())
     backend_->energy_resolved_start(energy_resolved_.site_ixy, static_cast<int>(energy_resolved_.total_hartree.size()),
                                     energy_resolved_.analysis_time_step);
+}
+
+// Restart (src/app/run.cpp:139-175, native files): the wave packet and analysis surface of the checkpoint, and the
+// earlier run's BK records, added to the S-matrix sums and kept as the first records of this run.
+void Propagation::restore(const std::string& dir) {
+  const RunSetup& s = *setup_;
+  const Config& c = s.config;
+  if (!make_energy_resolved_setup(c, s.gas).site_ixy.empty())
+    throw NotSupportedError("a restart with EnergyResolvedSites (the earlier run's sums) is not supported by this build");
+  const int n_phi = s.angular.n_phi();
+  const std::filesystem::path d(dir);
+  CheckpointInfo info;
+  backend_->read_wave_packet([&](WavePacket& psi) {
+    info = read_checkpoint((d / kCheckpointFile).string(), c, n_phi, psi.scattering, psi.n_rot, psi.n_xy, psi.n_zr,
+                           psi.specular);
+  });
+  surface_.restore(info.surface);
+  const std::optional<double> last =
+      read_bk_file((d / kBkFile).string(), c, n_phi, s.analysis, [&](double t, const std::vector<cplx>& bk) {
+        backend_->add_earlier_analysis_step(t, bk);
+        bk_.push_back(bk);
+        bk_times_.push_back(t);
+      });
+  if (last && !tol::eq(*last, info.time))
+    throw CheckpointError("the BK coefficient file in '" + dir + "' ends at t = " + std::to_string(*last) +
+                          ", the checkpoint at t = " + std::to_string(info.time));
+  if (!tol::eq(info.time, c.input.t_start))
+    throw CheckpointError("tStart = " + std::to_string(c.input.t_start) + " but the checkpoint in '" + dir +
+                          "' is at t = " + std::to_string(info.time));
+  earlier_records_ = static_cast<int>(bk_.size());
+}
+
+void Propagation::write_checkpoint(const std::string& dir) {
+  const RunSetup& s = *setup_;
+  const std::filesystem::path d(dir);
+  std::filesystem::create_directories(d);
+  const int n_phi = s.angular.n_phi();
+  if (bk_.size() != bk_times_.size())
+    throw std::logic_error("write_checkpoint needs every BK record (keep_bk_records was switched off)");
+  std::vector<std::pair<double, std::vector<cplx>>> records;
+  for (std::size_t k = 0; k < bk_.size(); ++k) records.emplace_back(bk_times_[k], bk_[k]);
+  write_bk_file((d / kBkFile).string(), s.config, n_phi, s.analysis, records);
+  const WavePacket& psi = backend_->wave_packet_on_host();
+  spo::write_checkpoint((d / kCheckpointFile).string(), s.config, n_phi, time(), psi.scattering, psi.n_rot, psi.n_xy,
+                        psi.n_zr, psi.specular, surface_.state());
 }
 
 void Propagation::energy_resolved(std::vector<cplx>* sums, std::vector<cplx>* specular_sums) {
@@ -74,7 +123,10 @@ void Propagation::step() {
   if (surface_.update(backend_->specular()) && !fixed_time_) fixed_time_ = t;
   std::vector<cplx> bk;
   backend_->analyse(surface_, t, keep_bk_ ? &bk : nullptr);
-  if (keep_bk_) bk_.push_back(std::move(bk));
+  if (keep_bk_) {
+    bk_.push_back(std::move(bk));
+    bk_times_.push_back(t);
+  }
   if (energy_resolved_.enabled()) {
     // exp(i E_tot t) per energy, formed as the reference forms it (src/analysis/energy_resolved.cpp:67-68).
     std::vector<cplx> phase(energy_resolved_.total_hartree.size());

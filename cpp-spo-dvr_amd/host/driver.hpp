@@ -30,6 +30,10 @@ struct RunSetup {
 // device_potential: evaluate the potential grid on the GPU (compute_potential_grid_on_device) instead of the host.
 std::shared_ptr<RunSetup> make_run_setup(const InputParameters& input, bool device_potential = false);
 
+// File names of the native restart files (src/app/run.cpp:67-68).
+inline constexpr const char* kCheckpointFile = "checkpoint.bin";
+inline constexpr const char* kBkFile = "bk_coefficients.bin";
+
 class Propagation {
  public:
   Propagation(std::shared_ptr<const RunSetup> setup, const BackendFactory& factory);
@@ -47,7 +51,12 @@ class Propagation {
 
   // Averages at tStart (before any step) and every dtAve.
   const std::vector<Averages>& averages() const { return averages_; }
+  // The BK coefficients of every analysis step: a restart's earlier records first.
   const std::vector<std::vector<cplx>>& bk_records() const { return bk_; }
+  int earlier_records() const { return earlier_records_; }
+  // The native checkpoint and BK coefficient files of the run so far (checkpoint.bin, bk_coefficients.bin in dir),
+  // for a restart with NewCalc = false, RestartDir = dir, tStart = time().
+  void write_checkpoint(const std::string& dir);
   void keep_bk_records(bool keep) { keep_bk_ = keep; }
   const AnalysisSurface& surface() const { return surface_; }
   std::optional<double> surface_fixed_time() const { return fixed_time_; }
@@ -61,6 +70,7 @@ class Propagation {
 
  private:
   void record_averages(double time);
+  void restore(const std::string& dir);
 
   std::shared_ptr<const RunSetup> setup_;
   std::unique_ptr<PropagationBackend> backend_;
@@ -69,6 +79,8 @@ class Propagation {
   bool keep_bk_ = true;
   std::vector<Averages> averages_;
   std::vector<std::vector<cplx>> bk_;
+  std::vector<double> bk_times_;
+  int earlier_records_ = 0;
   std::optional<double> fixed_time_;
   EnergyResolvedSetup energy_resolved_;
 };

@@ -29,6 +29,7 @@ class HipBackend final : public PropagationBackend {
   void read_wave_packet(const std::function<void(WavePacket&)>& read) override {
     read(host_);
     check(spo_hip_upload_wave(handle_, d(host_.scattering), d(host_.specular)));
+    specular_ = host_.specular;
   }
   const WavePacket& wave_packet_on_host() override {
     check(spo_hip_download_wave(handle_, reinterpret_cast<double*>(host_.scattering.data()),
@@ -78,6 +79,10 @@ class HipBackend final : public PropagationBackend {
       out = reinterpret_cast<double*>(coefficients->data());
     }
     check(spo_hip_analyse(handle_, surface.fixed() ? 1 : 0, surface.index(), time, out));
+  }
+  void add_earlier_analysis_step(double time, const std::vector<cplx>& coefficients) override {
+    if (coefficients.size() != n_coefficients_) throw std::invalid_argument("BK record of the wrong size");
+    check(spo_hip_accumulate(handle_, time, reinterpret_cast<const double*>(coefficients.data())));
   }
   std::vector<cplx> s_matrix_sums() override {
     std::vector<cplx> out(n_sums_);

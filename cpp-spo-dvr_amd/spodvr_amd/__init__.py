@@ -85,6 +85,7 @@ def host_library() -> ctypes.CDLL:
         lib.spo_run_results.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
                                         ctypes.c_void_p, ctypes.c_void_p]
         lib.spo_run_s_sums.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+        lib.spo_run_write_checkpoint.argtypes = [ctypes.c_void_p, ctypes.c_char_p]
         lib.spo_run_energy_resolved_shape.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int),
                                                       ctypes.POINTER(ctypes.c_int)]
         lib.spo_run_energy_resolved.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
@@ -290,6 +291,11 @@ class Run:
         out = np.zeros(self.info.n_energies * self.info.n_states * self.info.n_v, dtype=np.complex128)
         _check(host_library().spo_run_s_sums(self._h, _ptr(out)))
         return out
+
+    def write_checkpoint(self, directory: str) -> None:
+        """checkpoint.bin and bk_coefficients.bin of the run so far (the reference's native restart files); a run
+        with {"NewCalc": False, "RestartDir": directory, "tStart": <this time>} continues from them."""
+        _check(host_library().spo_run_write_checkpoint(self._h, os.fspath(directory).encode()))
 
     def kernel_timing(self, on: bool) -> None:
         """Starts (and clears) or stops per-kernel HIP-event timing on the backend's stream."""

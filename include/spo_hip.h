@@ -149,6 +149,11 @@ int spo_hip_energy_resolved_start(spo_hip_backend* backend, int n_sites, const i
 int spo_hip_energy_resolved_add(spo_hip_backend* backend, const double* phase_factors);
 int spo_hip_energy_resolved_read(spo_hip_backend* backend, double* sums, double* specular_sums);
 
+/* Adds BK coefficients (n_states * n_v complex, (is, v)) of an analysis step of an earlier run at its time to the
+ * S-matrix sums, as spo_hip_analyse does for its own; for a restart. Replaces SMatrixAccumulator::add of
+ * WavePacketPropagation::add_earlier_analysis_step (src/propagation/propagation.hpp:88). */
+int spo_hip_accumulate(spo_hip_backend* backend, double time, const double* coefficients);
+
 /* Kernel timing with HIP events recorded on the backend's stream around every launch (no counterpart in the
  * reference; for the roofline measurement). Enabling clears the record. */
 typedef struct spo_hip_kernel_time {

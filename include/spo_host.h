@@ -17,7 +17,8 @@ enum {
   SPO_OK = 0,
   SPO_E_OTHER = 1,         /* device or other runtime error */
   SPO_E_INPUT = 2,         /* invalid input */
-  SPO_E_NOT_SUPPORTED = 3  /* input or device not supported by this build */
+  SPO_E_NOT_SUPPORTED = 3, /* input or device not supported by this build */
+  SPO_E_CHECKPOINT = 4     /* a checkpoint or BK file missing, corrupt or written for another input */
 };
 
 enum {
@@ -74,6 +75,11 @@ int spo_run_site_column(spo_run* run, int ixy, double* out);           /* n_zr*n
 /* Recorded BK coefficients of analysis step k (0-based), n_states*n_v complex. */
 int spo_run_bk_record(const spo_run* run, int k, double* out);
 int spo_run_bk_count(const spo_run* run, int* count);
+/* Writes the native restart files of the run so far into directory (created if missing): checkpoint.bin (wave
+ * packet, time, analysis surface) and bk_coefficients.bin (every BK record, a restart's earlier ones first), in
+ * the reference's format version 2 (src/io/checkpoint.cpp:216-309; files named as src/app/run.cpp:67-68).
+ * A run made with NewCalc = false, RestartDir = directory and tStart = that time continues from them. */
+int spo_run_write_checkpoint(spo_run* run, const char* directory);
 /* Averages rows so far: count, then rows of (time, norm, z, r, rest). */
 int spo_run_averages(const spo_run* run, double* rows, int capacity, int* count);
 /* The analysis surface: fixed flag, index and Z of the fixed line (or 0). */

@@ -65,6 +65,9 @@ class OracleBackend final : public PropagationBackend {
   void site_column(int ixy, cplx* out) const override;
   void analyse(const spo::AnalysisSurface& surface, double time, std::vector<cplx>* coefficients) override;
   std::vector<cplx> s_matrix_sums() override { return sums_; }
+  void add_earlier_analysis_step(double time, const std::vector<cplx>& coefficients) override {
+    accumulate(time, coefficients);
+  }
 
   // Energy-resolved sums as EnergyResolvedAccumulator (src/analysis/energy_resolved.cpp:59-91): each site's column
   // read through site_column, [site, energy, izr, irot] and [energy, iz_specular, ir].
@@ -77,6 +80,7 @@ class OracleBackend final : public PropagationBackend {
 
   // One T block (specular, T_Zr, T_xy, V unless kinetic_only) with full or half kinetic factors.
   void kinetic_and_potential(bool half, bool kinetic_only) override;
+  void accumulate(double time, const std::vector<cplx>& bk);
 
  private:
   void specular_step(bool half, bool kinetic_only);
@@ -443,14 +447,19 @@ void OracleBackend::analyse(const spo::AnalysisSurface& surface, double time, st
       bk[a.initial + std::size_t(n_states) * v] = s * dr;
     }
   }
-  const double dt_ana = d.steps_per_analysis * in_.config.input.time_step;
+  accumulate(time, bk);
+  if (coefficients) *coefficients = std::move(bk);
+}
+
+// A(E) += c exp(i E t) dtAna (src/analysis/s_matrix.cpp:49-56).
+void OracleBackend::accumulate(double time, const std::vector<cplx>& bk) {
+  const double dt_ana = in_.config.derived.steps_per_analysis * in_.config.input.time_step;
   const std::size_t n = bk.size();
   for (int ien = 0; ien < in_.energies.size(); ++ien) {
     const cplx phase = std::exp(cplx(0.0, 1.0) * in_.energies.total_hartree[ien] * time);
     cplx* s = sums_.data() + n * ien;
     for (std::size_t k = 0; k < n; ++k) s[k] = s[k] + bk[k] * phase * dt_ana;
   }
-  if (coefficients) *coefficients = std::move(bk);
 }
 
 }  // namespace

@@ -1,0 +1,181 @@
+"""Native restart files (checkpoint.bin, bk_coefficients.bin) on the CPU oracle: a split run is bitwise the
+unsplit one (the reference's tests/test_restart.cpp:241), the files carry format version 2 with the reference's
+35-entry parameter list (:571-611), and mismatched or damaged files are refused with the reference's messages
+(:295, :332, :923-1040)."""
+
+import os
+import shutil
+import struct
+
+import numpy as np
+import pytest
+
+import spodvr_amd
+from oracle import oracle
+from tests import cases
+
+# The parameter names of format version 2 without low-Z absorbers, in file order (tests/test_restart.cpp:574-609).
+FORMAT_2 = ["ZStart", "dZ", "nZ", "nZSpecular", "RStart", "dR", "nR", "nX", "nY", "JMax", "MJMax",
+            "number of phi points (MJMax, nPhisOptimal)", "Mass1", "Mass2", "HomoNuc", "dtStep", "dtAna", "Zana",
+            "Z0", "EMinWP", "EMaxWP", "EParallel", "PhiParallel", "VIn", "JIn", "MJIn", "ZOptStart", "ZOptEnd",
+            "ROptStart", "ROptEnd", "ZSpOptStart", "ZSpOptEnd", "ZOptFactor (effective)", "ROptFactor (effective)",
+            "ZSpOptFactor (effective)"]
+
+FIRST = {"tEnd": 40.0, "nStep": 20}
+SECOND = {"NewCalc": False, "tStart": 40.0, "tEnd": 80.0, "nStep": 20}
+
+
+@pytest.fixture(scope="module")
+def split(tmp_path_factory):
+    """The unsplit 40-step run, and its first 20 steps with their restart files."""
+    base = tmp_path_factory.mktemp("restart")
+    path = cases.write_case("restart_smoke", cases.SMOKE_DAILIGHT, str(base))
+    full = oracle.run(path)
+    full.steps(full.info.analysis_steps_total)
+    whole = {"wave": full.get_wave(), "sums": full.s_sums(), "bk": full.bk_records(), "results": full.results()}
+    full.close()
+    first = oracle.run(path, FIRST)
+    first.steps(first.info.analysis_steps_total)
+    first_dir = base / "first"
+    first.write_checkpoint(str(first_dir))
+    first.close()
+    return path, first_dir, whole
+
+
+def _restart(path, directory, **extra):
+    return oracle.run(path, dict(SECOND, RestartDir=str(directory), **extra))
+
+
+def _error(path, directory, **extra):
+    with pytest.raises(spodvr_amd.SpoError) as e:
+        _restart(path, directory, **extra)
+    return e.value
+
+
+def test_split_run_is_bitwise_the_unsplit_run(split):
+    path, first_dir, whole = split
+    second = _restart(path, first_dir)
+    second.steps(second.info.analysis_steps_total)
+    sg, pg = second.get_wave()
+    assert np.array_equal(sg, whole["wave"][0]) and np.array_equal(pg, whole["wave"][1])
+    assert np.array_equal(second.s_sums(), whole["sums"])
+    assert np.array_equal(second.bk_records(), whole["bk"])
+    results = second.results()
+    for key in ("s", "p", "p_reaction"):
+        assert np.array_equal(results[key], whole["results"][key]), key
+    second.close()
+
+
+def _read_header(data):
+    """(magic, version, system id, [(name, value)], offset after the header) of a native file."""
+    magic = data[:16]
+    version, = struct.unpack_from("<I", data, 16)
+    n, = struct.unpack_from("<Q", data, 20)
+    system_id = data[28:28 + n].decode()
+    off = 28 + n
+    count, = struct.unpack_from("<Q", data, off)
+    off += 8
+    params = []
+    for _ in range(count):
+        n, = struct.unpack_from("<Q", data, off)
+        name = data[off + 8:off + 8 + n].decode()
+        value, = struct.unpack_from("<d", data, off + 8 + n)
+        params.append((name, value))
+        off += 8 + n + 8
+    return magic, version, system_id, params, off
+
+
+def test_files_have_format_version_2_and_its_parameter_list(split):
+    path, first_dir, _ = split
+    first = oracle.run(path, FIRST)
+    info = first.info
+    first.close()
+    for name, magic in (("checkpoint.bin", b"SPODVR-CHECKPNT\0"), ("bk_coefficients.bin", b"SPODVR-BKCOEFFS\0")):
+        data = (first_dir / name).read_bytes()
+        m, version, system_id, params, off = _read_header(data)
+        assert (m, version, system_id) == (magic, 2, "H2Cu111DaiLight"), name
+        assert [p[0] for p in params] == FORMAT_2, name
+        values = dict(params)
+        assert values["nZ"] == 64 and values["dtStep"] == 2.0 and values["MJMax"] == 4 and values["JMax"] == 4
+        assert values["number of phi points (MJMax, nPhisOptimal)"] == info.n_phi
+    data = (first_dir / "checkpoint.bin").read_bytes()
+    *_, off = _read_header(data)
+    time, = struct.unpack_from("<d", data, off)
+    shape = struct.unpack_from("<5i", data, off + 8)
+    n_xy, n_zr = info.nx * info.ny, info.nr * info.nz
+    assert time == 40.0 and shape == (info.n_rot, n_xy, n_zr, info.nr, info.nz_specular)
+    n_values = info.nr * info.nz_specular + info.n_rot * n_xy * n_zr
+    assert len(data) == off + 8 + 20 + 4 + 4 + 1 + 8 + 16 * n_values + 8
+    assert struct.unpack_from("<Q", data, len(data) - 8)[0] == 0x454E444F464350
+
+
+def test_mismatching_input_is_an_error(split):
+    path, first_dir, _ = split
+    e = _error(path, first_dir, dtStep=1.0, nStep=40)
+    assert e.code == 4 and "dtStep is 1 in the input but 2 in checkpoint" in str(e) and "same parameters" in str(e)
+
+
+def test_tstart_must_be_the_checkpoint_time(split):
+    path, first_dir, _ = split
+    e = _error(path, first_dir, tStart=42.0, tEnd=82.0)
+    assert e.code == 4 and "tStart = 42" in str(e) and "is at t = 40" in str(e)
+
+
+def test_restart_without_restart_dir_is_an_input_error(split):
+    path, _, _ = split
+    with pytest.raises(spodvr_amd.SpoError) as e:
+        oracle.run(path, SECOND)
+    assert e.value.code == 2 and "RestartDir" in str(e.value)
+
+
+def _patched(first_dir, tmp_path, name, patch):
+    d = tmp_path / "patched"
+    shutil.copytree(first_dir, d)
+    data = bytearray((d / name).read_bytes())
+    patch(data)
+    (d / name).write_bytes(bytes(data))
+    return d
+
+
+@pytest.mark.parametrize("name", ["checkpoint.bin", "bk_coefficients.bin"])
+def test_format_version_is_checked(split, tmp_path, name):
+    path, first_dir, _ = split
+    d = _patched(first_dir, tmp_path, name, lambda b: struct.pack_into("<I", b, 16, 3))
+    e = _error(path, d)
+    assert e.code == 4 and f"unsupported format version 3 in" in str(e) and name in str(e)
+
+
+def test_end_marker_is_checked(split, tmp_path):
+    path, first_dir, _ = split
+
+    def flip(b):
+        b[-1] ^= 1
+    d = _patched(first_dir, tmp_path, "checkpoint.bin", flip)
+    e = _error(path, d)
+    assert e.code == 4 and "is corrupt (bad end marker)" in str(e)
+
+
+def test_truncated_checkpoint_is_an_error(split, tmp_path):
+    path, first_dir, _ = split
+    d = _patched(first_dir, tmp_path, "checkpoint.bin", lambda b: b.__delitem__(slice(len(b) - 100, len(b))))
+    e = _error(path, d)
+    assert e.code == 4 and "ends too early" in str(e)
+
+
+def test_system_id_is_compared_without_case(split, tmp_path):
+    path, first_dir, whole = split
+
+    def lower(b):
+        b[28:28 + len("H2Cu111DaiLight")] = b"h2cu111dailight"
+    d = _patched(first_dir, tmp_path, "checkpoint.bin", lower)
+    run = _restart(path, d)
+    run.close()
+
+
+def test_analysis_surface_index_is_checked(split, tmp_path):
+    path, first_dir, _ = split
+    data = (first_dir / "checkpoint.bin").read_bytes()
+    *_, off = _read_header(data)
+    d = _patched(first_dir, tmp_path, "checkpoint.bin", lambda b: struct.pack_into("<i", b, off + 28, 4096))
+    e = _error(path, d)
+    assert e.code == 4 and "analysis surface index 4096 is outside the specular grid" in str(e)
