@@ -165,7 +165,7 @@ void set_lds(const void* kernel, std::size_t bytes) {
   X(48, 8, 12, 4) X(48, 4, 12, 4) X(48, 16, 12, 4) X(22, 8, 11, 2) X(12, 8, 4, 3) X(10, 8, 5, 2)             \
   X(8, 8, 4, 2) X(56, 8, 8, 7) X(56, 4, 8, 7)
 // T_xy tiles with whole-line DFTs in registers (NX = NY a single radix; C izr per tile, C * NX threads).
-#define SPO_XY_REG_SHAPES(X) X(16, 16, 16) X(16, 16, 8) X(8, 8, 32) X(8, 8, 16) X(4, 4, 64) X(4, 4, 32)
+#define SPO_XY_REG_SHAPES(X) X(16, 16, 32) X(16, 16, 16) X(16, 16, 8) X(8, 8, 32) X(8, 8, 16) X(4, 4, 64) X(4, 4, 32)
 
 // A launch variant of a kernel family: the kernel, its block size, its tile parameter (T_xy: izr per tile;
 // interaction: wavefronts) and its LDS bytes.
@@ -202,7 +202,7 @@ std::vector<Variant> xy_reg_variants(int nx, int ny) {
                    std::size_t(A) * (B * (CC + 1) + 1) * sizeof(cx<T>)});
   SPO_XY_REG_SHAPES(X)
 #undef X
-  // Default (first) variant, as measured on MI355X at 16 x 16: FP64 tiles of 8 izr, complex64 tiles of 16.
+  // Default (first) variant, as measured on MI355X at 16 x 16: FP64 tiles of 8 izr, complex64 tiles of 32.
   std::stable_sort(out.begin(), out.end(), [](const Variant& a, const Variant& b) {
     return sizeof(T) == 8 ? a.param < b.param : a.param > b.param;
   });
