@@ -6,6 +6,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
+#include <cstring>
 #include <filesystem>
 #include <fstream>
 #include <sstream>
@@ -39,6 +40,7 @@ class Out {
     bytes(s.data(), s.size());
   }
   void put_array(const std::vector<cplx>& a) { bytes(a.data(), a.size() * sizeof(cplx)); }
+  void put_bytes(const void* p, std::size_t n) { bytes(p, n); }
   void close() {
     f_.close();
     if (!f_) throw CheckpointError("error writing '" + path_ + "'");
@@ -73,6 +75,7 @@ class In {
     return s;
   }
   void get_array(std::vector<cplx>& a) { bytes(a.data(), a.size() * sizeof(cplx)); }
+  void get_bytes(void* p, std::size_t n) { bytes(p, n); }
   bool at_end() { return f_.peek() == std::ifstream::traits_type::eof(); }
   const std::string& path() const { return path_; }
 
@@ -303,6 +306,241 @@ std::optional<double> read_bk_file(const std::string& path, const Config& c, int
     last = time;
   }
   return last;
+}
+
+namespace {
+
+// gfortran sequential unformatted records: int32 length, the bytes, int32 length. Records above the largest
+// subrecord (2^31 - 9 bytes) would be split by gfortran; none of these files has one.
+class FortranOut {
+ public:
+  explicit FortranOut(const std::string& path) : out_(path) {}
+  void record(const void* p, std::size_t n) {
+    if (n > 2147483639u) throw CheckpointError("a Fortran record of " + std::to_string(n) + " bytes needs subrecords");
+    const auto len = static_cast<std::int32_t>(n);
+    out_.put(len);
+    out_.put_bytes(p, n);
+    out_.put(len);
+  }
+  void record(const std::vector<unsigned char>& b) { record(b.data(), b.size()); }
+  void close() { out_.close(); }
+
+ private:
+  Out out_;
+};
+
+class FortranIn {
+ public:
+  explicit FortranIn(const std::string& path) : in_(path) {}
+  // The bytes of the next record (what names it in the error message).
+  std::vector<unsigned char> record(const std::string& what) {
+    if (in_.at_end()) throw CheckpointError("'" + in_.path() + "' ends before " + what);
+    const auto n = in_.get<std::int32_t>();
+    if (n < 0) throw CheckpointError("'" + in_.path() + "': " + what + " is split into subrecords");
+    std::vector<unsigned char> b(static_cast<std::size_t>(n));
+    in_.get_bytes(b.data(), b.size());
+    if (in_.get<std::int32_t>() != n) throw CheckpointError("'" + in_.path() + "': bad record marker after " + what);
+    return b;
+  }
+  const std::string& path() const { return in_.path(); }
+
+ private:
+  In in_;
+};
+
+struct Bytes {
+  std::vector<unsigned char> b;
+  template <typename T>
+  Bytes& add(const T& v) {
+    const auto* p = reinterpret_cast<const unsigned char*>(&v);
+    b.insert(b.end(), p, p + sizeof(T));
+    return *this;
+  }
+  template <typename T>
+  Bytes& add_all(const std::vector<T>& v) {
+    for (const T& x : v) add(x);
+    return *this;
+  }
+};
+
+// Values of one record in order.
+struct Fields {
+  const std::vector<unsigned char>& b;
+  std::string what, path;
+  std::size_t at = 0;
+  template <typename T>
+  T next() {
+    if (at + sizeof(T) > b.size()) throw CheckpointError("'" + path + "': " + what + " is too short");
+    T v;
+    std::memcpy(&v, b.data() + at, sizeof(T));
+    at += sizeof(T);
+    return v;
+  }
+  void end() const {
+    if (at != b.size()) throw CheckpointError("'" + path + "': " + what + " is longer than expected");
+  }
+};
+
+std::array<std::vector<std::int32_t>, 6> state_labels(const AnalysisBasis& basis) {
+  std::array<std::vector<std::int32_t>, 6> labels;
+  for (int is = 0; is < basis.n_states(); ++is) {
+    const DiffractionChannel& ch = basis.channels[basis.state_diffraction(is)];
+    labels[0].push_back(basis.state_rotation(is) + 1);
+    labels[1].push_back(basis.state_diffraction(is) + 1);
+    labels[2].push_back(ch.n);
+    labels[3].push_back(ch.m);
+    labels[4].push_back(basis.state_j[is]);
+    labels[5].push_back(basis.state_mj[is]);
+  }
+  return labels;
+}
+
+double analysis_step(const Config& c) { return c.input.time_step * c.derived.steps_per_analysis; }
+
+void expect(std::int32_t stored, int expected, const std::string& what, const std::string& path) {
+  if (stored != expected)
+    throw CheckpointError(what + " is " + std::to_string(expected) + " but " + std::to_string(stored) + " in '" + path +
+                          "'");
+}
+
+}  // namespace
+
+void write_fortran_wave_function(const std::string& path, const std::vector<cplx>& scattering, int n_rot, int n_xy,
+                                 int n_zr, const std::vector<cplx>& specular, const AnalysisSurfaceState& surface) {
+  FortranOut f(path);
+  f.record(Bytes().add<std::int32_t>(2).b);
+  f.record(specular.data(), specular.size() * sizeof(cplx));
+  f.record(Bytes().add<std::int32_t>(surface.fixed ? 1 : 0).add<std::int32_t>(surface.index + 1).b);
+  const std::size_t block = std::size_t(n_rot) * n_xy;
+  for (int izr = 0; izr < n_zr; ++izr) f.record(scattering.data() + block * izr, block * sizeof(cplx));
+  f.close();
+}
+
+AnalysisSurfaceState read_fortran_wave_function(const std::string& path, std::vector<cplx>& scattering, int n_rot,
+                                                int n_xy, int n_zr, std::vector<cplx>& specular, int nr, int n_z_specular,
+                                                const AnalysisSurfaceState& fresh) {
+  FortranIn f(path);
+  {
+    const auto b = f.record("record 1 (nproj)");
+    Fields r{b, "record 1 (nproj)", path};
+    expect(r.next<std::int32_t>(), 2, "ProjOp", path);
+    r.end();
+  }
+  specular.assign(std::size_t(nr) * n_z_specular, cplx(0.0, 0.0));
+  {
+    const auto b = f.record("record 2 (SpecWave)");
+    if (b.size() != specular.size() * sizeof(cplx))
+      throw CheckpointError("'" + path + "': the specular wave (record 2) does not have nR x nZSpecular values");
+    std::memcpy(specular.data(), b.data(), b.size());
+  }
+  AnalysisSurfaceState state = fresh;
+  {
+    const auto b = f.record("record 3 (FixedZAnalysisSpec, izanlySpecChan)");
+    Fields r{b, "record 3 (FixedZAnalysisSpec, izanlySpecChan)", path};
+    state.fixed = r.next<std::int32_t>() != 0;
+    const std::int32_t iz = r.next<std::int32_t>();
+    r.end();
+    if (iz < 1 || iz > n_z_specular)
+      throw CheckpointError("'" + path + "': izanlySpecChan " + std::to_string(iz) +
+                            " is outside the specular grid, 1 .. " + std::to_string(n_z_specular));
+    state.index = iz - 1;
+    state.previous_index = state.index;
+  }
+  const std::size_t block = std::size_t(n_rot) * n_xy;
+  scattering.assign(block * n_zr, cplx(0.0, 0.0));
+  for (int izr = 0; izr < n_zr; ++izr) {
+    const std::string what = "ScatWave record " + std::to_string(izr + 1);
+    const auto b = f.record(what);
+    if (b.size() != block * sizeof(cplx))
+      throw CheckpointError("'" + path + "': " + what + " does not hold n_rot x nX x nY values");
+    std::memcpy(scattering.data() + block * izr, b.data(), b.size());
+  }
+  return state;
+}
+
+void write_fortran_bk_file(const std::string& path, const Config& c, const AnalysisBasis& basis,
+                           const FortranBkHeader& h, const std::vector<std::vector<cplx>>& records) {
+  const InputParameters& p = c.input;
+  const DerivedParameters& d = c.derived;
+  const int n_dif_total = static_cast<int>(basis.channels.size());
+  FortranOut f(path);
+  f.record(Bytes()
+               .add<std::int32_t>(h.n_records)
+               .add<std::int32_t>(n_dif_total * h.n_rotations)
+               .add<std::int32_t>(n_dif_total)
+               .add<std::int32_t>(h.n_rotations)
+               .b);
+  f.record(Bytes().add(analysis_step(c)).add(d.kz).add(p.z0).add(d.wave_packet_alpha).add(p.z_analysis)
+               .add(h.initial_energy).b);
+  f.record(Bytes()
+               .add(d.translational_energy)
+               .add(d.theta_incidence)
+               .add(d.phi_incidence)
+               .add<std::int32_t>(p.j_in)
+               .add<std::int32_t>(p.mj_in)
+               .add<std::int32_t>(h.initial_channel + 1)
+               .b);
+  for (const auto& labels : state_labels(basis)) f.record(Bytes().add_all(labels).b);
+  std::vector<double> energies;
+  for (int v = 0; v < basis.n_v; ++v)
+    for (int is = 0; is < basis.n_states(); ++is)
+      energies.push_back(basis.channel_energies[is + std::size_t(basis.n_states()) * v]);
+  f.record(Bytes().add_all(energies).b);
+  for (const auto& bk : records) f.record(bk.data(), bk.size() * sizeof(cplx));
+  f.close();
+}
+
+int read_fortran_bk_file(const std::string& path, const Config& c, const AnalysisBasis& basis, int n_rotations,
+                         const std::function<void(double, const std::vector<cplx>&)>& on_record) {
+  FortranIn f(path);
+  const int n_dif_total = static_cast<int>(basis.channels.size());
+  {
+    const auto b = f.record("record 1 (sizes)");
+    Fields r{b, "record 1 (sizes)", path};
+    (void)r.next<std::int32_t>();  // the records the file was meant to hold
+    expect(r.next<std::int32_t>(), n_dif_total * n_rotations, "nstatw", path);
+    expect(r.next<std::int32_t>(), n_dif_total, "ndftow", path);
+    expect(r.next<std::int32_t>(), n_rotations, "nrotw", path);
+    r.end();
+  }
+  const double deltat = analysis_step(c);
+  {
+    const auto b = f.record("record 2 (deltat, ...)");
+    Fields r{b, "record 2 (deltat, ...)", path};
+    if (r.next<double>() != deltat)
+      throw CheckpointError("the analysis time step (dtAna) differs from the one in '" + path + "'");
+  }
+  (void)f.record("record 3");
+  const auto expected = state_labels(basis);
+  for (int k = 0; k < 6; ++k) {
+    const std::string what = "record " + std::to_string(4 + k);
+    const auto b = f.record(what);
+    if (b.size() != expected[k].size() * sizeof(std::int32_t))
+      throw CheckpointError("the analysis basis (nstata) differs from the one in '" + path + "'");
+    Fields r{b, what, path};
+    for (const std::int32_t label : expected[k])
+      if (r.next<std::int32_t>() != label)
+        throw CheckpointError("the analysis basis differs from the one in '" + path + "'");
+  }
+  (void)f.record("record 10 (channel energies)");
+  const std::size_t n = std::size_t(basis.n_states()) * basis.n_v;
+  std::vector<cplx> bk(n);
+  int count = 0;
+  for (;;) {
+    std::vector<unsigned char> b;
+    try {
+      b = f.record("a BK record");
+    } catch (const CheckpointError& e) {
+      if (std::string(e.what()).find("ends before") != std::string::npos) break;
+      throw;
+    }
+    if (b.size() != n * sizeof(cplx))
+      throw CheckpointError("'" + path + "': BK record " + std::to_string(count + 1) + " has the wrong length");
+    std::memcpy(bk.data(), b.data(), b.size());
+    ++count;
+    on_record(count * deltat, bk);
+  }
+  return count;
 }
 
 }  // namespace spo

@@ -49,4 +49,35 @@ void write_bk_file(const std::string& path, const Config& config, int n_phi, con
 std::optional<double> read_bk_file(const std::string& path, const Config& config, int n_phi, const AnalysisBasis& basis,
                                    const std::function<void(double, const std::vector<cplx>&)>& on_record);
 
+// Restart files in the Fortran program's layout (gfortran sequential unformatted: every record framed by int32
+// byte counts; INTEGER 4 bytes, REAL 8, COMPLEX 16), so that runs continue across the two programs
+// (src/io/fortran_files.hpp):
+//   wave function (written as fort.9, read as fort.8): nproj = 2; the specular wave; (fixed, surface index + 1);
+//               then one record per izr with the scattering block (irot, ixy);
+//   BK file (written as fort.21, read as fort.22): (n_records, nstatw, ndftow, nrotw); (dtAna, kz, Z0, alpha,
+//               Zana, E(vin, jin)); (E_trans, theta_in, phi_in, JIn, MJIn, initial channel + 1); six label arrays
+//               of the analysed states (rotation + 1, diffraction + 1, n, m, J, mJ); the channel energies (is, v);
+//               then one record of coefficients (is, v) per analysis step.
+struct FortranBkHeader {
+  int n_records = 0;  // the records the file will hold
+  int n_rotations = 0, initial_channel = 0;
+  double initial_energy = 0.0;  // hartree
+};
+
+void write_fortran_wave_function(const std::string& path, const std::vector<cplx>& scattering, int n_rot, int n_xy,
+                                 int n_zr, const std::vector<cplx>& specular, const AnalysisSurfaceState& surface);
+
+// The Fortran file keeps the surface position and whether it is fixed; the rest of the state comes from `fresh`
+// (a new surface for the same input), with the previous index equal to the index.
+AnalysisSurfaceState read_fortran_wave_function(const std::string& path, std::vector<cplx>& scattering, int n_rot,
+                                                int n_xy, int n_zr, std::vector<cplx>& specular, int nr, int n_z_specular,
+                                                const AnalysisSurfaceState& fresh);
+
+void write_fortran_bk_file(const std::string& path, const Config& config, const AnalysisBasis& basis,
+                           const FortranBkHeader& header, const std::vector<std::vector<cplx>>& records);
+
+// Calls on_record(k dtAna, coefficients) for record k = 1, 2, ...; returns the number of records.
+int read_fortran_bk_file(const std::string& path, const Config& config, const AnalysisBasis& basis, int n_rotations,
+                         const std::function<void(double, const std::vector<cplx>&)>& on_record);
+
 }  // namespace spo

@@ -336,10 +336,6 @@ Config build_config(const InputParameters& input) {
     throw InputError("RestartFormat", "RestartFormat must be \"native\" or \"fortran\".");
   if (p.checkpoint_format != "native" && p.checkpoint_format != "fortran" && p.checkpoint_format != "both")
     throw InputError("CheckpointFormat", "CheckpointFormat must be \"native\", \"fortran\" or \"both\".");
-  if (!p.new_calculation && p.restart_format != "native")
-    throw NotSupportedError("RestartFormat \"fortran\" (fort.8 / fort.22) is not read by this build");
-  if (p.checkpoint_format != "native")
-    throw NotSupportedError("CheckpointFormat \"" + p.checkpoint_format + "\" (Fortran files) is not written by this build");
   if (tol::gt(p.t_start, p.t_end)) throw InputError("tStart", "tStart > tEnd.");
   if (!tol::eq(p.t_end, p.t_start + p.n_steps * p.time_step))
     throw InputError("tEnd", "tEnd /= tStart + nStep*dtStep.");

@@ -56,21 +56,40 @@ void Propagation::restore(const std::string& dir) {
     throw NotSupportedError("a restart with EnergyResolvedSites (the earlier run's sums) is not supported by this build");
   const int n_phi = s.angular.n_phi();
   const std::filesystem::path d(dir);
+  auto add_record = [&](double t, const std::vector<cplx>& bk) {
+    backend_->add_earlier_analysis_step(t, bk);
+    bk_.push_back(bk);
+    bk_times_.push_back(t);
+  };
   CheckpointInfo info;
-  backend_->read_wave_packet([&](WavePacket& psi) {
-    info = read_checkpoint((d / kCheckpointFile).string(), c, n_phi, psi.scattering, psi.n_rot, psi.n_xy, psi.n_zr,
-                           psi.specular);
-  });
-  surface_.restore(info.surface);
-  const std::optional<double> last =
-      read_bk_file((d / kBkFile).string(), c, n_phi, s.analysis, [&](double t, const std::vector<cplx>& bk) {
-        backend_->add_earlier_analysis_step(t, bk);
-        bk_.push_back(bk);
-        bk_times_.push_back(t);
-      });
-  if (last && !tol::eq(*last, info.time))
-    throw CheckpointError("the BK coefficient file in '" + dir + "' ends at t = " + std::to_string(*last) +
-                          ", the checkpoint at t = " + std::to_string(info.time));
+  if (c.input.restart_format == "native") {
+    backend_->read_wave_packet([&](WavePacket& psi) {
+      info = read_checkpoint((d / kCheckpointFile).string(), c, n_phi, psi.scattering, psi.n_rot, psi.n_xy, psi.n_zr,
+                             psi.specular);
+    });
+    surface_.restore(info.surface);
+    const std::optional<double> last = read_bk_file((d / kBkFile).string(), c, n_phi, s.analysis, add_record);
+    if (last && !tol::eq(*last, info.time))
+      throw CheckpointError("the BK coefficient file in '" + dir + "' ends at t = " + std::to_string(*last) +
+                            ", the checkpoint at t = " + std::to_string(info.time));
+  } else {
+    // Fortran files: fort.8 (else fort.9) and fort.22 (else fort.21); the wave function file has no time, the
+    // BK records give it (src/app/run.cpp:158-170).
+    auto first_existing = [&](const char* a, const char* b) {
+      if (std::filesystem::exists(d / a)) return (d / a).string();
+      if (std::filesystem::exists(d / b)) return (d / b).string();
+      throw InputError("RestartDir", std::string("neither ") + a + " nor " + b + " in '" + dir + "'");
+    };
+    const AnalysisSurfaceState fresh = surface_.state();
+    backend_->read_wave_packet([&](WavePacket& psi) {
+      info.surface = read_fortran_wave_function(first_existing(kFortranWaveIn, kFortranWaveOut), psi.scattering,
+                                                psi.n_rot, psi.n_xy, psi.n_zr, psi.specular, c.input.nr,
+                                                c.input.nz_specular, fresh);
+    });
+    surface_.restore(info.surface);
+    read_fortran_bk_file(first_existing(kFortranBkIn, kFortranBkOut), c, s.analysis, s.basis.size(), add_record);
+    info.time = bk_times_.empty() ? 0.0 : bk_times_.back();
+  }
   if (!tol::eq(info.time, c.input.t_start))
     throw CheckpointError("tStart = " + std::to_string(c.input.t_start) + " but the checkpoint in '" + dir +
                           "' is at t = " + std::to_string(info.time));
@@ -84,12 +103,25 @@ void Propagation::write_checkpoint(const std::string& dir) {
   const int n_phi = s.angular.n_phi();
   if (bk_.size() != bk_times_.size())
     throw std::logic_error("write_checkpoint needs every BK record (keep_bk_records was switched off)");
-  std::vector<std::pair<double, std::vector<cplx>>> records;
-  for (std::size_t k = 0; k < bk_.size(); ++k) records.emplace_back(bk_times_[k], bk_[k]);
-  write_bk_file((d / kBkFile).string(), s.config, n_phi, s.analysis, records);
+  const std::string& format = s.config.input.checkpoint_format;
   const WavePacket& psi = backend_->wave_packet_on_host();
-  spo::write_checkpoint((d / kCheckpointFile).string(), s.config, n_phi, time(), psi.scattering, psi.n_rot, psi.n_xy,
-                        psi.n_zr, psi.specular, surface_.state());
+  if (format != "fortran") {
+    std::vector<std::pair<double, std::vector<cplx>>> records;
+    for (std::size_t k = 0; k < bk_.size(); ++k) records.emplace_back(bk_times_[k], bk_[k]);
+    write_bk_file((d / kBkFile).string(), s.config, n_phi, s.analysis, records);
+    spo::write_checkpoint((d / kCheckpointFile).string(), s.config, n_phi, time(), psi.scattering, psi.n_rot,
+                          psi.n_xy, psi.n_zr, psi.specular, surface_.state());
+  }
+  if (format != "native") {
+    FortranBkHeader h;
+    h.n_records = static_cast<int>(bk_.size());
+    h.n_rotations = s.basis.size();
+    h.initial_channel = s.basis.initial;
+    h.initial_energy = s.gas.initial_energy;
+    write_fortran_bk_file((d / kFortranBkOut).string(), s.config, s.analysis, h, bk_);
+    write_fortran_wave_function((d / kFortranWaveOut).string(), psi.scattering, psi.n_rot, psi.n_xy, psi.n_zr,
+                                psi.specular, surface_.state());
+  }
 }
 
 void Propagation::energy_resolved(std::vector<cplx>* sums, std::vector<cplx>* specular_sums) {

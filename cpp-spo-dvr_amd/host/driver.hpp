@@ -33,6 +33,11 @@ std::shared_ptr<RunSetup> make_run_setup(const InputParameters& input, bool devi
 // File names of the native restart files (src/app/run.cpp:67-68).
 inline constexpr const char* kCheckpointFile = "checkpoint.bin";
 inline constexpr const char* kBkFile = "bk_coefficients.bin";
+// The Fortran program's restart files: written as fort.9 / fort.21, read as fort.8 / fort.22 (src/app/run.cpp:69-72).
+inline constexpr const char* kFortranWaveOut = "fort.9";
+inline constexpr const char* kFortranWaveIn = "fort.8";
+inline constexpr const char* kFortranBkOut = "fort.21";
+inline constexpr const char* kFortranBkIn = "fort.22";
 
 class Propagation {
  public:

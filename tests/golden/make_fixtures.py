@@ -12,6 +12,8 @@ case, into tests/golden/<case>/:
   stdout.json         the analysis limits (jana, nviba, nana, nstata, inchna) and the fixed specular line
   gas.npz             (cases with debug_asyp.dat) E(v, j) and the eigenvectors of the rovibrational states
   pes.npz             (cases with pes_reference.txt) the grid shape, angular grid and a subsample of V (hartree)
+  fort21_header.bin   the first 10 records of fort.21 (sizes, times, incidence, the basis labels, channel
+                      energies) with their gfortran record markers, verbatim
 The fixtures are data (inputs and expected outputs); no reference source is copied.
 
 Usage: python tests/golden/make_fixtures.py
@@ -56,6 +58,16 @@ def extract_bk(src: str, dst: str) -> None:
     idx = np.sort(order)
     np.savez_compressed(os.path.join(dst, "bk.npz"), records=bk[:, idx], indices=idx, scale=scale,
                         n_coefficients=bk.shape[1], n_records=bk.shape[0])
+
+
+def extract_fort21_header(src: str, dst: str) -> None:
+    data = open(os.path.join(src, "fortran_fort21.bin"), "rb").read()
+    pos = 0
+    for _ in range(10):
+        n = struct.unpack("<i", data[pos:pos + 4])[0]
+        pos += 8 + n
+    with open(os.path.join(dst, "fort21_header.bin"), "wb") as f:
+        f.write(data[:pos])
 
 
 ROW = re.compile(r"^ +(-?\d+) +(-?\d+) +(\S+) +(\S+) +(\S+) +(\S+) *$")
@@ -147,6 +159,7 @@ def main() -> None:
         shutil.copyfile(os.path.join(src, "input.json"), os.path.join(dst, "input.json"))
         shutil.copyfile(os.path.join(src, "fortran_averages.out"), os.path.join(dst, "averages.out"))
         extract_bk(src, dst)
+        extract_fort21_header(src, dst)
         extract_fort25(src, dst)
         extract_stdout(src, dst)
         extract_gas(src, dst)

@@ -179,3 +179,69 @@ def test_analysis_surface_index_is_checked(split, tmp_path):
     d = _patched(first_dir, tmp_path, "checkpoint.bin", lambda b: struct.pack_into("<i", b, off + 28, 4096))
     e = _error(path, d)
     assert e.code == 4 and "analysis surface index 4096 is outside the specular grid" in str(e)
+
+
+# ---- the Fortran program's layout (fort.9 / fort.21 written, fort.8 / fort.22 or those read back) ----
+
+def _records(data):
+    """The gfortran records of a sequential unformatted file (int32 markers around each)."""
+    pos, out = 0, []
+    while pos < len(data):
+        n, = struct.unpack_from("<i", data, pos)
+        out.append(data[pos + 4:pos + 4 + n])
+        assert struct.unpack_from("<i", data, pos + 4 + n)[0] == n
+        pos += 8 + n
+    return out
+
+
+@pytest.mark.parametrize("case", ["hardwall_short", "cu111_dailight_short", "cu111_dailight_oblique",
+                                  "cu111_dailight_short_sym", "cu111_dailight_short_lowz"])
+def test_fortran_bk_header_is_the_fortran_programs(case, tmp_path):
+    """The ten header records of fort.21 against those the Fortran program wrote for the same input: sizes,
+    labels and framing exactly, the reals to 1e-12 relative."""
+    from tests import fixtures
+    run = oracle.run(cases.golden_input(case), {"CheckpointFormat": "fortran"})
+    run.write_checkpoint(str(tmp_path))
+    run.close()
+    mine = _records((tmp_path / "fort.21").read_bytes())
+    ref = _records(open(fixtures.case_dir(case) + "/fort21_header.bin", "rb").read())
+    assert len(mine) == 10 and [len(r) for r in mine] == [len(r) for r in ref]
+    assert struct.unpack("<4i", mine[0])[1:] == struct.unpack("<4i", ref[0])[1:]  # nstatw, ndftow, nrotw
+    np.testing.assert_allclose(struct.unpack("<6d", mine[1]), struct.unpack("<6d", ref[1]), rtol=1e-12, atol=1e-300)
+    np.testing.assert_allclose(struct.unpack("<3d", mine[2][:24]), struct.unpack("<3d", ref[2][:24]), rtol=1e-12,
+                               atol=1e-14)
+    assert mine[2][24:] == ref[2][24:]  # JIn, MJIn, initial channel
+    for k in range(3, 9):
+        assert mine[k] == ref[k], f"label record {k + 1}"
+    np.testing.assert_allclose(np.frombuffer(mine[9], "<f8"), np.frombuffer(ref[9], "<f8"), rtol=1e-12, atol=1e-14)
+
+
+def test_split_run_through_fortran_files_is_bitwise_the_unsplit_run(split, tmp_path):
+    path, _, whole = split
+    first = oracle.run(path, dict(FIRST, CheckpointFormat="both"))
+    first.steps(first.info.analysis_steps_total)
+    first.write_checkpoint(str(tmp_path))
+    first.close()
+    assert {"fort.9", "fort.21", "checkpoint.bin", "bk_coefficients.bin"} <= set(os.listdir(tmp_path))
+    recs = _records((tmp_path / "fort.21").read_bytes())
+    assert struct.unpack("<4i", recs[0])[0] == 4 and len(recs) == 10 + 4
+    second = oracle.run(path, dict(SECOND, RestartDir=str(tmp_path), RestartFormat="fortran"))
+    second.steps(second.info.analysis_steps_total)
+    sg, pg = second.get_wave()
+    assert np.array_equal(sg, whole["wave"][0]) and np.array_equal(pg, whole["wave"][1])
+    assert np.array_equal(second.s_sums(), whole["sums"])
+    assert np.array_equal(second.bk_records(), whole["bk"])
+    second.close()
+
+
+def test_fortran_restart_prefers_fort8_and_fort22(split, tmp_path):
+    """A Fortran restart reads fort.8 / fort.22 when they are there (the names the Fortran program reads), before
+    fort.9 / fort.21; a wrong fort.8 is therefore the one refused."""
+    path, _, _ = split
+    first = oracle.run(path, dict(FIRST, CheckpointFormat="fortran"))
+    first.steps(first.info.analysis_steps_total)
+    first.write_checkpoint(str(tmp_path))
+    first.close()
+    (tmp_path / "fort.8").write_bytes(struct.pack("<iii", 4, 3, 4))  # nproj = 3
+    e = _error(path, tmp_path, RestartFormat="fortran")
+    assert e.code == 4 and "ProjOp is 2 but 3 in" in str(e) and "fort.8" in str(e)

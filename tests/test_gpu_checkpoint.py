@@ -77,3 +77,24 @@ def test_oracle_checkpoint_continues_on_the_gpu(tmp_path):
     # the BK gate's measure: 1e-10 max(max |c|, 1e-4) (tests/regression_gate.hpp:19-45)
     assert np.abs(b2[n_first:] - bk[n_first:]).max() <= 1e-10 * max(np.abs(bk).max(), 1e-4)
     second.close()
+
+
+def test_split_run_through_fortran_files_on_the_gpu(tmp_path):
+    import spodvr_amd
+    _note("split run through fort.9 / fort.21, SMOKE_DAILIGHT FP64")
+    path = cases.write_case("restart_smoke", cases.SMOKE_DAILIGHT, str(tmp_path))
+    full = spodvr_amd.Run(path, device="hip")
+    full.steps(full.info.analysis_steps_total)
+    wave, sums = full.get_wave(), full.s_sums()
+    full.close()
+    first = spodvr_amd.Run(path, dict(FIRST, CheckpointFormat="fortran"), device="hip")
+    first.steps(first.info.analysis_steps_total)
+    first.write_checkpoint(str(tmp_path / "first"))
+    first.close()
+    second = spodvr_amd.Run(path, dict(SECOND, RestartDir=str(tmp_path / "first"), RestartFormat="fortran"),
+                            device="hip")
+    second.steps(second.info.analysis_steps_total)
+    sg, pg = second.get_wave()
+    assert np.array_equal(sg, wave[0]) and np.array_equal(pg, wave[1])
+    assert np.array_equal(second.s_sums(), sums)
+    second.close()
