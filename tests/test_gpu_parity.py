@@ -152,16 +152,20 @@ def test_two_runs_are_bitwise_identical(precision):
 
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
-@pytest.mark.parametrize("case,settings", [("dz_angular", ["interaction=4", "interaction=16", "xy=8", "xy=16", "xy=32"]),
-                                           ("hardwall_short", ["zr=256", "zr=512", "xy=32", "xy=64"])])
+@pytest.mark.parametrize("case,settings", [("dz_angular", ["interaction=4", "interaction=16", "xy=32", "xy=64"]),
+                                           ("hardwall_short", ["zr=256", "zr=512", "xy=32", "xy=64"]),
+                                           ("dz", ["xy=8", "xy=16", "xy=32"])])
 def test_launch_variants_match_the_default(case, settings, precision):
     """The launch variants (block sizes, tile widths) against the default variant after a full T block: the same
     arithmetic up to the compiler's multiply-add contraction, so within the parity bar (FP64 1e-13, complex64 1e-6
     relative)."""
     import os
     import spodvr_amd
-    path = CASE_PARAMS[case]()
-    out = {}
+    # dz: the 16 x 16 (x, y) grid of the DZ workload, whose T_xy tiles are 8, 16 or 32 izr wide
+    path = cases.write_case("dz", cases.DZ_6D) if case == "dz" else CASE_PARAMS[case]()
+    tol = 1e-13 if precision == "fp64" else 1e-6
+    wave = None
+    reference = None
     for setting in ["default"] + settings:
         print(f"case {case} seed 51 {precision} SPO_HIP_TUNE={setting}", flush=True)
         if setting != "default":
@@ -170,14 +174,17 @@ def test_launch_variants_match_the_default(case, settings, precision):
             run = spodvr_amd.Run(path, {"Precision": precision}, device="hip")
         finally:
             os.environ.pop("SPO_HIP_TUNE", None)
-        run.set_wave(pseudo_random(run.scattering_size, 51), pseudo_random(run.specular_size, 52))
+        if wave is None:
+            wave = (pseudo_random(run.scattering_size, 51), pseudo_random(run.specular_size, 52))
+        run.set_wave(*wave)
         run.kinetic_and_potential(False, False)
-        out[setting] = run.get_wave()
+        result = run.get_wave()
         run.close()
-    tol = 1e-13 if precision == "fp64" else 1e-6
-    for setting in settings:
-        ds = relative_difference(out["default"][0], out[setting][0])
-        dp = relative_difference(out["default"][1], out[setting][1])
+        if reference is None:
+            reference = result
+            continue
+        ds = relative_difference(reference[0], result[0])
+        dp = relative_difference(reference[1], result[1])
         print(f"  {setting}: scattering {ds:.3e} specular {dp:.3e}", flush=True)
         assert ds < tol and dp < tol, setting
 
