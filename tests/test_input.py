@@ -32,3 +32,19 @@ def test_unknown_precision_is_refused():
     params = dict(cases.SMOKE_DAILIGHT, Precision="fp16")
     with pytest.raises(spodvr_amd.SpoError, match="Precision"):
         oracle.run(cases.write_case("invalid_precision", params))
+
+
+# The restart and time keys (src/input/validate.cpp:90-103): each refusal names its key, status 2.
+@pytest.mark.parametrize("overrides,key,message", [
+    ({"tStart": 10.0, "tEnd": 90.0}, "tStart", "This is a new calculation but tStart /= 0."),
+    ({"NewCalc": False, "RestartDir": "somewhere"}, "tStart", "This is a restart but tStart = 0."),
+    ({"NewCalc": False, "tStart": 10.0, "tEnd": 90.0}, "RestartDir", "RestartDir is not given"),
+    ({"tEnd": 81.0}, "tEnd", "tEnd /= tStart + nStep*dtStep."),
+    ({"RestartFormat": "hdf5"}, "RestartFormat", "must be"),
+    ({"CheckpointFormat": "all"}, "CheckpointFormat", "must be"),
+])
+def test_restart_and_time_keys_are_checked(overrides, key, message):
+    params = dict(cases.SMOKE_DAILIGHT, **overrides)
+    with pytest.raises(spodvr_amd.SpoError) as e:
+        oracle.run(cases.write_case("invalid_restart_" + key, params))
+    assert e.value.code == 2 and key in str(e.value) and message in str(e.value)
