@@ -247,6 +247,10 @@ struct spo_hip_backend {
   spo_hip_shape shape{};
   spo_dev::Shape s{};
   hipStream_t stream = nullptr;
+  // The specular grid's step runs on `side` (default priority, like `stream`), overlapping T_Zr and T_xy of the
+  // scattering grid; `stream` waits for spec_ready before anything that touches the specular wave again.
+  hipStream_t side = nullptr;
+  hipEvent_t main_ready = nullptr, spec_ready = nullptr;
   int max_lds = 65536;
   spo_hip_analysis analysis{};
 
@@ -278,21 +282,23 @@ struct spo_hip_backend {
     return e;
   }
   // Runs `launch_fn` (one kernel launch on `stream`), bracketed by events when profiling.
+  // `on`: the stream launch_fn launches on (default `stream`).
   template <typename F>
-  void timed(const char* name, F&& launch_fn) {
+  void timed(const char* name, F&& launch_fn, hipStream_t on = nullptr) {
     if (!profiling) {
       launch_fn();
       HIP_CHECK(hipGetLastError());
       return;
     }
+    const hipStream_t st = on ? on : stream;
     int id = 0;
     while (id < static_cast<int>(kernel_names.size()) && kernel_names[id] != name) ++id;
     if (id == static_cast<int>(kernel_names.size())) kernel_names.emplace_back(name);
     Timed t{id, take_event(), take_event()};
-    HIP_CHECK(hipEventRecord(t.start, stream));
+    HIP_CHECK(hipEventRecord(t.start, st));
     launch_fn();
     HIP_CHECK(hipGetLastError());
-    HIP_CHECK(hipEventRecord(t.stop, stream));
+    HIP_CHECK(hipEventRecord(t.stop, st));
     timed_launches.push_back(t);
   }
 
@@ -302,6 +308,12 @@ struct spo_hip_backend {
       (void)hipEventDestroy(t.stop);
     }
     for (hipEvent_t e : event_pool) (void)hipEventDestroy(e);
+    if (side) {
+      (void)hipStreamSynchronize(side);
+      (void)hipStreamDestroy(side);
+    }
+    if (main_ready) (void)hipEventDestroy(main_ready);
+    if (spec_ready) (void)hipEventDestroy(spec_ready);
     if (stream) {
       (void)hipStreamSynchronize(stream);
       (void)hipStreamDestroy(stream);
@@ -335,26 +347,31 @@ void spo_hip_backend::lines_d(cx<double>* data, spo_dev::Lines L, const DevicePl
   const std::size_t lds = std::size_t(L.lines_per_block) * L.n * sizeof(cx<double>);
   const unsigned grid = blocks_for(L.count, L.lines_per_block);
   timed("specular_lines_fft", [&] {
-    hipLaunchKernelGGL(spo_dev::lines_fft_kernel<double>, dim3(grid), dim3(kThreads), lds, stream, data, L, p.plan,
+    hipLaunchKernelGGL(spo_dev::lines_fft_kernel<double>, dim3(grid), dim3(kThreads), lds, side, data, L, p.plan,
                        sign, mode, factor, f_line, f_elem);
-  });
+  }, side);
 }
 
 // Specular grid: r lines, Z lines with the kinetic factor (FFT scales folded in), r lines back, then the gas phase
 // beyond zasymp and the absorbers (src/propagation/specular_operator.cpp:40-71). FP64.
 void spo_hip_backend::specular_step(bool half, bool kinetic_only) {
+  // On `side`, after everything already on `stream` (the last projection wrote the specular wave).
+  HIP_CHECK(hipEventRecord(main_ready, stream));
+  HIP_CHECK(hipStreamWaitEvent(side, main_ready, 0));
   cx<double>* sp = spec.as<cx<double>>();
   spo_dev::Lines r_lines{s.nr, s.nzsp, s.nzsp, s.nr, 0, 1, 1};
   spo_dev::Lines z_lines{s.nzsp, s.nr, s.nr, 1, 0, s.nr, 1};
   lines_d(sp, r_lines, *plan_r_d, -1, 0, nullptr, 0, 0);
   lines_d(sp, z_lines, *plan_zsp, -1, 1, (half ? spec_half : spec_full).as<cx<double>>(), 1, s.nr);
   lines_d(sp, r_lines, *plan_r_d, +1, 0, nullptr, 0, 0);
-  if (kinetic_only) return;
-  timed("specular_potential", [&] {
-    hipLaunchKernelGGL(spo_dev::specular_potential_kernel<double>, dim3(blocks_for(long(s.nr) * s.nzsp, kThreads)),
-                       dim3(kThreads), 0, stream, sp, s, spec_gas.as<cx<double>>(), z_spec_opt.as<double>(),
-                       r_opt_d.as<double>());
-  });
+  if (!kinetic_only) {
+    timed("specular_potential", [&] {
+      hipLaunchKernelGGL(spo_dev::specular_potential_kernel<double>, dim3(blocks_for(long(s.nr) * s.nzsp, kThreads)),
+                         dim3(kThreads), 0, side, sp, s, spec_gas.as<cx<double>>(), z_spec_opt.as<double>(),
+                         r_opt_d.as<double>());
+    }, side);
+  }
+  HIP_CHECK(hipEventRecord(spec_ready, side));
 }
 
 void spo_hip_backend::analyse(int surface_fixed, int iz_specular, double t, double* coefficients) {
@@ -595,6 +612,8 @@ void State<T>::kinetic_and_potential(bool half, bool kinetic_only) {
                          shape.needs_shift, kinetic_only ? 0 : 1, zo, before.as<cx<T>>(), gas_step.as<cx<T>>());
     });
   }
+  // The specular step (on `side`) is complete before the specular wave is read or written on `stream` again.
+  HIP_CHECK(hipStreamWaitEvent(stream, spec_ready, 0));
   if (kinetic_only || s.n_zr_pot == 0) return;
   // V in the interaction region, then the projection onto the specular grid.
   spo_dev::AngularTables at{legendre.as<double>(),  block_first.as<int>(), block_size.as<int>(),
@@ -773,6 +792,9 @@ std::unique_ptr<spo_hip_backend> create_state(const spo_hip_shape* shape, const 
     throw Status(SPO_HIP_E_NOT_SUPPORTED, "not enough device memory: need " + std::to_string(need >> 20) +
                                               " MiB, free " + std::to_string(free_bytes >> 20) + " MiB");
   HIP_CHECK(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+  HIP_CHECK(hipStreamCreateWithFlags(&b->side, hipStreamNonBlocking));
+  HIP_CHECK(hipEventCreateWithFlags(&b->main_ready, hipEventDisableTiming));
+  HIP_CHECK(hipEventCreateWithFlags(&b->spec_ready, hipEventDisableTiming));
 
   const std::size_t cT = sizeof(cx<T>), c16 = sizeof(cx<double>);
   // Zero slack behind the last site: the interaction kernel's phase-1 loads of a partial k step or of a column past
