@@ -227,21 +227,32 @@ __global__ __launch_bounds__(THREADS) void zr_reg_kernel(cx<T>* psi, const cx<T>
 // T_xy factor, transforms it back and parks it again; thread (c, ix) finishes its y line and stores it to HBM with
 // the shift and the potential factors applied. Two LDS round trips, two barriers. LDS element (c, iy, ix) at
 // ix * XS + iy * (C + 1) + c, XS = NY (C + 1) + 1.
+// SPLIT: each round trip goes through LDS in two halves of the y lines (rows iy < NY / 2, then the rest; four
+// barriers each), so that the tile takes half the LDS and twice as many workgroups fit on a CU. At least three
+// wavefronts per SIMD are asked for (FP64 at 16 x 16 needs 169 VGPRs otherwise, two wavefronts per SIMD).
 template <int N>
 constexpr bool single_radix() {
   return is_radix(N);
 }
 
-template <typename T, int NX, int NY, int C>
-__global__ __launch_bounds__(C * NX) void xy_reg_kernel(cx<T>* psi, Shape s, const cx<T>* factor,
+// LDS bytes of xy_reg_kernel.
+template <typename T>
+constexpr std::size_t xy_reg_lds(int nx, int ny, int c, bool split) {
+  return std::size_t(nx) * ((split ? ny / 2 : ny) * (c + 1) + 1) * sizeof(cx<T>);
+}
+
+template <typename T, int NX, int NY, int C, bool SPLIT = false>
+__global__ __launch_bounds__(C * NX) __attribute__((amdgpu_waves_per_eu(3))) void xy_reg_kernel(cx<T>* psi, Shape s, const cx<T>* factor,
                                                          const cx<T>* shift_f, const cx<T>* shift_b, int needs_shift,
                                                          int apply_potential, const T* z_opt, const cx<T>* before,
                                                          const cx<T>* gas_step) {
   static_assert(single_radix<NX>() && single_radix<NY>() && NX == NY, "xy_reg_kernel: one radix per axis, NX = NY");
+  static_assert(!SPLIT || NY % 2 == 0, "xy_reg_kernel: halves of the y lines");
   extern __shared__ unsigned char smem_raw[];
   cx<T>* tile = reinterpret_cast<cx<T>*>(smem_raw);
   constexpr int YS = C + 1;
-  constexpr int XS = NY * YS + 1;
+  constexpr int NH = SPLIT ? NY / 2 : NY;  // y rows held in LDS at once
+  constexpr int XS = NH * YS + 1;
   const int tiles_per_rot = (s.n_zr + C - 1) / C;
   const int irot = blockIdx.x / tiles_per_rot;
   const int izr0 = (blockIdx.x % tiles_per_rot) * C;
@@ -261,23 +272,50 @@ __global__ __launch_bounds__(C * NX) void xy_reg_kernel(cx<T>* psi, Shape s, con
     if (needs_shift) v[iy] = v[iy] * shift_f[ixy];
   }
   rdft<NY, T>(v, -1);
-#pragma unroll
-  for (int iy = 0; iy < NY; ++iy) tile[a * XS + iy * YS + c] = v[iy];
-  __syncthreads();
-
   cx<T> w[NX];
+  if constexpr (!SPLIT) {
 #pragma unroll
-  for (int ix = 0; ix < NX; ++ix) w[ix] = tile[ix * XS + a * YS + c];
+    for (int iy = 0; iy < NY; ++iy) tile[a * XS + iy * YS + c] = v[iy];
+    __syncthreads();
+#pragma unroll
+    for (int ix = 0; ix < NX; ++ix) w[ix] = tile[ix * XS + a * YS + c];
+  } else {
+    // Half h holds the rows iy = h NH .. h NH + NH - 1; the thread of row a = iy reads in its half's round.
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      if (h) __syncthreads();  // the previous round's reads are done
+#pragma unroll
+      for (int iy = 0; iy < NH; ++iy) tile[a * XS + iy * YS + c] = v[h * NH + iy];
+      __syncthreads();
+      if (a / NH == h) {
+#pragma unroll
+        for (int ix = 0; ix < NX; ++ix) w[ix] = tile[ix * XS + (a - h * NH) * YS + c];
+      }
+    }
+  }
   rdft<NX, T>(w, -1);
 #pragma unroll
   for (int ix = 0; ix < NX; ++ix) w[ix] = factor[ix * NY + a] * w[ix];
   rdft<NX, T>(w, +1);
+  if constexpr (!SPLIT) {
 #pragma unroll
-  for (int ix = 0; ix < NX; ++ix) tile[ix * XS + a * YS + c] = w[ix];
-  __syncthreads();
-
+    for (int ix = 0; ix < NX; ++ix) tile[ix * XS + a * YS + c] = w[ix];
+    __syncthreads();
 #pragma unroll
-  for (int iy = 0; iy < NY; ++iy) v[iy] = tile[a * XS + iy * YS + c];
+    for (int iy = 0; iy < NY; ++iy) v[iy] = tile[a * XS + iy * YS + c];
+  } else {
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      __syncthreads();  // every thread is done with the LDS of the previous round
+      if (a / NH == h) {
+#pragma unroll
+        for (int ix = 0; ix < NX; ++ix) tile[ix * XS + (a - h * NH) * YS + c] = w[ix];
+      }
+      __syncthreads();
+#pragma unroll
+      for (int iy = 0; iy < NH; ++iy) v[h * NH + iy] = tile[a * XS + iy * YS + c];
+    }
+  }
   rdft<NY, T>(v, +1);
   if (!live) return;
   const int izr = izr0 + c;
