@@ -701,8 +701,10 @@ void State<T>::z_plane(int iz) {
 
 template <typename T>
 void State<T>::block_densities() {
-  const int chunk = std::max(1, (s.n_xy + 15) / 16);
-  const int parts = (s.n_xy + chunk - 1) / chunk;
+  // One part per site (n_zr x n_xy partial sums over irot), then the parts added in site order: enough workgroups
+  // to cover the device, where a few long per-thread loops over sites left it latency-bound.
+  const int chunk = 1;
+  const int parts = s.n_xy;
   timed("densities", [&] {
     hipLaunchKernelGGL(spo_dev::densities_partial_kernel<T>, dim3(blocks_for(s.n_zr, kThreads), parts), dim3(kThreads),
                        0, stream, psi.as<cx<T>>(), s, chunk, partial.as<double>());
@@ -813,7 +815,7 @@ std::unique_ptr<spo_hip_backend> create_state(const spo_hip_shape* shape, const 
   HIP_CHECK(hipMemsetAsync(b->psi.template as<void>(), 0, (psi_values + psi_slack) * cT, b->stream));
   b->spec = upload(initial_specular, std::size_t(s.nr) * s.nzsp * c16);
   b->densities = Buffer(std::size_t(s.n_zr) * sizeof(double));
-  b->partial = Buffer(std::size_t(s.n_zr) * 16 * sizeof(double));
+  b->partial = Buffer(std::size_t(s.n_zr) * s.n_xy * sizeof(double));
   b->plane = Buffer(std::size_t(s.n_rot) * s.n_xy * s.nr * c16);
   b->staging_values = std::max<std::size_t>(std::size_t(s.n_rot) * s.n_xy, std::size_t(4) << 20);
   b->staging = Buffer(b->staging_values * c16);
