@@ -156,7 +156,7 @@ void set_lds(const void* kernel, std::size_t bytes) {
 // T_Zr planes with register-resident passes (NZ, NR, A1, A2, B1, B2, threads): NR = A1 A2, NZ = B1 B2. Entries of
 // one shape that differ only in the thread count compute identically; the launch tuner picks among them.
 #define SPO_ZR_REG_SHAPES(X)                                                                                 \
-  X(96, 48, 3, 16, 8, 12, 192) X(96, 48, 3, 16, 8, 12, 384) X(128, 32, 4, 8, 16, 8, 256)                      \
+  X(96, 48, 3, 16, 8, 12, 192) X(96, 48, 3, 16, 8, 12, 384) X(96, 48, 3, 16, 8, 12, 512) X(128, 32, 4, 8, 16, 8, 256)                      \
   X(128, 32, 4, 8, 16, 8, 512) X(64, 16, 4, 4, 8, 8, 128) X(80, 16, 4, 4, 8, 10, 128)                         \
   X(176, 56, 7, 8, 11, 16, 256) X(176, 56, 7, 8, 11, 16, 512)
 // Interaction with the theta contractions on MFMA (phi length, wavefronts, phi radices R1 x R2 of the fused phi
