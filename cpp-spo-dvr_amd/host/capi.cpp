@@ -9,6 +9,11 @@
 #include "checkpoint.hpp"
 #include "driver.hpp"
 #include "json.hpp"
+#include "potential.hpp"
+
+struct spo_crp {
+  std::shared_ptr<const spo::crp::QdSurface> qd;
+};
 
 struct spo_run {
   std::shared_ptr<spo::RunSetup> setup;
@@ -35,6 +40,8 @@ int guarded(F&& f) {
     return fail(SPO_E_NOT_SUPPORTED, e.what());
   } catch (const spo::CheckpointError& e) {
     return fail(SPO_E_CHECKPOINT, e.what());
+  } catch (const spo::crp::DataError& e) {
+    return fail(SPO_E_INPUT, e.what());
   } catch (const std::exception& e) {
     return fail(SPO_E_OTHER, e.what());
   }
@@ -335,6 +342,136 @@ int spo_run_kernel_times(spo_run* run, char* names, long* launches, double* tota
 int spo_pes_value(const char* system_id, double x, double y, double z, double r, double theta, double phi,
                   double* out) {
   return guarded([&] { *out = spo::make_pes(system_id)->value(x, y, z, r, theta, phi); });
+}
+
+}  // extern "C"
+
+namespace {
+
+spo::crp::Units crp_units(int length, int energy) {
+  if (length != 0 && length != 1) throw spo::InputError("length_unit", "0 (bohr) or 1 (angstrom)");
+  if (energy < 2 || energy > 6) throw spo::InputError("energy_unit", "2 (hartree) .. 6 (J)");
+  return {static_cast<spo::crp::LengthUnit>(length), static_cast<spo::crp::EnergyUnit>(energy)};
+}
+
+// Runs f(k) for k < n in parallel; the first exception is rethrown after the loop.
+template <typename F>
+void for_points(long n, F&& f) {
+  std::exception_ptr error;
+#pragma omp parallel for schedule(dynamic, 64)
+  for (long k = 0; k < n; ++k) {
+    try {
+      f(k);
+    } catch (...) {
+#pragma omp critical
+      if (!error) error = std::current_exception();
+    }
+  }
+  if (error) std::rethrow_exception(error);
+}
+
+}  // namespace
+
+extern "C" {
+
+int spo_crp_open(const char* directory, spo_crp** out) {
+  return guarded([&] { *out = new spo_crp{spo::load_crp_surface(directory)}; });
+}
+
+int spo_crp_close(spo_crp* crp) {
+  delete crp;
+  return SPO_OK;
+}
+
+int spo_crp_molecule(const spo_crp* crp, long n, const double* atoms, int length_unit, int energy_unit, double* out) {
+  return guarded([&] {
+    using namespace spo::crp;
+    const Units u = crp_units(length_unit, energy_unit);
+    const MoleculeSurface& s = crp->qd->surface();
+    for_points(n, [&](long k) {
+      const double* p = atoms + 6 * k;
+      const Vec3 a{to_length(p[0], u.length, LengthUnit::bohr), to_length(p[1], u.length, LengthUnit::bohr),
+                   to_length(p[2], u.length, LengthUnit::bohr)};
+      const Vec3 b{to_length(p[3], u.length, LengthUnit::bohr), to_length(p[4], u.length, LengthUnit::bohr),
+                   to_length(p[5], u.length, LengthUnit::bohr)};
+      const MoleculeValue v = s.evaluate(a, b).value_or(MoleculeValue{0.00, {0.00, 0.00, 0.00}, {0.00, 0.00, 0.00}});
+      double* o = out + 7 * k;
+      o[0] = to_energy(v.v, EnergyUnit::hartree, u.energy);
+      const double g[6] = {v.gradient_a.x, v.gradient_a.y, v.gradient_a.z, v.gradient_b.x, v.gradient_b.y,
+                           v.gradient_b.z};
+      for (int i = 0; i < 6; ++i) o[1 + i] = to_force(g[i], kInternalUnits, u);
+    });
+  });
+}
+
+int spo_crp_atom(const spo_crp* crp, long n, const double* positions, int length_unit, int energy_unit,
+                 double* out) {
+  return guarded([&] {
+    using namespace spo::crp;
+    const Units u = crp_units(length_unit, energy_unit);
+    const AtomSurface& s = crp->qd->surface().atoms();
+    for_points(n, [&](long k) {
+      const double* p = positions + 3 * k;
+      const AtomValue v = s.evaluate({to_length(p[0], u.length, LengthUnit::bohr),
+                                      to_length(p[1], u.length, LengthUnit::bohr),
+                                      to_length(p[2], u.length, LengthUnit::bohr)});
+      double* o = out + 4 * k;
+      o[0] = to_energy(v.v, EnergyUnit::hartree, u.energy);
+      o[1] = to_force(v.gradient.x, kInternalUnits, u);
+      o[2] = to_force(v.gradient.y, kInternalUnits, u);
+      o[3] = to_force(v.gradient.z, kInternalUnits, u);
+    });
+  });
+}
+
+int spo_crp_qd(const spo_crp* crp, long n, const double* points, double* out) {
+  return guarded([&] {
+    for_points(n, [&](long k) {
+      const double* p = points + 6 * k;
+      out[k] = crp->qd->potential_ev(p[0], p[1], p[2], p[3], p[4], p[5]);
+    });
+  });
+}
+
+int spo_crp_gas_phase(const spo_crp* crp, long n, const double* zr, double* out) {
+  return guarded([&] {
+    const spo::crp::MoleculeSurface& s = crp->qd->surface();
+    for_points(n, [&](long k) {
+      const auto sw = s.switch_function(zr[2 * k]);
+      const auto gp = s.gas_phase(zr[2 * k], zr[2 * k + 1]);
+      double* o = out + 5 * k;
+      o[0] = sw.f;
+      o[1] = sw.df_dz;
+      o[2] = gp.v;
+      o[3] = gp.d_z;
+      o[4] = gp.d_r;
+    });
+  });
+}
+
+int spo_crp_geometry(long n, const double* atoms, double* out) {
+  return guarded([&] {
+    for (long k = 0; k < n; ++k) {
+      const double* p = atoms + 6 * k;
+      const auto g = spo::crp::molecule_geometry({p[0], p[1], p[2]}, {p[3], p[4], p[5]});
+      double* o = out + 6 * k;
+      o[0] = g.x;
+      o[1] = g.y;
+      o[2] = g.z;
+      o[3] = g.r;
+      o[4] = g.theta;
+      o[5] = g.phi;
+    }
+  });
+}
+
+int spo_crp_parameters(const spo_crp* crp, double* out) {
+  return guarded([&] {
+    const spo::crp::MoleculeSurface& s = crp->qd->surface();
+    const double v[9] = {s.lattice_constant(), s.z_asymptotic(), crp->qd->skewing_angle(), s.zso_min(), s.zso_max(),
+                         s.r_min(), s.r_max(), s.z_min(), s.symmetry() == spo::crp::Symmetry::c3v ? 3.0 : 6.0};
+    for (int i = 0; i < 9; ++i) out[i] = v[i];
+  });
 }
 
 }  // extern "C"

@@ -1,6 +1,7 @@
 #include "config.hpp"
 
 #include "grids.hpp"
+#include "potential.hpp"
 
 #include <array>
 #include <cmath>
@@ -24,10 +25,16 @@ double machine_epsilon() {
   return eps;
 }
 
-SystemParameters system_parameters_for(const std::string& id) {
+SystemParameters system_parameters_for(const std::string& id, const std::string& crp_data_set) {
   // src/potential/hard_wall_potential.cpp:15 and src/potential/dai_light_cu111_potential.cpp:38-39, :71-72.
   if (id == "HardWallPES") return {6.239660, 5.239660, 7.0, kPi / 2.0, "C2v"};
   if (id == "H2Cu111DaiLight") return {4.84, 4.84, 15.0, kPi / 3.0, "C6v"};
+  if (id == "CRPlibrary") {  // the data set's lattice constant, Zasy and point group (src/potential/crp_potential.cpp:85-87)
+    const auto crp = load_crp_surface(crp_data_set);
+    const crp::MoleculeSurface& s = crp->surface();
+    return {s.lattice_constant(), s.lattice_constant(), s.z_asymptotic(), crp->skewing_angle(),
+            s.symmetry() == crp::Symmetry::c3v ? "C3v" : "C6v"};
+  }
   throw NotSupportedError("SystemID " + id + " is not supported by this backend's set-up");
 }
 
@@ -121,6 +128,7 @@ InputParameters read_input(const json::Value& d) {
   InputParameters p;
   get(d, "Title", p.title);
   get(d, "SystemID", p.system_id);
+  get(d, "CRPlibDataSet", p.crp_data_set);
   get(d, "XDoF", p.x_dof);
   get(d, "YDoF", p.y_dof);
   get(d, "Mass1", p.mass1);
@@ -243,7 +251,7 @@ Config build_config(const InputParameters& input) {
   Config c{input, {}};
   const InputParameters& p = c.input;
   DerivedParameters& d = c.derived;
-  d.system = system_parameters_for(p.system_id);
+  d.system = system_parameters_for(p.system_id, p.crp_data_set);
   d.z_asymptotic_unswitched = d.system.z_asymptotic;
   d.z_asymptotic = p.use_switching ? p.z_max_switch : d.system.z_asymptotic;
 

@@ -37,6 +37,7 @@ struct LowAbsorber {
 struct InputParameters {
   std::string title;
   std::string system_id;
+  std::string crp_data_set;  // CRPlibDataSet: the CRP data set directory of SystemID CRPlibrary
   bool x_dof = true, y_dof = true;
   double mass1 = 1.0078, mass2 = 1.0078;
   bool homonuclear = true;
@@ -116,8 +117,8 @@ struct Config {
   DerivedParameters derived;
 };
 
-// The system parameters of a potential by SystemID (HardWallPES, H2Cu111DaiLight).
-SystemParameters system_parameters_for(const std::string& system_id);
+// The system parameters of a potential by SystemID (HardWallPES, H2Cu111DaiLight; CRPlibrary from its data set).
+SystemParameters system_parameters_for(const std::string& system_id, const std::string& crp_data_set = "");
 
 // Reads the keys from a parsed input document; unknown keys are ignored, keys of other subsystems are not read.
 InputParameters read_input(const json::Value& document);

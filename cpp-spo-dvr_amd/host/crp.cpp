@@ -1,0 +1,1247 @@
+// The CRP potential energy surface of a hexagonal data set. The order of operations of every expression follows the
+// Fortran CRP library as the reference's restatement keeps it (file:line of src/crplib/ at each part), so that the
+// results agree with the Fortran's bit for bit; integer powers are written as products.
+#include "crp.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdlib>
+#include <filesystem>
+#include <fstream>
+#include <sstream>
+
+namespace spo::crp {
+
+namespace {
+
+// pi as CRP_Constants_PI and sqrt(3) as the Fortran literal (src/crplib/constants.hpp:8-14).
+constexpr double kPi = 3.1415926535897932384626433832795028841971;
+constexpr double kSq3 = 1.732050807568877293500;
+constexpr double kUsq3 = 1.0 / kSq3;
+// cos and sin of pi/3 as gfortran folds them at compile time (bits 0x3FE0000000000001 and 0x3FEBB67AE8584CAA;
+// src/crplib/constants.hpp:16-22): the skewing transform of the QD wrapper and the C3v cluster use these values.
+constexpr double kCosPi3 = 0x1.0000000000001p-1;
+constexpr double kSinPi3 = 0x1.bb67ae8584caap-1;
+
+// ---------------------------------------------------------------------------------------------------------------
+// Units (src/crplib/units.cpp:13-79)
+
+constexpr double kBohrToAngstrom = 0.529177249;
+constexpr double kHartreeToEv = 27.2113961;
+constexpr double kHartreeToKcal = 627.50956;
+constexpr double kHartreeToKj = 627.50956 * 4.184;
+constexpr double kHartreeToJoule = 4.35974434e-18;
+
+double hartree_to(EnergyUnit u) {
+  switch (u) {
+    case EnergyUnit::ev: return kHartreeToEv;
+    case EnergyUnit::kcal_per_mol: return kHartreeToKcal;
+    case EnergyUnit::kj_per_mol: return kHartreeToKj;
+    case EnergyUnit::joule: return kHartreeToJoule;
+    case EnergyUnit::hartree: break;
+  }
+  return 1.0;
+}
+
+double from_hartree_reciprocal(EnergyUnit u) {
+  switch (u) {
+    case EnergyUnit::ev: return 1.0 / kHartreeToEv;
+    case EnergyUnit::kcal_per_mol: return 1.0 / kHartreeToKcal;
+    case EnergyUnit::kj_per_mol: return 1.0 / kHartreeToKj;
+    case EnergyUnit::joule: return 1.0 / kHartreeToJoule;
+    case EnergyUnit::hartree: break;
+  }
+  return 1.0;
+}
+
+}  // namespace
+
+double to_length(double x, LengthUnit from, LengthUnit to) {
+  if (from == to) return x;
+  double factor = from == LengthUnit::bohr ? 1.0 : 1.0 / kBohrToAngstrom;
+  if (to == LengthUnit::angstrom) factor = factor * kBohrToAngstrom;
+  return x * factor;
+}
+
+double to_energy(double x, EnergyUnit from, EnergyUnit to) {
+  if (from == to) return x;
+  double factor = from_hartree_reciprocal(from);
+  if (to != EnergyUnit::hartree) factor = factor * hartree_to(to);
+  return x * factor;
+}
+
+double to_force(double x, Units from, Units to) {
+  return x * to_energy(1.0, from.energy, to.energy) / to_length(1.0, from.length, to.length);
+}
+
+// ---------------------------------------------------------------------------------------------------------------
+// Data set (src/crplib/dataset.cpp:70-233; list-directed input, src/fortran_text/list_directed_reader.cpp)
+
+namespace {
+
+// Fortran list-directed READ over the records of a text file: every READ starts at the next record, takes values
+// separated by blanks or commas from as many records as it needs (r*c repeats, a slash ends it) and skips the rest
+// of its last record.
+class ListReader {
+ public:
+  explicit ListReader(const std::filesystem::path& file) : file_(file.string()) {
+    std::ifstream in(file);
+    if (!in) throw DataError(file_ + ": cannot open the file");
+    std::string line;
+    while (std::getline(in, line)) records_.push_back(line);
+  }
+
+  // n values, or nullopt when the file ends first.
+  std::optional<std::vector<std::string>> try_read(int n) {
+    std::vector<std::string> values;
+    if (next_ >= records_.size()) return std::nullopt;
+    bool ended = false;
+    while (static_cast<int>(values.size()) < n && !ended) {
+      if (next_ >= records_.size()) return std::nullopt;
+      const std::string& rec = records_[next_++];
+      std::size_t i = 0;
+      while (i < rec.size() && static_cast<int>(values.size()) < n) {
+        while (i < rec.size() && (rec[i] == ' ' || rec[i] == '\t' || rec[i] == ',' || rec[i] == '\r')) ++i;
+        if (i >= rec.size()) break;
+        if (rec[i] == '/') {
+          ended = true;
+          break;
+        }
+        std::string token;
+        if (rec[i] == '\'' || rec[i] == '"') {
+          const char q = rec[i++];
+          while (i < rec.size()) {
+            if (rec[i] == q) {
+              if (i + 1 < rec.size() && rec[i + 1] == q) {
+                token += q;
+                i += 2;
+                continue;
+              }
+              ++i;
+              break;
+            }
+            token += rec[i++];
+          }
+        } else {
+          while (i < rec.size() && rec[i] != ' ' && rec[i] != '\t' && rec[i] != ',' && rec[i] != '/' &&
+                 rec[i] != '\r')
+            token += rec[i++];
+        }
+        const std::size_t star = token.find('*');
+        if (star != std::string::npos && star > 0 &&
+            token.find_first_not_of("0123456789") == star) {  // r*c: r copies of c
+          const int repeat = std::atoi(token.substr(0, star).c_str());
+          for (int k = 0; k < repeat && static_cast<int>(values.size()) < n; ++k)
+            values.push_back(token.substr(star + 1));
+        } else {
+          values.push_back(token);
+        }
+      }
+    }
+    return values;
+  }
+
+  std::vector<std::string> read(int n) {
+    if (n == 0) {  // an empty READ still skips a record
+      if (next_ < records_.size()) ++next_;
+      return {};
+    }
+    auto v = try_read(n);
+    if (!v) throw DataError(file_ + ": unexpected end of file");
+    if (static_cast<int>(v->size()) < n) throw DataError(file_ + ": a slash ended a READ before all values");
+    return *v;
+  }
+
+  double real(const std::string& token) const {
+    std::string t = token;
+    for (char& c : t)
+      if (c == 'd' || c == 'D') c = 'e';
+    char* end = nullptr;
+    const double x = std::strtod(t.c_str(), &end);
+    if (t.empty() || *end != '\0') throw DataError(file_ + ": '" + token + "' is not a real number");
+    return x;
+  }
+  int integer(const std::string& token) const {
+    char* end = nullptr;
+    const long x = std::strtol(token.c_str(), &end, 10);
+    if (token.empty() || *end != '\0') throw DataError(file_ + ": '" + token + "' is not an integer");
+    return static_cast<int>(x);
+  }
+  bool logical(const std::string& token) const {
+    std::size_t i = token.empty() || token[0] != '.' ? 0 : 1;
+    if (i < token.size() && (token[i] == 'T' || token[i] == 't')) return true;
+    if (i < token.size() && (token[i] == 'F' || token[i] == 'f')) return false;
+    throw DataError(file_ + ": '" + token + "' is not a logical value");
+  }
+  const std::string& name() const { return file_; }
+
+ private:
+  std::string file_;
+  std::vector<std::string> records_;
+  std::size_t next_ = 0;
+};
+
+Table read_table(ListReader& in, int count) {
+  if (count < 0) throw DataError(in.name() + ": negative number of points");
+  Table t;
+  for (int i = 0; i < count; ++i) {
+    const auto s = in.read(2);
+    t.x.push_back(in.real(s[0]));
+    t.f.push_back(in.real(s[1]));
+  }
+  return t;
+}
+
+void check_grid(const std::vector<double>& x, const std::string& what) {
+  if (x.size() < 4) throw DataError(what + ": a cubic spline needs at least 4 points");
+  for (std::size_t i = 1; i < x.size(); ++i)
+    if (!(x[i] > x[i - 1])) throw DataError(what + ": the grid is not strictly increasing");
+}
+
+int atom_cuts_needed(Symmetry s) { return s == Symmetry::c3v ? 10 : (s == Symmetry::c6v ? 7 : 6); }
+
+}  // namespace
+
+int molecule_cuts_used(Symmetry s) {
+  switch (s) {
+    case Symmetry::c3v: return 29;
+    case Symmetry::c4v: return 16;
+    case Symmetry::c6v: return 18;
+  }
+  return 0;
+}
+
+DataSet read_data_set(const std::string& directory) {
+  namespace fs = std::filesystem;
+  const fs::path dir(directory);
+  DataSet d;
+  const fs::path crpdata = dir / "CRPData";
+  if (!fs::exists(crpdata)) throw DataError("CRP data set " + directory + " does not exist (no file CRPData)");
+  {
+    ListReader in(crpdata);
+    const std::string sym = in.read(1)[0];
+    if (sym == "C3v") {
+      d.symmetry = Symmetry::c3v;
+    } else if (sym == "C6v") {
+      d.symmetry = Symmetry::c6v;
+    } else if (sym == "C4v") {
+      throw DataError(crpdata.string() + ": C4v data sets are not supported by this library (C3v, C6v)");
+    } else {
+      throw DataError(crpdata.string() + ": unknown symmetry '" + sym + "'");
+    }
+    const auto u = in.read(2);
+    const int lu = in.integer(u[0]), eu = in.integer(u[1]);
+    if (lu != 0 && lu != 1) throw DataError(crpdata.string() + ": unknown length unit code " + u[0]);
+    if (eu < 2 || eu > 6) throw DataError(crpdata.string() + ": unknown energy unit code " + u[1]);
+    d.units = Units{static_cast<LengthUnit>(lu), static_cast<EnergyUnit>(eu)};
+    d.lattice_constant = in.real(in.read(1)[0]);
+    const int layers = in.integer(in.read(1)[0]);
+    if (layers < 1) throw DataError(crpdata.string() + ": the number of layers must be at least 1");
+    for (const std::string& s : in.read(layers - 1)) d.interlayer_spacings.push_back(in.real(s));
+    if (auto disp = in.try_read(2); disp && disp->size() == 2) {
+      try {
+        d.displacement_1 = in.real((*disp)[0]);
+        d.displacement_2 = in.real((*disp)[1]);
+      } catch (const DataError&) {  // READ(..., ERR=100): unreadable displacements are 0
+        d.displacement_1 = d.displacement_2 = 0.0;
+      }
+    }
+  }
+  {
+    ListReader list(dir / "CRPData.3D");
+    const int n = list.integer(list.read(1)[0]);
+    for (int i = 0; i < n; ++i) {
+      const auto s = list.read(3);
+      AtomCut cut;
+      cut.x = list.real(s[0]);
+      cut.y = list.real(s[1]);
+      cut.file = s[2];
+      d.atom_cuts.push_back(cut);
+    }
+    ListReader corr(dir / "Atom" / "intrep.dat");
+    d.corrugation_slope_first = corr.real(corr.read(1)[0]);
+    d.corrugation_slope_last = corr.real(corr.read(1)[0]);
+    d.corrugation = read_table(corr, corr.integer(corr.read(1)[0]));
+    for (AtomCut& cut : d.atom_cuts) {
+      const fs::path file = dir / "Atom" / cut.file;
+      if (!fs::exists(file)) throw DataError(file.string() + ": atom-surface cut file not found");
+      ListReader in(file);
+      const int count = in.integer(in.read(1)[0]);
+      cut.slope_first = in.real(in.read(1)[0]);
+      cut.slope_last = in.real(in.read(1)[0]);
+      cut.v = read_table(in, count);
+    }
+  }
+  {
+    ListReader list(dir / "CRPData.6D");
+    const auto header = list.read(2);
+    const int n = list.integer(header[0]);
+    d.cuts_interpolation = list.logical(header[1]);
+    for (int i = 0; i < n; ++i) {
+      const auto s = list.read(5);
+      MoleculeCut cut;
+      cut.x = list.real(s[0]);
+      cut.y = list.real(s[1]);
+      cut.theta = list.real(s[2]);
+      cut.phi = list.real(s[3]);
+      cut.file = s[4];
+      d.molecule_cuts.push_back(cut);
+    }
+    ListReader ex(dir / "Molecule" / "extrapol.dat");
+    d.zso_min = ex.real(ex.read(1)[0]);
+    d.zso_max = ex.real(ex.read(1)[0]);
+    d.z_dependence = read_table(ex, ex.integer(ex.read(1)[0]));
+    ListReader h2(dir / "Molecule" / "VH2.dat");
+    d.h2 = read_table(h2, h2.integer(h2.read(1)[0]));
+    const int used = molecule_cuts_used(d.symmetry);
+    for (std::size_t k = 0; k < d.molecule_cuts.size(); ++k) {
+      MoleculeCut& cut = d.molecule_cuts[k];
+      const fs::path file = dir / "Molecule" / cut.file;
+      if (!fs::exists(file)) {  // the Fortran skips a missing file; harmless only for an unused extra cut
+        if (static_cast<int>(k) < used) throw DataError(file.string() + ": molecule-surface cut file not found");
+        cut.present = false;
+        continue;
+      }
+      ListReader in(file);
+      const int nr = in.integer(in.read(1)[0]);
+      const int nz = in.integer(in.read(1)[0]);
+      if (nr < 0 || nz < 0) throw DataError(file.string() + ": negative grid size");
+      cut.r.resize(nr);
+      cut.z.resize(nz);
+      cut.v.resize(static_cast<std::size_t>(nr) * nz);
+      for (int j = 0; j < nz; ++j) {  // rows "Z r V", r fastest; the last row read sets the grid values
+        for (int i = 0; i < nr; ++i) {
+          const auto s = in.read(3);
+          cut.z[j] = in.real(s[0]);
+          cut.r[i] = in.real(s[1]);
+          cut.v[i + static_cast<std::size_t>(nr) * j] = in.real(s[2]);
+        }
+      }
+    }
+  }
+  // Checks (src/crplib/dataset.cpp:236-283)
+  if (!(d.lattice_constant > 0.0)) throw DataError("CRPData: the lattice constant must be positive");
+  if (d.symmetry == Symmetry::c3v && d.interlayer_spacings.size() + 1 < 4)
+    throw DataError("CRPData: C3v needs at least 4 layers (the atom cluster has 4)");
+  if (d.symmetry == Symmetry::c3v && d.displacement_1 != 0.0)
+    throw DataError("CRPData: the C3v atom potential does not support displacements");
+  check_grid(d.corrugation.x, "Atom/intrep.dat");
+  if (static_cast<int>(d.atom_cuts.size()) != atom_cuts_needed(d.symmetry))
+    throw DataError("CRPData.3D: needs exactly " + std::to_string(atom_cuts_needed(d.symmetry)) +
+                    " atom-surface cuts, the data set has " + std::to_string(d.atom_cuts.size()));
+  for (const AtomCut& cut : d.atom_cuts) check_grid(cut.v.x, "Atom/" + cut.file);
+  if (d.zso_min > d.zso_max) throw DataError("Molecule/extrapol.dat: ZsoMin is larger than ZsoMax");
+  check_grid(d.z_dependence.x, "Molecule/extrapol.dat");
+  check_grid(d.h2.x, "Molecule/VH2.dat");
+  if (static_cast<int>(d.molecule_cuts.size()) < molecule_cuts_used(d.symmetry))
+    throw DataError("CRPData.6D: needs at least " + std::to_string(molecule_cuts_used(d.symmetry)) +
+                    " molecule-surface cuts, the data set has " + std::to_string(d.molecule_cuts.size()));
+  for (const MoleculeCut& cut : d.molecule_cuts) {
+    if (!cut.present) continue;
+    check_grid(cut.r, "Molecule/" + cut.file + " (r)");
+    check_grid(cut.z, "Molecule/" + cut.file + " (Z)");
+    if (cut.r.size() > 999 || cut.z.size() > 999)
+      throw DataError("Molecule/" + cut.file + ": at most 999 points per direction");
+  }
+  return d;
+}
+
+// ---------------------------------------------------------------------------------------------------------------
+// Splines (src/crplib/spline.cpp)
+
+namespace {
+
+// Tridiagonal solve: forward elimination, back substitution (TRIDIA, src/crplib/spline.cpp:13-23). sub[i - 1] is
+// the sub-diagonal of row i, sup[i] the super-diagonal of row i; diag and rhs are overwritten.
+void tridiagonal(std::vector<double>& diag, const std::vector<double>& sub, const std::vector<double>& sup,
+                 std::vector<double>& rhs, double* out) {
+  const std::size_t n = diag.size();
+  for (std::size_t i = 1; i < n; ++i) {
+    const double rap = sub[i - 1] / diag[i - 1];
+    diag[i] = diag[i] - rap * sup[i - 1];
+    rhs[i] = rhs[i] - rap * rhs[i - 1];
+  }
+  out[n - 1] = rhs[n - 1] / diag[n - 1];
+  for (std::size_t i = n - 1; i-- > 0;) out[i] = (rhs[i] - sup[i] * out[i + 1]) / diag[i];
+}
+
+// F'' at the knots (DSPLIN, src/crplib/spline.cpp:52-146): the interior unknowns from the tridiagonal system of
+// the continuity conditions, the end conditions folded into its first and last rows, then the two end values.
+std::vector<double> second_derivatives(const double* x, const double* y, std::size_t n, Spline1D::Condition first,
+                                       Spline1D::Condition last) {
+  using End = Spline1D::End;
+  if (n < 4) throw DataError("a cubic spline needs at least 4 points");
+  // (right-hand side, diagonal, off-diagonal) terms of each end
+  auto end_terms = [&](Spline1D::Condition c, double h_end, double h_next, double secant, double sign) {
+    std::array<double, 3> t{0.0, 0.0, 0.0};
+    switch (c.kind) {
+      case End::not_a_knot:
+        t[1] = h_end * (1.0 + h_end / h_next) / 6.0;
+        t[2] = -(h_end * h_end) / (6.0 * h_next);
+        break;
+      case End::second_derivative: t[0] = -c.value * h_end / 6.0; break;
+      case End::first_derivative:
+        t[0] = sign * 0.5 * (c.value - secant);
+        t[1] = -h_end / 12.0;
+        break;
+      case End::natural: break;
+    }
+    return t;
+  };
+  const auto s = end_terms(first, x[1] - x[0], x[2] - x[1], (y[1] - y[0]) / (x[1] - x[0]), 1.0);
+  const auto e = end_terms(last, x[n - 1] - x[n - 2], x[n - 2] - x[n - 3],
+                           (y[n - 1] - y[n - 2]) / (x[n - 1] - x[n - 2]), -1.0);
+  const std::size_t m = n - 2;
+  std::vector<double> diag(m), sub(m), sup(m), rhs(m);
+  double h_right = x[1] - x[0];
+  double slope_right = (y[1] - y[0]) / h_right;
+  for (std::size_t i = 2; i < n; ++i) {
+    const double h_left = h_right;
+    h_right = x[i] - x[i - 1];
+    if (h_left * h_right <= 0.0) throw DataError("spline abscissae are not monotonic");
+    diag[i - 2] = (h_left + h_right) / 3.0;
+    sup[i - 2] = h_right / 6.0;
+    sub[i - 2] = sup[i - 2];
+    const double slope_left = slope_right;
+    slope_right = (y[i] - y[i - 1]) / h_right;
+    rhs[i - 2] = slope_right - slope_left;
+  }
+  rhs[0] = rhs[0] + s[0];
+  rhs[m - 1] = rhs[m - 1] + e[0];
+  diag[0] = diag[0] + s[1];
+  diag[m - 1] = diag[m - 1] + e[1];
+  sub[m - 2] = sub[m - 2] + e[2];
+  sup[0] = sup[0] + s[2];
+  std::vector<double> f2(n);
+  tridiagonal(diag, sub, sup, rhs, f2.data() + 1);
+  const double h0 = x[1] - x[0];
+  switch (first.kind) {
+    case End::not_a_knot: f2[0] = f2[1] * (1.0 + h0 / (x[2] - x[1])) - f2[2] * h0 / (x[2] - x[1]); break;
+    case End::second_derivative: f2[0] = first.value; break;
+    case End::first_derivative: f2[0] = -6.0 * s[0] / h0 - f2[1] / 2.0; break;
+    case End::natural: f2[0] = 0.0; break;
+  }
+  const double hn = x[n - 1] - x[n - 2];
+  switch (last.kind) {
+    case End::not_a_knot:
+      f2[n - 1] = f2[n - 2] * (1.0 + hn / (x[n - 2] - x[n - 3])) - f2[n - 3] * hn / (x[n - 2] - x[n - 3]);
+      break;
+    case End::second_derivative: f2[n - 1] = last.value; break;
+    case End::first_derivative: f2[n - 1] = -6.0 * e[0] / hn - f2[n - 2] / 2.0; break;
+    case End::natural: f2[n - 1] = 0.0; break;
+  }
+  return f2;
+}
+
+// The lower knot of the interval holding t by bisection (SPLINT, UBICAR).
+int bisect(const std::vector<double>& x, double t) {
+  int lo = 0, hi = static_cast<int>(x.size()) - 1;
+  while (hi - lo > 1) {
+    const int k = (hi + lo) / 2;
+    if (x[k] > t) {
+      hi = k;
+    } else {
+      lo = k;
+    }
+  }
+  return lo;
+}
+
+}  // namespace
+
+Spline1D::Spline1D(std::vector<double> x, std::vector<double> y, Condition first, Condition last)
+    : x_(std::move(x)), y_(std::move(y)) {
+  if (x_.size() != y_.size()) throw DataError("spline: different numbers of abscissae and values");
+  m_ = second_derivatives(x_.data(), y_.data(), x_.size(), first, last);
+}
+
+// SPLINT (src/crplib/spline.cpp:226-254)
+void Spline1D::splint(double t, double& value, double& slope) const {
+  const int lo = bisect(x_, t), hi = lo + 1;
+  const double h = x_[hi] - x_[lo];
+  const double a = (x_[hi] - t) / h;
+  const double b = (t - x_[lo]) / h;
+  value = a * y_[lo] + b * y_[hi] + ((a * a * a - a) * m_[lo] + (b * b * b - b) * m_[hi]) * (h * h) / 6.0;
+  slope = (y_[hi] - y_[lo]) / h + h * (m_[hi] * (3.0 * b * b - 1.0) - m_[lo] * (3.0 * a * a - 1.0)) / 6.0;
+}
+
+// DPL (src/crplib/spline.cpp:171-190). k - 1 = the number of knots below t, except k = 2 at t = x_0; k = 1 and
+// k = n + 1 extrapolate linearly.
+double Spline1D::dpl(double t) const {
+  const int n = static_cast<int>(x_.size());
+  int k = static_cast<int>(std::lower_bound(x_.begin(), x_.end(), t) - x_.begin()) + 1;
+  if (t == x_[0]) k = 2;
+  if (k <= 1) {
+    const double e = x_[1] - x_[0];
+    return ((y_[1] - y_[0]) / e - m_[1] * e / 6.0) * (t - x_[0]) + y_[0];
+  }
+  if (k > n) {
+    const double e = x_[n - 1] - x_[n - 2];
+    return ((y_[n - 1] - y_[n - 2]) / e + m_[n - 2] * e / 6.0) * (t - x_[n - 1]) + y_[n - 1];
+  }
+  const int i = k - 1;
+  const double f = x_[i] - t;
+  const double g = t - x_[i - 1];
+  const double e = f + g;
+  return (-g * f * (m_[i - 1] * (f + e) + m_[i] * (g + e)) + 6.0 * (g * y_[i] + f * y_[i - 1])) / (6.0 * e);
+}
+
+// DPLP (src/crplib/spline.cpp:193-215)
+double Spline1D::dplp(double t) const {
+  const int n = static_cast<int>(x_.size());
+  int k = static_cast<int>(std::lower_bound(x_.begin(), x_.end(), t) - x_.begin()) + 1;
+  if (t == x_[0]) k = 2;
+  int lo, hi;
+  if (k <= 1) {
+    lo = 0;
+    hi = 1;
+  } else if (k > n) {
+    lo = n - 2;
+    hi = n - 1;
+  } else {
+    const int i = k - 1;
+    const double f = x_[i] - t;
+    const double g = t - x_[i - 1];
+    const double e = f + g;
+    const double e2 = e * e;
+    return (-m_[i - 1] * (3.0 * f * f - e2) + m_[i] * (3.0 * g * g - e2) + 6.0 * (y_[i] - y_[i - 1])) / (6.0 * e);
+  }
+  const double e = x_[hi] - x_[lo];
+  const double g = t - x_[lo];
+  const double f = x_[hi] - t;
+  return (y_[hi] - y_[lo]) / e + (m_[hi] * (2.0 * g - f) + m_[lo] * (g - 2.0 * f)) / 6.0;
+}
+
+// SPLIN2D (src/crplib/spline.cpp:302-331): F_xx along x for every y, F_yy along y for every x, then F_xxyy along x
+// of F_yy.
+Spline2D::Spline2D(std::vector<double> x, std::vector<double> y, std::vector<double> f)
+    : x_(std::move(x)), y_(std::move(y)), f_(std::move(f)) {
+  const std::size_t nx = x_.size(), ny = y_.size();
+  if (f_.size() != nx * ny) throw DataError("2D spline: the number of values does not match the grid");
+  check_grid(x_, "2D spline (x)");
+  check_grid(y_, "2D spline (y)");
+  fxx_.resize(nx * ny);
+  fyy_.resize(nx * ny);
+  fxxyy_.resize(nx * ny);
+  const Spline1D::Condition nak{};
+  for (std::size_t j = 0; j < ny; ++j) {
+    const auto m = second_derivatives(x_.data(), f_.data() + nx * j, nx, nak, nak);
+    std::copy(m.begin(), m.end(), fxx_.begin() + nx * j);
+  }
+  std::vector<double> column(ny);
+  for (std::size_t i = 0; i < nx; ++i) {
+    for (std::size_t j = 0; j < ny; ++j) column[j] = f_[i + nx * j];
+    const auto m = second_derivatives(y_.data(), column.data(), ny, nak, nak);
+    for (std::size_t j = 0; j < ny; ++j) fyy_[i + nx * j] = m[j];
+  }
+  for (std::size_t j = 0; j < ny; ++j) {
+    const auto m = second_derivatives(x_.data(), fyy_.data() + nx * j, nx, nak, nak);
+    std::copy(m.begin(), m.end(), fxxyy_.begin() + nx * j);
+  }
+}
+
+namespace {
+
+// SPLCOFII, one direction (src/crplib/spline.cpp:336-347)
+void cell_weights(const std::vector<double>& x, int i, double t, Spline2D::Weights& w, Spline2D::Weights& s) {
+  const double h2s6 = (x[i + 1] - x[i]) * (x[i + 1] - x[i]) / 6.0;
+  w.a = (x[i + 1] - t) / (x[i + 1] - x[i]);
+  w.b = 1.0 - w.a;
+  w.c = (w.a * w.a * w.a - w.a) * h2s6;
+  w.d = (w.b * w.b * w.b - w.b) * h2s6;
+  s.a = -1.0 / (x[i + 1] - x[i]);
+  s.b = -s.a;
+  s.c = (3.0 * (w.a * w.a) * s.a - s.a) * h2s6;
+  s.d = (3.0 * (w.b * w.b) * s.b - s.b) * h2s6;
+}
+
+}  // namespace
+
+std::optional<Spline2D::Cell> Spline2D::locate(double x, double y) const {
+  if (x < x_.front() || x > x_.back() || y < y_.front() || y > y_.back()) return std::nullopt;
+  Cell c{};
+  c.i = bisect(x_, x);
+  c.j = bisect(y_, y);
+  cell_weights(x_, c.i, x, c.wx, c.sx);
+  cell_weights(y_, c.j, y, c.wy, c.sy);
+  return c;
+}
+
+// SPLINII (src/crplib/spline.cpp:366-393): the value and d/dx sum over the x weights outside, d/dy over the y ones.
+Spline2D::Value Spline2D::at(const Cell& c) const {
+  const std::size_t nx = x_.size();
+  const std::size_t p00 = c.i + nx * c.j, p10 = p00 + 1, p01 = p00 + nx, p11 = p01 + 1;
+  const auto& v = f_;
+  auto x_outer = [&](const Weights& wx, const Weights& wy) {
+    return wx.a * (wy.a * v[p00] + wy.b * v[p01] + wy.c * fyy_[p00] + wy.d * fyy_[p01]) +
+           wx.b * (wy.a * v[p10] + wy.b * v[p11] + wy.c * fyy_[p10] + wy.d * fyy_[p11]) +
+           wx.c * (wy.a * fxx_[p00] + wy.b * fxx_[p01] + wy.c * fxxyy_[p00] + wy.d * fxxyy_[p01]) +
+           wx.d * (wy.a * fxx_[p10] + wy.b * fxx_[p11] + wy.c * fxxyy_[p10] + wy.d * fxxyy_[p11]);
+  };
+  auto y_outer = [&](const Weights& wx, const Weights& wy) {
+    return wy.a * (wx.a * v[p00] + wx.b * v[p10] + wx.c * fxx_[p00] + wx.d * fxx_[p10]) +
+           wy.b * (wx.a * v[p01] + wx.b * v[p11] + wx.c * fxx_[p01] + wx.d * fxx_[p11]) +
+           wy.c * (wx.a * fyy_[p00] + wx.b * fyy_[p10] + wx.c * fxxyy_[p00] + wx.d * fxxyy_[p10]) +
+           wy.d * (wx.a * fyy_[p01] + wx.b * fyy_[p11] + wx.c * fxxyy_[p01] + wx.d * fxxyy_[p11]);
+  };
+  return {x_outer(c.wx, c.wy), y_outer(c.wx, c.sy), x_outer(c.sx, c.wy)};
+}
+
+// ---------------------------------------------------------------------------------------------------------------
+// Atom-surface PES (src/crplib/hexagonal_atom_surface.cpp, atom_surface.cpp)
+
+namespace {
+
+// Fourier tables of the interpolation over the sites, element (site, j) at site + n j (src/crplib/fourier_tables.cpp;
+// Fortran DATA statements of FOUR3D and FOUR6D, with their scale factors). Integer weights written out.
+constexpr std::array<double, 49> kAtomC6vWeights = {
+    1.0, 3.0,  2.0,  6.0,  12.0, 6.0,  6.0,  2.0, -2.0, -2.0, 2.0,  -8.0, 8.0,  0.0, 2.0, -2.0, 4.0,
+    -4.0, 4.0, 2.0,  -6.0, 2.0,  6.0,  -2.0, -6.0, 0.0, 0.0, 0.0, 2.0,  -2.0, -2.0, 2.0, 4.0, -4.0,
+    0.0, 1.0,  -1.0, 2.0,  -2.0, -4.0, -2.0, 6.0, 0.666666666666667, 2.0, 1.333333333333, 4.0, -4.0, -2.0, -2.0};
+
+constexpr double kA = 5.1961524227066318805600;
+constexpr double kB = 20.7846096908265275222400;
+constexpr double kC = 15.5884572681198956416800;
+constexpr double kD = 10.3923048454132637611200;
+constexpr std::array<double, 100> kAtomC3vWeights = {
+    3.00, 9.00,  3.00,  9.00,  18.00,  18.00,  18.00,  9.00,  18.00,  3.00,    //
+    6.00, -6.00, -3.00, 3.00,  -12.00, 24.00,  0.00,   3.00,  -12.00, -3.00,   //
+    0.00, 0.00,  -kA,   -kA,   -kB,    0.00,   0.00,   kA,    kB,     kA,      //
+    6.00, -6.00, 6.00,  -6.00, 6.00,   6.00,   -18.00, -6.00, 6.00,   6.00,    //
+    6.00, 18.00, -3.00, -9.00, 0.00,   0.00,   0.00,   -9.00, 0.00,   -3.00,   //
+    0.00, 0.00,  kA,    -kC,   0.00,   0.00,   0.00,   kC,    0.00,   -kA,     //
+    6.00, -6.00, -3.00, 3.00,  6.00,   -12.00, 0.00,   3.00,  6.00,   -3.00,   //
+    0.00, 0.00,  -kA,   -kA,   kD,     0.00,   0.00,   kA,    -kD,    kA,      //
+    3.00, -3.00, 3.00,  -3.00, -6.00,  -6.00,  18.00,  -3.00, -6.00,  3.00,    //
+    2.00, 6.00,  2.00,  6.00,  -6.00,  -6.00,  -6.00,  6.00,  -6.00,  2.00};
+
+constexpr std::array<double, 144> kMoleculeWeights = {
+    1, 1,  1,  1,  1,  1,  1,  1,  1,  1,  1,  1,  2, -2, -1, 1,  -2, 2,  -1, 1,  -1, 1,  -1, 1,
+    2, 2,  -1, 1,  -2, -2, 1,  -1, -1, 1,  1,  -1, 2, -2, -1, -1, 2,  -2, 1,  1,  -1, -1, 1,  1,
+    0, 0,  1,  1,  0,  0,  -1, 1,  -1, -1, 1,  -1, 0, 0,  1,  1,  0,  0,  1,  -1, -1, -1, -1, 1,
+    0, 0,  -1, 1,  0,  0,  -1, -1, 1,  -1, 1,  1,  1, 1,  1,  -1, -1, -1, -1, 1,  1,  -1, -1, 1,
+    1, -1, 1,  -1, -1, 1,  1,  -1, 1,  -1, 1,  -1, 1, -1, 1,  1,  1,  -1, -1, -1, 1,  1,  -1, -1,
+    2, 2,  -1, -1, 2,  2,  -1, -1, -1, -1, -1, -1, 0, 0,  1,  -1, 0,  0,  -1, -1, -1, 1,  1,  1};
+
+template <std::size_t N>
+constexpr std::array<double, N> scaled(const std::array<double, N>& w, double divisor) {
+  std::array<double, N> t{};
+  for (std::size_t i = 0; i < N; ++i) t[i] = w[i] / divisor;
+  return t;
+}
+
+constexpr std::array<double, 144> molecule_table() {
+  constexpr double r1 = kUsq3 / 4.00;
+  constexpr double r2 = 1.00 / 36.00;
+  std::array<double, 144> t{};
+  for (int j = 0; j < 12; ++j) {
+    double factor = 3.0 * r2;
+    if (j >= 4 && j <= 6) factor = r1;
+    if (j == 10) factor = r2;
+    if (j == 11) factor = r1 / 3.0;
+    for (int i = 0; i < 12; ++i) t[i + 12 * j] = factor * kMoleculeWeights[i + 12 * j];
+  }
+  return t;
+}
+
+constexpr std::array<double, 49> kAtomC6vTable = scaled(kAtomC6vWeights, 36.00);
+constexpr std::array<double, 100> kAtomC3vTable = scaled(kAtomC3vWeights, 108.00);
+constexpr std::array<double, 144> kMoleculeTable = molecule_table();
+
+// The C6v cluster: crystal coordinates of 30 top-layer atoms (Fortran DATA ivec of MURII).
+constexpr std::array<int, 30> kC6vI = {0,  1,  2, 3, -1, 0, 1,  2,  3, -2, -1, 0,  1,  2, 3,
+                                       -2, -1, 0, 1, 2,  3, -2, -1, 0, 1,  2,  -2, -1, 0, 1};
+constexpr std::array<int, 30> kC6vJ = {-2, -2, -2, -2, -1, -1, -1, -1, -1, 0, 0, 0, 0, 0, 0,
+                                       1,  1,  1,  1,  1,  1,  2,  2,  2,  2, 2, 3, 3, 3, 3};
+
+// INTEGER = REAL of the cell index, one further for negative values (ORIGEN2, src/crplib/atom_surface.cpp:58-70).
+int cell_index(double ori) {
+  if (!(std::abs(ori) < 2.0e9))
+    throw EvaluationError("CRP atom-surface PES: coordinate " + std::to_string(ori) +
+                          " lattice constants from the origin");
+  return ori >= 0.0 ? static_cast<int>(ori) : static_cast<int>(ori - 1.0);
+}
+
+// (x, y) folded into the skewed unit cell at the origin (ORIGEN2, src/crplib/atom_surface.cpp:73-82).
+std::pair<double, double> fold(double x, double y, double delta) {
+  double bigx = x - y * kUsq3;
+  double bigy = 2.00 * y * kUsq3;
+  const int i1 = cell_index(bigx / delta);
+  const int i2 = cell_index(bigy / delta);
+  bigx = bigx - i1 * delta;
+  bigy = bigy - i2 * delta;
+  return {bigx + bigy * 0.500, kSq3 * bigy * 0.500};
+}
+
+}  // namespace
+
+AtomSurface::AtomSurface(const DataSet& data)
+    : c3v_(data.symmetry == Symmetry::c3v),
+      a_(to_length(data.lattice_constant, data.units.length, LengthUnit::bohr)),
+      table_(data.symmetry == Symmetry::c3v ? kAtomC3vTable.data() : kAtomC6vTable.data()),
+      n_(data.symmetry == Symmetry::c3v ? 10 : 7) {
+  for (double s : data.interlayer_spacings) spacings_.push_back(to_length(s, data.units.length, LengthUnit::bohr));
+  {  // the corrugation function, clamped at both ends (src/crplib/atom_surface.cpp:41-51)
+    const double s1 = to_force(data.corrugation_slope_first, data.units, kInternalUnits);
+    const double s2 = to_force(data.corrugation_slope_last, data.units, kInternalUnits);
+    std::vector<double> x, f;
+    for (std::size_t i = 0; i < data.corrugation.x.size(); ++i) {
+      x.push_back(to_length(data.corrugation.x[i], data.units.length, LengthUnit::bohr));
+      f.push_back(to_energy(data.corrugation.f[i], data.units.energy, EnergyUnit::hartree));
+    }
+    corrugation_ = Spline1D(x, f, {Spline1D::End::first_derivative, s1}, {Spline1D::End::first_derivative, s2});
+  }
+  // MURII cluster in its loop order (src/crplib/hexagonal_atom_surface.cpp:29-60)
+  if (!c3v_) {
+    for (std::size_t k = 0; k < kC6vI.size(); ++k) {
+      const double bx = kC6vI[k] * a_;
+      const double by = kC6vJ[k] * a_;
+      cluster_.push_back({bx + by * 0.500, by * kSq3 * 0.500, 0.0, 0});
+    }
+  } else {
+    for (int i = -3; i <= 4; ++i) {
+      for (int j = -3; j <= 4; ++j) {
+        for (int k = 0; k < 4; ++k) {  // fcc(111) ABC stacking: layer k shifted by (k mod 3)/3 of both vectors
+          const double u = (static_cast<double>(i) + (k % 3) * (1.0 / 3.0)) * a_;
+          const double v = (static_cast<double>(j) + (k % 3) * (1.0 / 3.0)) * a_;
+          double z = 0.0;
+          for (int l = 0; l < k; ++l) z = z - spacings_[l];
+          cluster_.push_back({u + v * kCosPi3, v * kSinPi3, z, k});
+        }
+      }
+    }
+  }
+  // The cuts reduced by the repulsive sum at their sites (src/crplib/hexagonal_atom_surface.cpp:80-106)
+  for (const AtomCut& cut : data.atom_cuts) {
+    const double s1 = to_force(cut.slope_first, data.units, kInternalUnits);
+    const double s2 = to_force(cut.slope_last, data.units, kInternalUnits);
+    const std::size_t n = cut.v.x.size();
+    std::vector<double> z(n), reduced(n);
+    double d1 = 0.0, d2 = 0.0;
+    for (std::size_t i = 0; i < n; ++i) {
+      z[i] = to_length(cut.v.x[i], data.units.length, LengthUnit::bohr);
+      const double pot = to_energy(cut.v.f[i], data.units.energy, EnergyUnit::hartree);
+      std::array<double, 4> rep{};
+      try {
+        rep = repulsion(cut.x * a_, cut.y * a_, z[i]);
+      } catch (const EvaluationError& e) {
+        throw DataError("Atom/" + cut.file + ": " + e.what());
+      }
+      reduced[i] = pot - rep[0];
+      if (i == 0) d1 = s1 - rep[3];
+      if (i == n - 1) d2 = s2 - rep[3];
+    }
+    cuts_.emplace_back(z, reduced, Spline1D::Condition{Spline1D::End::first_derivative, d1},
+                       Spline1D::Condition{Spline1D::End::first_derivative, d2});
+  }
+}
+
+// MURII: the corrugation function of every cluster atom within its reach, summed in cluster order
+// (src/crplib/hexagonal_atom_surface.cpp:109-154).
+std::array<double, 4> AtomSurface::repulsion(double x, double y, double z) const {
+  std::array<double, 4> s{0.0, 0.0, 0.0, 0.0};
+  const double reach = corrugation_.x().back(), nearest = corrugation_.x().front();
+  for (const ClusterAtom& at : cluster_) {
+    const double dx = x - at.x, dy = y - at.y, dz = z - at.z;
+    const double r = std::sqrt(dx * dx + dy * dy + dz * dz);
+    double vr = 0.0, dvdr = 0.0;
+    if (!(r > reach)) {
+      if (r < nearest)
+        throw EvaluationError("CRP atom-surface PES: atom " + std::to_string(r) +
+                              " bohr from a surface atom, closer than the corrugation function");
+      corrugation_.splint(r, vr, dvdr);
+    }
+    s[0] = s[0] + vr;
+    if (!(r > 0.0) && at.layer != 0) throw EvaluationError("CRP atom-surface PES: atom on a sub-surface atom");
+    if (r > 0.0) {
+      const double q = dvdr / r;
+      s[1] = s[1] + q * dx;
+      s[2] = s[2] + q * dy;
+      s[3] = s[3] + q * dz;
+    } else {
+      s[3] = s[3] + dvdr;
+    }
+  }
+  return s;
+}
+
+// FOUR3II: the star functions of the hexagonal cell and their derivatives, in crystal coordinates, the y
+// derivative turned Cartesian at the end (src/crplib/hexagonal_atom_surface.cpp:187-248).
+void AtomSurface::basis(double xa, double ya, double* f, double* dfdx, double* dfdy) const {
+  const double fac = 2.00 * kPi / a_;
+  const double bigx = xa - ya * kUsq3;
+  const double bigy = 2.00 * ya * kUsq3;
+  const double p = fac * bigx, q = fac * bigy;
+  const double c2a = std::cos(p), c2b = std::cos(q), c2c = std::cos(p + q);
+  const double s2a = std::sin(p), s2b = std::sin(q), s2c = std::sin(p + q);
+  const double c3a = std::cos(2.00 * p + q), c3b = std::cos(p + 2.00 * q), c3c = std::cos(p - q);
+  const double s3a = std::sin(2.00 * p + q), s3b = std::sin(p + 2.00 * q), s3c = std::sin(p - q);
+  const double c4a = std::cos(2.00 * p), c4b = std::cos(2.00 * q), c4c = std::cos(2.00 * (p + q));
+  const double s4a = std::sin(2.00 * p), s4b = std::sin(2.00 * q), s4c = std::sin(2.00 * (p + q));
+  const double c5a = std::cos(3.00 * p + q), c5b = std::cos(p + 3.00 * q), c5c = std::cos(2.00 * p + 3.00 * q);
+  const double c5d = std::cos(3.00 * p + 2.00 * q), c5e = std::cos(-p + 2.00 * q), c5f = std::cos(2.00 * p - q);
+  const double s5a = std::sin(3.00 * p + q), s5b = std::sin(p + 3.00 * q), s5c = std::sin(2.00 * p + 3.00 * q);
+  const double s5d = std::sin(3.00 * p + 2.00 * q), s5e = std::sin(-p + 2.00 * q), s5f = std::sin(2.00 * p - q);
+  const double c6a = std::cos(3.00 * p), c6b = std::cos(3.00 * q), c6c = std::cos(3.00 * (p + q));
+  const double s6a = std::sin(3.00 * p), s6b = std::sin(3.00 * q), s6c = std::sin(3.00 * (p + q));
+  const double c7a = std::cos(2.00 * (2.00 * p + q)), c7b = std::cos(2.00 * (p + 2.00 * q));
+  const double c7c = std::cos(2.00 * (p - q));
+  const double s7a = std::sin(2.00 * (2.00 * p + q)), s7b = std::sin(2.00 * (p + 2.00 * q));
+  const double s7c = std::sin(2.00 * (p - q));
+  int k = 0;
+  auto put = [&](double v, double dx, double dy) {
+    f[k] = v;
+    dfdx[k] = dx;
+    dfdy[k] = dy;
+    ++k;
+  };
+  put(1.00, 0.00, 0.00);
+  put(c2a + c2b + c2c, -fac * (s2a + s2c), -fac * (s2b + s2c));
+  if (c3v_) put(-s2a - s2b + s2c, -fac * (c2a - c2c), -fac * (c2b - c2c));
+  put(c3a + c3b + c3c, -fac * (2.00 * s3a + s3b + s3c), -fac * (s3a + 2.00 * s3b - s3c));
+  put(c4a + c4b + c4c, -2.00 * fac * (s4a + s4c), -2.00 * fac * (s4b + s4c));
+  if (c3v_) put(-s4a - s4b + s4c, -2.00 * fac * (c4a - c4c), -2.00 * fac * (c4b - c4c));
+  put(c5a + c5b + c5c + c5d + c5e + c5f, -fac * (3.00 * s5a + s5b + 2.00 * s5c + 3.00 * s5d - s5e + 2.00 * s5f),
+      -fac * (s5a + 3.00 * s5b + 3.00 * s5c + 2.00 * s5d + 2.00 * s5e - s5f));
+  if (c3v_)
+    put(-s5a - s5b + s5c + s5d - s5e - s5f, fac * (-3.00 * c5a - c5b + 2.00 * c5c + 3.00 * c5d + c5e - 2.00 * c5f),
+        fac * (-c5a - 3.00 * c5b + 3.00 * c5c + 2.00 * c5d - 2.00 * c5e + c5f));
+  put(c6a + c6b + c6c, -3.00 * fac * (s6a + s6c), -3.00 * fac * (s6b + s6c));
+  put(c7a + c7b + c7c, -2.00 * fac * (2.00 * s7a + s7b + s7c), -2.00 * fac * (s7a + 2.00 * s7b - s7c));
+  for (int i = 0; i < n_; ++i) dfdy[i] = (-dfdx[i] + 2.00 * dfdy[i]) * kUsq3;
+}
+
+// ATII (src/crplib/hexagonal_atom_surface.cpp:253-283): the repulsive sum at the folded position plus the Fourier
+// interpolation of the reduced cuts.
+AtomValue AtomSurface::evaluate(const Vec3& p) const {
+  const auto [xf, yf] = fold(p.x, p.y, a_);
+  const std::array<double, 4> rep = repulsion(xf, yf, p.z);
+  const std::vector<double>& z = cuts_.front().x();
+  if (p.z < z.front() || p.z > z.back())
+    throw EvaluationError("CRP atom-surface PES: z = " + std::to_string(p.z) + " bohr outside the data range");
+  std::array<double, 10> vs{}, dvs{}, f{}, dfdx{}, dfdy{};
+  for (int i = 0; i < n_; ++i) {
+    vs[i] = cuts_[i].dpl(p.z);
+    dvs[i] = cuts_[i].dplp(p.z);
+  }
+  basis(p.x, p.y, f.data(), dfdx.data(), dfdy.data());
+  double v = 0.00, gx = 0.00, gy = 0.00, gz = 0.00;
+  for (int j = 0; j < n_; ++j) {
+    double a = 0.00, b = 0.00;
+    for (int s = 0; s < n_; ++s) {
+      a = a + table_[s + n_ * j] * vs[s];
+      b = b + table_[s + n_ * j] * dvs[s];
+    }
+    v = v + a * f[j];
+    gx = gx + a * dfdx[j];
+    gy = gy + a * dfdy[j];
+    gz = gz + b * f[j];
+  }
+  return {v + rep[0], {gx + rep[1], gy + rep[2], gz + rep[3]}};
+}
+
+// ---------------------------------------------------------------------------------------------------------------
+// Molecule-surface PES
+
+// CAMBCORD with the centre at the midpoint (src/crplib/molecule_surface.cpp:22-41)
+Geometry molecule_geometry(const Vec3& a, const Vec3& b) {
+  constexpr double cc = 0.500;
+  const double r = std::sqrt((b.x - a.x) * (b.x - a.x) + (b.y - a.y) * (b.y - a.y) + (b.z - a.z) * (b.z - a.z));
+  if (r == 0.0) throw EvaluationError("CRP molecule-surface PES: the two atoms coincide");
+  Geometry g{};
+  g.r = r;
+  g.x = a.x * (1.0 - cc) + b.x * cc;
+  g.y = a.y * (1.0 - cc) + b.y * cc;
+  g.z = a.z * (1.0 - cc) + b.z * cc;
+  double ct = (b.z - a.z) / r;
+  if (ct > 1.0) ct = 1.0;
+  g.theta = std::acos(ct);
+  const double dx = b.x - a.x, dy = b.y - a.y;
+  g.phi = (dx == 0.0 && dy == 0.0) ? 0.0 : std::atan2(dy, dx);
+  return g;
+}
+
+namespace {
+
+// The site functions: angular interpolation at one site from its cuts at fixed orientations
+// (src/crplib/site_interpolation.cpp:60-222). Each returns (value, d/dZ, d/dr, d/dphi, d/dtheta); the first three
+// apply the same formula to the cut values, their Z and their r derivatives.
+struct Site {
+  double v[3];
+  double d_phi, d_theta;
+};
+struct Trig {
+  double c2, s2, c4, s4;  // of 2 theta, 4 theta
+};
+
+double cutc(const Spline2D::Value* cuts, int comp, int k) {
+  const Spline2D::Value& c = cuts[k];
+  return comp == 0 ? c.v : (comp == 1 ? c.d_dy : c.d_dx);
+}
+
+// theta = 0, 45, 90, 135 degrees combined (the hollow and t2 sites)
+double four(double v0, double v45, double v90, double v135, const Trig& t) {
+  return 0.2500 * ((v0 + v45 + v90 + v135) + (v0 + v90 - v45 - v135) * t.c4) +
+         0.500 * ((v0 - v90) * t.c2 + (v45 - v135) * t.s2);
+}
+double four_dtheta(double v0, double v45, double v90, double v135, const Trig& t) {
+  return -(v0 - v90) * t.s2 + (v45 - v135) * t.c2 - (v0 + v90 - v45 - v135) * t.s4;
+}
+double four_dphi(double d45, double d90, double d135, const Trig& t) {
+  return 0.2500 * ((d45 + d90 + d135) + (d90 - d45 - d135) * t.c4) + 0.500 * ((-d90) * t.c2 + (d45 - d135) * t.s2);
+}
+
+// top: theta 0; 90/0; 90/30
+Site top_site(const Spline2D::Value* c, const Trig& t, double phi) {
+  const double c6 = std::cos(6.00 * phi), s6 = std::sin(6.00 * phi);
+  Site s{};
+  for (int i = 0; i < 3; ++i) {
+    const double v0 = cutc(c, i, 0);
+    const double v90 = 0.500 * ((cutc(c, i, 1) + cutc(c, i, 2)) + (cutc(c, i, 1) - cutc(c, i, 2)) * c6);
+    s.v[i] = 0.500 * ((v0 + v90) + (v0 - v90) * t.c2);
+    if (i == 0) {
+      const double d90 = -3.00 * (cutc(c, 0, 1) - cutc(c, 0, 2)) * s6;
+      s.d_phi = 0.500 * d90 * (1.00 - t.c2);
+      s.d_theta = -(v0 - v90) * t.s2;
+    }
+  }
+  return s;
+}
+
+// bridge: theta 0; 90/0; 90/60; 90/90
+Site bridge_site(const Spline2D::Value* c, const Trig& t, double phi) {
+  constexpr double u3 = 1.00 / 3.00;
+  const double c2 = std::cos(2.00 * phi), c4 = std::cos(4.00 * phi);
+  const double s2 = std::sin(2.00 * phi), s4 = std::sin(4.00 * phi);
+  Site s{};
+  for (int i = 0; i < 3; ++i) {
+    const double v0 = cutc(c, i, 0), a = cutc(c, i, 1), b = cutc(c, i, 2), d = cutc(c, i, 3);
+    const double v90 = (a + 2.00 * b) * u3 - 0.500 * (d - a) * c2 + (0.500 * (d - a) - 2.00 * (b - a) * u3) * c4;
+    s.v[i] = 0.500 * (v0 * (1.00 + t.c2) + v90 * (1.00 - t.c2));
+    if (i == 0) {
+      const double d90 = (d - a) * s2 - (2.00 * (d - a) - 8.00 * (b - a) * u3) * s4;
+      s.d_phi = 0.500 * d90 * (1.00 - t.c2);
+      s.d_theta = (v90 - v0) * t.s2;
+    }
+  }
+  return s;
+}
+
+// fcc / hcp hollow: theta 0; 90/0; 90/30; 45/30; 45/210
+Site hollow_site(const Spline2D::Value* c, const Trig& t, double phi) {
+  const double s3 = std::sin(3.00 * phi), c6 = std::cos(6.00 * phi);
+  const double c3 = std::cos(3.00 * phi), s6 = std::sin(6.00 * phi);
+  Site s{};
+  for (int i = 0; i < 3; ++i) {
+    const double v0 = cutc(c, i, 0), a = cutc(c, i, 1), b = cutc(c, i, 2), d = cutc(c, i, 3), e = cutc(c, i, 4);
+    const double v45 = 0.500 * ((d + e) + (d - e) * s3);
+    const double v135 = 0.500 * ((d + e) - (d - e) * s3);
+    const double v90 = 0.500 * ((a + b) + (a - b) * c6);
+    s.v[i] = four(v0, v45, v90, v135, t);
+    if (i == 0) {
+      const double d45 = 1.500 * (d - e) * c3;
+      const double d90 = -3.00 * (a - b) * s6;
+      s.d_phi = d90 * (0.2500 * (1.00 + t.c4) - 0.500 * t.c2) + d45 * t.s2;
+      s.d_theta = four_dtheta(v0, v45, v90, v135, t);
+    }
+  }
+  return s;
+}
+
+// The t2 sites: theta 0, two 90 cuts (p, q), three 45 cuts (g, h, k) at the azimuth phi' = phi - shift. `mid`
+// selects which 45 cut is the middle one of the cos 2 phi' term and `sign` the orientation of the cos phi' terms
+// (C6v t2h: 45 cuts 30, 210, 120 with mid 5; C3v t2h: 30, 120, 210 with mid 4; C3v t2f: 150, 240, 330 with mid 4
+// and the odd terms reversed; src/crplib/site_interpolation.cpp:120-194).
+Site t2_site(const Spline2D::Value* c, const Trig& t, double phi, double shift, int mid, bool reversed) {
+  const double ph = phi - shift;
+  const double cp = std::cos(ph), sp = std::sin(ph), c2p = std::cos(2.00 * ph), s2p = std::sin(2.00 * ph);
+  const int first45 = 3, last45 = mid == 5 ? 4 : 5;
+  Site s{};
+  for (int i = 0; i < 3; ++i) {
+    const double v0 = cutc(c, i, 0), p = cutc(c, i, 1), q = cutc(c, i, 2);
+    const double g = cutc(c, i, first45), k = cutc(c, i, last45), m = cutc(c, i, mid);
+    const double odd = reversed ? (-g + k) : (g - k);
+    const double v45 = 0.2500 * ((g + 2.00 * m + k) + (g - 2.00 * m + k) * c2p) + 0.500 * odd * cp;
+    const double v135 = 0.2500 * ((g + 2.00 * m + k) + (g - 2.00 * m + k) * c2p) - 0.500 * odd * cp;
+    const double v90 = reversed ? 0.500 * ((p + q) + (-p + q) * c2p) : 0.500 * ((p + q) + (p - q) * c2p);
+    s.v[i] = four(v0, v45, v90, v135, t);
+    if (i == 0) {
+      const double d45 = -0.500 * (odd * sp + (g - 2.00 * m + k) * s2p);
+      const double d135 = 0.500 * (odd * sp - (g - 2.00 * m + k) * s2p);
+      const double d90 = reversed ? -(-p + q) * s2p : -(p - q) * s2p;
+      s.d_phi = four_dphi(d45, d90, d135, t);
+      s.d_theta = four_dtheta(v0, v45, v90, v135, t);
+    }
+  }
+  return s;
+}
+
+// FOUR6II: the 12 basis functions of the molecular interpolation and their Cartesian x, y derivatives
+// (src/crplib/hexagonal_interaction.cpp:16-72).
+void molecule_basis(double x, double y, double a, double* f, double* dfdx, double* dfdy) {
+  const double fac = 2.00 * kPi / a;
+  const double bigx = x - y * kUsq3;
+  const double bigy = 2.00 * y * kUsq3;
+  const double p = fac * bigx, q = fac * bigy;
+  const double c2x = std::cos(2.00 * p), c2y = std::cos(2.00 * q), c2xy = std::cos(2.00 * (p + q));
+  const double s2x = std::sin(2.00 * p), s2y = std::sin(2.00 * q), s2xy = std::sin(2.00 * (p + q));
+  f[0] = 1.00;
+  f[1] = std::cos(p);
+  f[2] = std::cos(q);
+  f[3] = std::cos(p + q);
+  f[4] = std::sin(p);
+  f[5] = std::sin(q);
+  f[6] = std::sin(p + q);
+  f[7] = std::cos(2.00 * p + q);
+  f[8] = std::cos(p + 2.00 * q);
+  f[9] = std::cos(p - q);
+  f[10] = c2x + c2y + c2xy;
+  f[11] = -s2x - s2y + s2xy;
+  dfdx[0] = 0.00;
+  dfdx[1] = -fac * f[4];
+  dfdx[2] = 0.00;
+  dfdx[3] = -fac * f[6];
+  dfdx[4] = fac * f[1];
+  dfdx[5] = 0.00;
+  dfdx[6] = fac * f[3];
+  dfdx[7] = -2.00 * fac * std::sin(2.00 * p + q);
+  dfdx[8] = -fac * std::sin(p + 2.00 * q);
+  dfdx[9] = -fac * std::sin(p - q);
+  dfdx[10] = -2.0 * fac * (s2x + s2xy);
+  dfdx[11] = 2.0 * fac * (-c2x + c2xy);
+  dfdy[0] = 0.00;
+  dfdy[1] = 0.00;
+  dfdy[2] = -fac * f[5];
+  dfdy[3] = -fac * f[6];
+  dfdy[4] = 0.00;
+  dfdy[5] = fac * f[2];
+  dfdy[6] = fac * f[3];
+  dfdy[7] = 0.500 * dfdx[7];
+  dfdy[8] = 2.00 * dfdx[8];
+  dfdy[9] = -dfdx[9];
+  dfdy[10] = -2.0 * fac * (s2y + s2xy);
+  dfdy[11] = 2.0 * fac * (-c2y + c2xy);
+  for (int i = 0; i < 12; ++i) dfdy[i] = (-dfdx[i] + 2.00 * dfdy[i]) * kUsq3;
+}
+
+std::vector<double> lengths(const std::vector<double>& x, const Units& u) {
+  std::vector<double> y;
+  for (double v : x) y.push_back(to_length(v, u.length, LengthUnit::bohr));
+  return y;
+}
+std::vector<double> energies(const std::vector<double>& x, const Units& u) {
+  std::vector<double> y;
+  for (double v : x) y.push_back(to_energy(v, u.energy, EnergyUnit::hartree));
+  return y;
+}
+
+}  // namespace
+
+// Read_MoleculeSurface_Data (src/crplib/molecule_surface.cpp:119-192)
+MoleculeSurface::MoleculeSurface(const DataSet& data)
+    : symmetry_(data.symmetry),
+      a_(to_length(data.lattice_constant, data.units.length, LengthUnit::bohr)),
+      atom_(data) {
+  if (symmetry_ == Symmetry::c4v) throw DataError("C4v data sets are not supported by this library");
+  const Units& in = data.units;
+  zso_min_ = to_length(data.zso_min, in.length, LengthUnit::bohr);
+  zso_max_ = to_length(data.zso_max, in.length, LengthUnit::bohr);
+  z_dependence_ = Spline1D(lengths(data.z_dependence.x, in), energies(data.z_dependence.f, in), {},
+                           {Spline1D::End::first_derivative, 0.00});
+  h2_ = Spline1D(lengths(data.h2.x, in), energies(data.h2.f, in), {}, {});
+  for (const MoleculeCut& cut : data.molecule_cuts) {
+    if (!cut.present) continue;
+    const std::vector<double> r = lengths(cut.r, in), z = lengths(cut.z, in);
+    std::vector<double> v = energies(cut.v, in);
+    if (!data.cuts_interpolation) {  // total energies: subtract both atom potentials (corrugation reduction)
+      const double ucm = cut.x * a_, vcm = cut.y * a_;
+      const double xcm = ucm + vcm / 2.0, ycm = std::sqrt(3.0) * vcm / 2.0;
+      const double theta = cut.theta * kPi / 180.0, phi = cut.phi * kPi / 180.0;
+      for (std::size_t j = 0; j < z.size(); ++j) {
+        for (std::size_t i = 0; i < r.size(); ++i) {
+          const Vec3 pa{xcm - 0.5 * r[i] * std::sin(theta) * std::cos(phi),
+                        ycm - 0.5 * r[i] * std::sin(theta) * std::sin(phi), z[j] - 0.5 * r[i] * std::cos(theta)};
+          const Vec3 pb{xcm + 0.5 * r[i] * std::sin(theta) * std::cos(phi),
+                        ycm + 0.5 * r[i] * std::sin(theta) * std::sin(phi), z[j] + 0.5 * r[i] * std::cos(theta)};
+          double va = 0.0, vb = 0.0;
+          try {
+            va = atom_.evaluate(pa).v;
+            vb = atom_.evaluate(pb).v;
+          } catch (const EvaluationError& e) {
+            throw DataError("Molecule/" + cut.file + ": " + e.what());
+          }
+          double& value = v[i + r.size() * j];
+          value = value - va - vb;
+        }
+      }
+    }
+    if (r_min_ < r.front()) r_min_ = r.front();
+    if (r_max_ > r.back()) r_max_ = r.back();
+    if (z_min_ < z.front()) z_min_ = z.front();
+    cuts_.emplace_back(r, z, v);
+  }
+}
+
+MoleculeSurface::Switch MoleculeSurface::switch_function(double z) const {
+  if (z <= zso_min_) return {1.00, 0.00};
+  if (z >= zso_max_) return {0.00, 0.00};
+  const double f = 0.500 + 0.500 * std::cos((z - zso_min_) * kPi / (zso_max_ - zso_min_));
+  const double df = -0.500 * kPi / (zso_max_ - zso_min_) * std::sin((z - zso_min_) * kPi / (zso_max_ - zso_min_));
+  return {f, df};
+}
+
+MoleculeSurface::GasPhase MoleculeSurface::gas_phase(double z, double r) const {
+  double hv = 0.0, hs = 0.0;
+  h2_.splint(r, hv, hs);
+  double v1 = 0.0, dz = 0.0;
+  if (z >= zso_min_ && z <= z_dependence_.x().back()) {
+    z_dependence_.splint(z, v1, dz);
+  } else if (z > z_dependence_.x().back()) {
+    v1 = z_dependence_.y().back();
+    dz = 0.00;
+  } else {
+    throw EvaluationError("CRP gas-phase potential: Z = " + std::to_string(z) + " below ZsoMin");
+  }
+  return {hv + v1, dz, hs};
+}
+
+// RRINTII's site part and SISII (src/crplib/hexagonal_interaction.cpp:77-160): the 12 Fourier sites from their
+// cuts (rotated azimuths for the symmetry images), then the Fourier interpolation over the centre. Returns
+// (I, dI/dZ, dI/dr, dI/dphi, dI/dtheta, dI/dX, dI/dY).
+std::array<double, 7> MoleculeSurface::interaction(const Spline2D::Value* c, double theta, double phi, double x,
+                                                   double y) const {
+  constexpr double pis3 = kPi / 3.00;
+  constexpr double pi2s3 = 2.00 * pis3;
+  constexpr double pis6 = kPi / 6.00;
+  const Trig t{std::cos(2.00 * theta), std::sin(2.00 * theta), std::cos(4.00 * theta), std::sin(4.00 * theta)};
+  std::array<Site, 12> s{};
+  s[0] = top_site(c, t, phi);
+  s[1] = bridge_site(c + 3, t, phi);
+  s[4] = bridge_site(c + 3, t, phi + pis3);
+  s[5] = bridge_site(c + 3, t, phi - pis3);
+  if (symmetry_ == Symmetry::c6v) {
+    s[2] = hollow_site(c + 7, t, phi);
+    s[8] = hollow_site(c + 7, t, phi + kPi);
+    const Spline2D::Value* t2 = c + 12;
+    s[3] = t2_site(t2, t, phi, pis6, 5, false);
+    s[6] = t2_site(t2, t, phi - pi2s3, pis6, 5, false);
+    s[7] = t2_site(t2, t, phi + pi2s3, pis6, 5, false);
+    s[9] = t2_site(t2, t, phi + kPi, pis6, 5, false);
+    s[10] = t2_site(t2, t, phi + pis3, pis6, 5, false);
+    s[11] = t2_site(t2, t, phi - pis3, pis6, 5, false);
+  } else {
+    s[2] = hollow_site(c + 7, t, phi);
+    const Spline2D::Value* t2h = c + 12;
+    s[3] = t2_site(t2h, t, phi, pis6, 4, false);
+    s[6] = t2_site(t2h, t, phi - pi2s3, pis6, 4, false);
+    s[7] = t2_site(t2h, t, phi + pi2s3, pis6, 4, false);
+    const Spline2D::Value* t2f = c + 18;
+    s[10] = t2_site(t2f, t, phi, 11.00 * pis6, 4, true);
+    s[9] = t2_site(t2f, t, phi + pi2s3, 11.00 * pis6, 4, true);
+    s[11] = t2_site(t2f, t, phi - pi2s3, 11.00 * pis6, 4, true);
+    s[8] = hollow_site(c + 24, t, phi);
+  }
+  std::array<double, 12> f{}, dfdx{}, dfdy{};
+  molecule_basis(x, y, a_, f.data(), dfdx.data(), dfdy.data());
+  std::array<double, 7> out{};
+  for (int j = 0; j < 12; ++j) {
+    double a = 0.00;
+    for (int k = 0; k < 12; ++k) a = a + kMoleculeTable[k + 12 * j] * s[k].v[0];
+    out[0] = out[0] + a * f[j];
+    out[5] = out[5] + a * dfdx[j];
+    out[6] = out[6] + a * dfdy[j];
+  }
+  for (int i = 1; i < 5; ++i) {
+    for (int j = 0; j < 12; ++j) {
+      double a = 0.00;
+      for (int k = 0; k < 12; ++k) {
+        const double comp = i == 1 ? s[k].v[1] : (i == 2 ? s[k].v[2] : (i == 3 ? s[k].d_phi : s[k].d_theta));
+        a = a + kMoleculeTable[k + 12 * j] * comp;
+      }
+      out[i] = out[i] + a * f[j];
+    }
+  }
+  return out;
+}
+
+// POT6D -> CAMBCORD -> RRINTII (src/crplib/molecule_surface.cpp:249-315)
+std::optional<MoleculeValue> MoleculeSurface::evaluate(const Vec3& a, const Vec3& b) const {
+  const Geometry g = molecule_geometry(a, b);
+  if (!(g.z < zso_max_)) {  // far from the surface: the gas-phase potential only
+    if (g.r < h2_.x().front() || g.r > h2_.x().back()) return std::nullopt;
+    const GasPhase gp = gas_phase(g.z, g.r);
+    const double rxa = -(b.x - a.x) / g.r, rya = -(b.y - a.y) / g.r, rza = -(b.z - a.z) / g.r;
+    return MoleculeValue{gp.v,
+                         {rxa * gp.d_r, rya * gp.d_r, 0.500 * gp.d_z + rza * gp.d_r},
+                         {-rxa * gp.d_r, -rya * gp.d_r, 0.500 * gp.d_z + -rza * gp.d_r}};
+  }
+  const int n = molecule_cuts_used(symmetry_);
+  std::array<Spline2D::Value, 29> cv{};
+  for (int k = 0; k < n; ++k) {
+    const auto cell = cuts_[k].locate(g.r, g.z);
+    if (!cell) return std::nullopt;  // outside a cut's grid the Fortran gives up silently
+    cv[k] = cuts_[k].at(*cell);
+  }
+  const std::array<double, 7> vi = interaction(cv.data(), g.theta, g.phi, g.x, g.y);
+  // derivatives of r, theta, phi with respect to atom a (RRINTII)
+  const double rxa = -(b.x - a.x) / g.r, rya = -(b.y - a.y) / g.r, rza = -(b.z - a.z) / g.r;
+  double txa = 0.0, tya = 0.0, pxa = 0.0, pya = 0.0;
+  const double rho2 = (b.x - a.x) * (b.x - a.x) + (b.y - a.y) * (b.y - a.y);
+  const double rho = std::sqrt(rho2);
+  if (rho2 > 0.00) {
+    txa = -(b.z - a.z) * (b.x - a.x) / (g.r * g.r * rho);
+    tya = -(b.z - a.z) * (b.y - a.y) / (g.r * g.r * rho);
+    pxa = (b.y - a.y) / rho2;
+    pya = -(b.x - a.x) / rho2;
+  }
+  const double tza = rho / (g.r * g.r);
+  const AtomValue va = atom_.evaluate(a);
+  const AtomValue vb = atom_.evaluate(b);
+  const double iv = vi[0], iz = vi[1], ir = vi[2], iphi = vi[3], itheta = vi[4], ix = vi[5], iy = vi[6];
+  const Vec3 fa{va.gradient.x + 0.500 * ix + rxa * ir + txa * itheta + pxa * iphi,
+                va.gradient.y + 0.500 * iy + rya * ir + tya * itheta + pya * iphi,
+                va.gradient.z + 0.500 * iz + rza * ir + tza * itheta};
+  const Vec3 fb{vb.gradient.x + 0.500 * ix + (-rxa) * ir + (-txa) * itheta + (-pxa) * iphi,
+                vb.gradient.y + 0.500 * iy + (-rya) * ir + (-tya) * itheta + (-pya) * iphi,
+                vb.gradient.z + 0.500 * iz + (-rza) * ir + (-tza) * itheta};
+  if (g.z <= zso_min_) return MoleculeValue{va.v + vb.v + iv, fa, fb};
+  const Switch sw = switch_function(g.z);  // switching region: (Va + Vb + I) F + (1 - F) V2D
+  const GasPhase gp = gas_phase(g.z, g.r);
+  const double dfz = 0.500 * sw.df_dz;
+  const double near = va.v + vb.v + iv;
+  return MoleculeValue{near * sw.f + (1.00 - sw.f) * gp.v,
+                       {fa.x * sw.f + (1.00 - sw.f) * gp.d_r * rxa, fa.y * sw.f + (1.00 - sw.f) * gp.d_r * rya,
+                        fa.z * sw.f + near * dfz - dfz * gp.v + (1.00 - sw.f) * (gp.d_z * 0.500 + rza * gp.d_r)},
+                       {fb.x * sw.f + (1.00 - sw.f) * gp.d_r * (-rxa), fb.y * sw.f + (1.00 - sw.f) * gp.d_r * (-rya),
+                        fb.z * sw.f + near * dfz - dfz * gp.v +
+                            (1.00 - sw.f) * (gp.d_z * 0.500 + (-rza) * gp.d_r)}};
+}
+
+// ---------------------------------------------------------------------------------------------------------------
+// QD wrapper (src/crplib/qd_wrapper.cpp:20-66)
+
+QdSurface::QdSurface(const DataSet& data) : surface_(data) {}
+
+double QdSurface::skewing_angle() const { return kPi / 3.0; }
+
+double QdSurface::potential_ev(double u, double v, double z, double r, double theta, double phi) const {
+  constexpr double delta = 1.0e-12;  // keeps the clamped values inside the data
+  double rr = r, zz = z;
+  if (rr < surface_.r_min() + delta) {
+    rr = surface_.r_min() + delta;
+  } else if (rr > surface_.r_max() - delta) {
+    rr = surface_.r_max() - delta;
+  }
+  if (zz < surface_.z_min() + delta) zz = surface_.z_min() + delta;
+  const double x = 1.0 * u + kCosPi3 * v;
+  const double y = 0.0 * u + kSinPi3 * v;
+  const Vec3 a{x - 0.5 * rr * std::sin(theta) * std::cos(phi), y - 0.5 * rr * std::sin(theta) * std::sin(phi),
+               zz - 0.5 * rr * std::cos(theta)};
+  const Vec3 b{x + 0.5 * rr * std::sin(theta) * std::cos(phi), y + 0.5 * rr * std::sin(theta) * std::sin(phi),
+               zz + 0.5 * rr * std::cos(theta)};
+  const std::optional<MoleculeValue> m = surface_.evaluate(a, b);
+  const double value = to_energy(m ? m->v : 0.00, EnergyUnit::hartree, EnergyUnit::ev);
+  return value - to_energy(surface_.h2_asymptote(), EnergyUnit::hartree, EnergyUnit::ev);
+}
+
+}  // namespace spo::crp

@@ -1,5 +1,8 @@
 #include "potential.hpp"
 
+#include <map>
+#include <mutex>
+
 #include <cmath>
 #include <stdexcept>
 #include <string>
@@ -86,9 +89,38 @@ double DaiLightPes::value(double x, double y, double z, double r, double theta, 
   return Units::hartree_to_ev(v);
 }
 
-std::unique_ptr<Pes> make_pes(const std::string& id) {
+double CrpPes::value(double x, double y, double z, double r, double theta, double phi) const {
+  try {
+    return surface_->potential_ev(x, y, z, r, theta, phi);
+  } catch (const crp::EvaluationError& e) {
+    throw InputError("CRPlibDataSet", std::string("the CRP PES cannot be evaluated on this grid: ") + e.what());
+  }
+}
+
+std::shared_ptr<const crp::QdSurface> load_crp_surface(const std::string& directory) {
+  static std::mutex lock;
+  static std::map<std::string, std::shared_ptr<const crp::QdSurface>> loaded;
+  if (directory.empty()) throw InputError("CRPlibDataSet", "SystemID CRPlibrary needs CRPlibDataSet (the directory of the CRP data set)");
+  const std::lock_guard<std::mutex> guard(lock);
+  auto& slot = loaded[directory];
+  if (!slot) {
+    try {
+      slot = std::make_shared<const crp::QdSurface>(crp::read_data_set(directory));
+    } catch (const crp::DataError& e) {
+      loaded.erase(directory);
+      throw InputError("CRPlibDataSet", e.what());
+    } catch (const crp::EvaluationError& e) {
+      loaded.erase(directory);
+      throw InputError("CRPlibDataSet", e.what());
+    }
+  }
+  return slot;
+}
+
+std::unique_ptr<Pes> make_pes(const std::string& id, const std::string& crp_data_set) {
   if (id == "HardWallPES") return std::make_unique<HardWallPes>();
   if (id == "H2Cu111DaiLight") return std::make_unique<DaiLightPes>();
+  if (id == "CRPlibrary") return std::make_unique<CrpPes>(load_crp_surface(crp_data_set));
   throw NotSupportedError("SystemID " + id + " is not supported");
 }
 

@@ -7,6 +7,7 @@
 #include <vector>
 
 #include "config.hpp"
+#include "crp.hpp"
 #include "grids.hpp"
 #include "symmetry.hpp"
 
@@ -34,7 +35,22 @@ class DaiLightPes final : public Pes {
   double value(double x, double y, double z, double r, double theta, double phi) const override;
 };
 
-std::unique_ptr<Pes> make_pes(const std::string& system_id);
+// The CRP surface of a data set (SystemID CRPlibrary, src/potential/crp_potential.cpp): the QD wrapper of the CRP
+// library, V in eV at skewed (x, y).
+class CrpPes final : public Pes {
+ public:
+  explicit CrpPes(std::shared_ptr<const crp::QdSurface> surface) : surface_(std::move(surface)) {}
+  double value(double x, double y, double z, double r, double theta, double phi) const override;
+
+ private:
+  std::shared_ptr<const crp::QdSurface> surface_;
+};
+
+// The surface of a CRP data set directory, read once per process and shared; InputError naming CRPlibDataSet for
+// a missing or unusable data set.
+std::shared_ptr<const crp::QdSurface> load_crp_surface(const std::string& directory);
+
+std::unique_ptr<Pes> make_pes(const std::string& system_id, const std::string& crp_data_set = "");
 
 // V in hartree on (itp, irep, izr), itp = iphi + nphi * itheta, izr = ir + nr * iz < nz_potential * nr, and
 // the gas-phase V(r) (src/potential/potential_grid.hpp:14-31).

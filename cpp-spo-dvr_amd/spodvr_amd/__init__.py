@@ -95,6 +95,15 @@ def host_library() -> ctypes.CDLL:
         lib.spo_run_kernel_timing.argtypes = [ctypes.c_void_p, ctypes.c_int]
         lib.spo_run_kernel_times.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_void_p, ctypes.c_void_p,
                                              ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
+        lib.spo_crp_open.argtypes = [ctypes.c_char_p, ctypes.POINTER(ctypes.c_void_p)]
+        lib.spo_crp_close.argtypes = [ctypes.c_void_p]
+        for fn in ("spo_crp_molecule", "spo_crp_atom"):
+            getattr(lib, fn).argtypes = [ctypes.c_void_p, ctypes.c_long, ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
+                                         ctypes.c_void_p]
+        for fn in ("spo_crp_qd", "spo_crp_gas_phase"):
+            getattr(lib, fn).argtypes = [ctypes.c_void_p, ctypes.c_long, ctypes.c_void_p, ctypes.c_void_p]
+        lib.spo_crp_geometry.argtypes = [ctypes.c_long, ctypes.c_void_p, ctypes.c_void_p]
+        lib.spo_crp_parameters.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
         _host = lib
     return _host
 
@@ -131,6 +140,79 @@ def device_pes_values(system_id: str, points: np.ndarray) -> np.ndarray:
     if status != 0:
         raise SpoError(status, lib.spo_hip_last_error().decode(errors="replace"))
     return out
+
+
+# Unit codes of the CRP library (src/crplib/units.hpp:11-17)
+CRP_BOHR, CRP_ANGSTROM = 0, 1
+CRP_HARTREE, CRP_EV, CRP_KCAL_PER_MOL, CRP_KJ_PER_MOL, CRP_JOULE = 2, 3, 4, 5, 6
+
+
+def _points(a, width: int) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.float64).reshape(-1, width)
+
+
+class CrpSurface:
+    """The CRP potential of a C3v or C6v data set directory (SystemID CRPlibrary), evaluated by libspo_host.so.
+
+    molecule(atoms (n, 6) = xa ya za xb yb zb) -> (n, 7) V and its gradient on both atoms; atom(positions (n, 3))
+    -> (n, 4); qd(points (n, 6) = u v Z r theta phi) -> V in eV as the propagation uses it; gas_phase(zr (n, 2)) ->
+    (n, 5) F, dF/dZ, V2D, dV2D/dZ, dV2D/dr; parameters() -> dict."""
+
+    def __init__(self, directory: str):
+        lib = host_library()
+        handle = ctypes.c_void_p()
+        _check(lib.spo_crp_open(directory.encode(), ctypes.byref(handle)))
+        self._handle = handle
+
+    def close(self) -> None:
+        if self._handle:
+            host_library().spo_crp_close(self._handle)
+            self._handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def molecule(self, atoms, length_unit: int = CRP_BOHR, energy_unit: int = CRP_HARTREE) -> np.ndarray:
+        pts = _points(atoms, 6)
+        out = np.empty((len(pts), 7))
+        _check(host_library().spo_crp_molecule(self._handle, len(pts), _ptr(pts), length_unit, energy_unit, _ptr(out)))
+        return out
+
+    def atom(self, positions, length_unit: int = CRP_BOHR, energy_unit: int = CRP_HARTREE) -> np.ndarray:
+        pts = _points(positions, 3)
+        out = np.empty((len(pts), 4))
+        _check(host_library().spo_crp_atom(self._handle, len(pts), _ptr(pts), length_unit, energy_unit, _ptr(out)))
+        return out
+
+    def qd(self, points) -> np.ndarray:
+        pts = _points(points, 6)
+        out = np.empty(len(pts))
+        _check(host_library().spo_crp_qd(self._handle, len(pts), _ptr(pts), _ptr(out)))
+        return out
+
+    def gas_phase(self, zr) -> np.ndarray:
+        pts = _points(zr, 2)
+        out = np.empty((len(pts), 5))
+        _check(host_library().spo_crp_gas_phase(self._handle, len(pts), _ptr(pts), _ptr(out)))
+        return out
+
+    @staticmethod
+    def geometry(atoms) -> np.ndarray:
+        pts = _points(atoms, 6)
+        out = np.empty((len(pts), 6))
+        _check(host_library().spo_crp_geometry(len(pts), _ptr(pts), _ptr(out)))
+        return out
+
+    def parameters(self) -> dict:
+        out = np.empty(9)
+        _check(host_library().spo_crp_parameters(self._handle, _ptr(out)))
+        keys = ("lattice_constant", "z_asymptotic", "skewing_angle", "zso_min", "zso_max", "r_min", "r_max", "z_min")
+        d = dict(zip(keys, (float(x) for x in out[:8])))
+        d["point_group"] = "C3v" if out[8] == 3.0 else "C6v"
+        return d
 
 
 class Run:

@@ -117,6 +117,28 @@ int spo_run_kernel_times(spo_run* run, char* names, long* launches, double* tota
 int spo_pes_value(const char* system_id, double x, double y, double z, double r, double theta, double phi,
                   double* out);
 
+/* The CRP potential of a data set directory (SystemID CRPlibrary; the reference's CRP library, src/crplib/, as
+ * src/potential/crp_potential.cpp uses it). C3v and C6v data sets; SPO_E_INPUT for a missing or unusable one.
+ * Points are evaluated in parallel; every function takes n points and writes n records. */
+typedef struct spo_crp spo_crp;
+int spo_crp_open(const char* directory, spo_crp** out);
+int spo_crp_close(spo_crp* crp);
+/* length_unit 0 bohr / 1 angstrom, energy_unit 2 hartree / 3 eV / 4 kcal/mol / 5 kJ/mol / 6 J
+ * (src/crplib/units.hpp:11-17). atoms[6k..] = xa ya za xb yb zb (Cartesian); out[7k..] = V, dV/dxa dya dza dxb dyb
+ * dzb, all 0 where the surface gives up silently (src/crplib/wrapper.cpp, Calculate_CRP_MoleculeSurface_PES). */
+int spo_crp_molecule(const spo_crp* crp, long n, const double* atoms, int length_unit, int energy_unit, double* out);
+/* positions[3k..] = x y z; out[4k..] = V, dV/dx dy dz (Calculate_CRP_AtomSurface_PES). */
+int spo_crp_atom(const spo_crp* crp, long n, const double* positions, int length_unit, int energy_unit, double* out);
+/* points[6k..] = u v Z r theta phi (skewed bohr, rad); out[k] = V in eV (src/crplib/qd_wrapper.cpp:46-66). */
+int spo_crp_qd(const spo_crp* crp, long n, const double* points, double* out);
+/* zr[2k..] = Z r (bohr); out[5k..] = switch F, dF/dZ, V2D, dV2D/dZ, dV2D/dr (hartree). */
+int spo_crp_gas_phase(const spo_crp* crp, long n, const double* zr, double* out);
+/* atoms as for spo_crp_molecule (bohr); out[6k..] = centre x y Z, r, theta, phi (CAMBCORD). */
+int spo_crp_geometry(long n, const double* atoms, double* out);
+/* out[0..8] = lattice constant, Zasy, skewing angle, ZsoMin, ZsoMax, Rmin, Rmax, Zmin (bohr, rad), and 3 or 6 for
+ * C3v or C6v. */
+int spo_crp_parameters(const spo_crp* crp, double* out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -2,6 +2,7 @@
 //
 // Stockham autosort over `batch` interleaved sequences. The work arrays hold real and imaginary parts apart
 // (element i of sequence b at [i * batch + b]), so the innermost loop over b is unit-stride and vectorises.
+#include <string>
 #include "oracle_fft.hpp"
 
 #include <cmath>
@@ -131,7 +132,7 @@ Fft1d::Fft1d(int n) : n_(n) {
       radices_.push_back(p);
       m /= p;
     }
-  if (m != 1) throw std::invalid_argument("Fft1d: length with a prime factor above 13");
+  if (m != 1) throw std::invalid_argument("Fft1d: length " + std::to_string(n) + " has a prime factor above 13");
   // Per stage: exp(-2 pi i r k / (span R)) for k < span, r = 1..R-1; then the roots of each radix.
   int span = 1;
   for (const int radix : radices_) {

@@ -36,6 +36,8 @@ CASES = ["hardwall_short", "cu111_dailight_short", "cu111_dailight_short_sym", "
 BK_INDICES = 96          # coefficient indices kept per case
 FORT25_BLOCKS = 5        # energy blocks kept per case (evenly spread)
 PES_POINTS = 1500        # subsample of the PES fixture
+# set-up only cases (the PES and the gas-phase states; no propagation output)
+SETUP_CASES = ["crp_cu111_srp48_small"]
 
 
 def fortran_records(path: str) -> list[bytes]:
@@ -162,6 +164,14 @@ def main() -> None:
         extract_fort21_header(src, dst)
         extract_fort25(src, dst)
         extract_stdout(src, dst)
+        extract_gas(src, dst)
+        extract_pes(src, dst)
+        print(case, sorted(os.listdir(dst)))
+    for case in SETUP_CASES:
+        src = os.path.join(REFERENCE, case)
+        dst = os.path.join(HERE, case)
+        os.makedirs(dst, exist_ok=True)
+        shutil.copyfile(os.path.join(src, "input.json"), os.path.join(dst, "input.json"))
         extract_gas(src, dst)
         extract_pes(src, dst)
         print(case, sorted(os.listdir(dst)))
