@@ -7,6 +7,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include "kernels_crp.hpp"
+
 namespace spo_dev {
 
 constexpr double kPesPi = 3.1415926535897932384626433832795028841971;
@@ -117,6 +119,7 @@ struct PesGrid {
   const double* gas_phase;  // nr values, hartree: the switch's asymptote
   int impose_v_min, impose_v_max;
   double v_min, v_max;
+  CrpDev crp;  // pes == 3 (CRP) only
 };
 
 // One value per thread, in the table's own order: index = iphi + n_phi (itheta + n_theta (irep + n_rep izr)).
@@ -133,7 +136,9 @@ __global__ __launch_bounds__(256) void potential_grid_kernel(PesGrid g, double* 
   const int ix = ixy / g.ny, iy = ixy % g.ny;
   const double z = g.z[iz];
   // eV -> hartree by a multiplication with 1 / 27.2114, as the reference's grid builder does
-  double v = pes_value(g.pes, g.x[ix], g.y[iy], z, g.r[ir], g.theta[it], g.phi[ip]) * (1.0 / kHartreeInEv);
+  const double ev = g.pes == 3 ? crp_qd(g.crp, g.x[ix], g.y[iy], z, g.r[ir], g.theta[it], g.phi[ip])
+                               : pes_value(g.pes, g.x[ix], g.y[iy], z, g.r[ir], g.theta[it], g.phi[ip]);
+  double v = ev * (1.0 / kHartreeInEv);
   if (g.use_switching && tol_gt(z, g.z_min_switch)) {
     const double va = g.gas_phase[ir];
     if (tol_ge(z, g.z_max_switch)) {

@@ -7,6 +7,7 @@
 // Balint-Kurti coefficients and the S-matrix sums are FP64 in both modes.
 #include <hip/hip_runtime.h>
 
+
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
@@ -14,6 +15,7 @@
 #include <cstring>
 #include <map>
 #include <memory>
+#include <optional>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -1283,12 +1285,85 @@ int spo_hip_pes_values(int pes, long n, const double* points, double* values) {
   });
 }
 
+}  // extern "C"
+
+namespace {
+
+// A packed CRP surface checked against its own offsets and uploaded; dev() gives the kernels' view.
+struct CrpUpload {
+  Buffer corrugation, cluster, atom_offsets, atom_splines, cut_shape, cut_offsets, cut_data, h2, zdep;
+  spo_dev::CrpDev view{};
+
+  explicit CrpUpload(const spo_hip_crp_surface& s) {
+    const int n_basis = s.c3v ? 10 : 7, n_cuts = s.c3v ? 29 : 18;
+    auto bad = [](const char* what) { throw Status(SPO_HIP_E_ARGUMENT, std::string("CRP surface: ") + what); };
+    if (s.n_cuts != n_cuts) bad("the number of molecular cuts does not match the symmetry");
+    if (s.n_corrugation < 4 || s.n_cluster < 1 || s.n_h2 < 4 || s.n_z_dependence < 4) bad("too few points");
+    if (!s.corrugation || !s.cluster || !s.atom_offsets || !s.atom_splines || !s.cut_shape || !s.cut_offsets ||
+        !s.cut_data || !s.h2 || !s.z_dependence)
+      bad("NULL array");
+    if (s.atom_offsets[0] != 0) bad("atom offsets");
+    for (int i = 0; i < n_basis; ++i)
+      if (s.atom_offsets[i + 1] - s.atom_offsets[i] < 4) bad("an atom cut with fewer than 4 knots");
+    if (s.cut_offsets[0] != 0) bad("cut offsets");
+    for (int k = 0; k < n_cuts; ++k) {
+      const long nr = s.cut_shape[3 * k], nz = s.cut_shape[3 * k + 1], g = s.cut_shape[3 * k + 2];
+      if (nr < 4 || nz < 4 || g < 0 || g > k) bad("cut shape");
+      if (g != k && (s.cut_shape[3 * g] != nr || s.cut_shape[3 * g + 1] != nz)) bad("cut group of another shape");
+      if (s.cut_offsets[k + 1] - s.cut_offsets[k] != nr + nz + 4 * nr * nz) bad("cut block size");
+    }
+    corrugation = upload(s.corrugation, 3 * std::size_t(s.n_corrugation) * sizeof(double));
+    cluster = upload(s.cluster, 4 * std::size_t(s.n_cluster) * sizeof(double));
+    atom_offsets = upload(s.atom_offsets, (n_basis + 1) * sizeof(int));
+    atom_splines = upload(s.atom_splines, 3 * std::size_t(s.atom_offsets[n_basis]) * sizeof(double));
+    cut_shape = upload(s.cut_shape, 3 * std::size_t(n_cuts) * sizeof(int));
+    cut_offsets = upload(s.cut_offsets, (n_cuts + 1) * sizeof(long));
+    cut_data = upload(s.cut_data, std::size_t(s.cut_offsets[n_cuts]) * sizeof(double));
+    h2 = upload(s.h2, 3 * std::size_t(s.n_h2) * sizeof(double));
+    zdep = upload(s.z_dependence, 3 * std::size_t(s.n_z_dependence) * sizeof(double));
+    view = spo_dev::CrpDev{s.c3v, n_basis, n_cuts, s.n_corrugation, s.n_cluster, s.n_h2, s.n_z_dependence,
+                           s.lattice_constant, s.zso_min, s.zso_max, s.r_min, s.r_max, s.z_min, s.h2_asymptote,
+                           corrugation.as<double>(), cluster.as<double>(), atom_offsets.as<int>(),
+                           atom_splines.as<double>(), cut_shape.as<int>(), cut_offsets.as<long>(),
+                           cut_data.as<double>(), h2.as<double>(), zdep.as<double>()};
+  }
+};
+
+void check_finite(const double* v, long n, const char* where) {
+  for (long i = 0; i < n; ++i)
+    if (std::isnan(v[i]))
+      throw Status(SPO_HIP_E_ARGUMENT, std::string(where) + ": the CRP surface cannot be evaluated at point " +
+                                           std::to_string(i) + " (an atom on a surface atom or outside the data)");
+}
+
+}  // namespace
+
+extern "C" {
+
+int spo_hip_crp_values(const spo_hip_crp_surface* surface, long n, const double* points, double* values) {
+  return guarded([&] {
+    if (!surface) throw Status(SPO_HIP_E_ARGUMENT, "spo_hip_crp_values: NULL surface");
+    if (n < 0 || (n > 0 && (!points || !values))) throw Status(SPO_HIP_E_ARGUMENT, "spo_hip_crp_values: bad arrays");
+    if (n == 0) return;
+    require_device();
+    CrpUpload crp(*surface);
+    Buffer p = upload(points, std::size_t(n) * 6 * sizeof(double));
+    Buffer out(std::size_t(n) * sizeof(double));
+    hipLaunchKernelGGL(spo_dev::crp_points_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, nullptr, crp.view, n,
+                       p.as<double>(), out.as<double>());
+    HIP_CHECK(hipGetLastError());
+    HIP_CHECK(hipMemcpy(values, out.as<void>(), std::size_t(n) * sizeof(double), hipMemcpyDeviceToHost));
+    check_finite(values, n, "spo_hip_crp_values");
+  });
+}
+
 int spo_hip_potential_grid(const spo_hip_pes_grid* grid, double* values) {
   return guarded([&] {
     if (!grid || !values) throw Status(SPO_HIP_E_ARGUMENT, "spo_hip_potential_grid: NULL argument");
     const spo_hip_pes_grid& g = *grid;
-    if (g.pes != SPO_HIP_PES_HARDWALL && g.pes != SPO_HIP_PES_DAILIGHT)
+    if (g.pes != SPO_HIP_PES_HARDWALL && g.pes != SPO_HIP_PES_DAILIGHT && g.pes != SPO_HIP_PES_CRP)
       throw Status(SPO_HIP_E_ARGUMENT, "spo_hip_potential_grid: unknown surface " + std::to_string(g.pes));
+    if (g.pes == SPO_HIP_PES_CRP && !g.crp) throw Status(SPO_HIP_E_ARGUMENT, "spo_hip_potential_grid: no CRP surface");
     if (g.nx <= 0 || g.ny <= 0 || g.nr <= 0 || g.nz_potential < 0 || g.n_theta <= 0 || g.n_phi <= 0 ||
         g.n_representatives <= 0 || g.n_representatives > g.nx * g.ny)
       throw Status(SPO_HIP_E_ARGUMENT, "spo_hip_potential_grid: bad extents");
@@ -1309,15 +1384,18 @@ int spo_hip_potential_grid(const spo_hip_pes_grid* grid, double* values) {
     Buffer reps = up(g.representatives, g.n_representatives * sizeof(int));
     Buffer gas = g.use_switching ? up(g.gas_phase, g.nr * sizeof(double)) : Buffer();
     Buffer out(std::size_t(n) * sizeof(double));
+    std::optional<CrpUpload> crp;
+    if (g.pes == SPO_HIP_PES_CRP) crp.emplace(*g.crp);
     spo_dev::PesGrid d{g.pes, g.ny, g.nr, g.n_theta, g.n_phi, g.n_representatives, static_cast<int>(n_zr),
                        x.as<double>(), y.as<double>(), z.as<double>(), r.as<double>(), th.as<double>(),
                        ph.as<double>(), reps.as<int>(), g.use_switching, g.z_min_switch, g.z_max_switch,
                        g.use_switching ? gas.as<double>() : nullptr, g.impose_v_min, g.impose_v_max, g.v_min,
-                       g.v_max};
+                       g.v_max, crp ? crp->view : spo_dev::CrpDev{}};
     hipLaunchKernelGGL(spo_dev::potential_grid_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, nullptr, d,
                        out.as<double>());
     HIP_CHECK(hipGetLastError());
     HIP_CHECK(hipMemcpy(values, out.as<void>(), std::size_t(n) * sizeof(double), hipMemcpyDeviceToHost));
+    if (g.pes == SPO_HIP_PES_CRP) check_finite(values, n, "spo_hip_potential_grid");
   });
 }
 

@@ -10,6 +10,7 @@
 #include "driver.hpp"
 #include "json.hpp"
 #include "potential.hpp"
+#include "../../include/spo_hip.h"
 
 struct spo_crp {
   std::shared_ptr<const spo::crp::QdSurface> qd;
@@ -462,6 +463,15 @@ int spo_crp_geometry(long n, const double* atoms, double* out) {
       o[4] = g.theta;
       o[5] = g.phi;
     }
+  });
+}
+
+int spo_crp_device_values(const spo_crp* crp, long n, const double* points, double* out) {
+  return guarded([&] {
+    const spo::crp::Packed packed = spo::crp::pack(*crp->qd);
+    const spo_hip_crp_surface view = spo::crp_surface_view(packed);
+    if (spo_hip_crp_values(&view, n, points, out) != SPO_HIP_OK)
+      throw std::runtime_error(std::string("device CRP values: ") + spo_hip_last_error());
   });
 }
 

@@ -2,6 +2,7 @@
 // Fortran CRP library as the reference's restatement keeps it (file:line of src/crplib/ at each part), so that the
 // results agree with the Fortran's bit for bit; integer powers are written as products.
 #include "crp.hpp"
+#include "crp_tables.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -14,14 +15,14 @@ namespace spo::crp {
 
 namespace {
 
-// pi as CRP_Constants_PI and sqrt(3) as the Fortran literal (src/crplib/constants.hpp:8-14).
-constexpr double kPi = 3.1415926535897932384626433832795028841971;
-constexpr double kSq3 = 1.732050807568877293500;
-constexpr double kUsq3 = 1.0 / kSq3;
-// cos and sin of pi/3 as gfortran folds them at compile time (bits 0x3FE0000000000001 and 0x3FEBB67AE8584CAA;
-// src/crplib/constants.hpp:16-22): the skewing transform of the QD wrapper and the C3v cluster use these values.
-constexpr double kCosPi3 = 0x1.0000000000001p-1;
-constexpr double kSinPi3 = 0x1.bb67ae8584caap-1;
+using crp_tables::kCosPi3;
+using crp_tables::kPi;
+using crp_tables::kSinPi3;
+using crp_tables::kSq3;
+using crp_tables::kUsq3;
+using crp_tables::kAtomC3vTable;
+using crp_tables::kAtomC6vTable;
+using crp_tables::kMoleculeTable;
 
 // ---------------------------------------------------------------------------------------------------------------
 // Units (src/crplib/units.cpp:13-79)
@@ -592,62 +593,6 @@ Spline2D::Value Spline2D::at(const Cell& c) const {
 
 namespace {
 
-// Fourier tables of the interpolation over the sites, element (site, j) at site + n j (src/crplib/fourier_tables.cpp;
-// Fortran DATA statements of FOUR3D and FOUR6D, with their scale factors). Integer weights written out.
-constexpr std::array<double, 49> kAtomC6vWeights = {
-    1.0, 3.0,  2.0,  6.0,  12.0, 6.0,  6.0,  2.0, -2.0, -2.0, 2.0,  -8.0, 8.0,  0.0, 2.0, -2.0, 4.0,
-    -4.0, 4.0, 2.0,  -6.0, 2.0,  6.0,  -2.0, -6.0, 0.0, 0.0, 0.0, 2.0,  -2.0, -2.0, 2.0, 4.0, -4.0,
-    0.0, 1.0,  -1.0, 2.0,  -2.0, -4.0, -2.0, 6.0, 0.666666666666667, 2.0, 1.333333333333, 4.0, -4.0, -2.0, -2.0};
-
-constexpr double kA = 5.1961524227066318805600;
-constexpr double kB = 20.7846096908265275222400;
-constexpr double kC = 15.5884572681198956416800;
-constexpr double kD = 10.3923048454132637611200;
-constexpr std::array<double, 100> kAtomC3vWeights = {
-    3.00, 9.00,  3.00,  9.00,  18.00,  18.00,  18.00,  9.00,  18.00,  3.00,    //
-    6.00, -6.00, -3.00, 3.00,  -12.00, 24.00,  0.00,   3.00,  -12.00, -3.00,   //
-    0.00, 0.00,  -kA,   -kA,   -kB,    0.00,   0.00,   kA,    kB,     kA,      //
-    6.00, -6.00, 6.00,  -6.00, 6.00,   6.00,   -18.00, -6.00, 6.00,   6.00,    //
-    6.00, 18.00, -3.00, -9.00, 0.00,   0.00,   0.00,   -9.00, 0.00,   -3.00,   //
-    0.00, 0.00,  kA,    -kC,   0.00,   0.00,   0.00,   kC,    0.00,   -kA,     //
-    6.00, -6.00, -3.00, 3.00,  6.00,   -12.00, 0.00,   3.00,  6.00,   -3.00,   //
-    0.00, 0.00,  -kA,   -kA,   kD,     0.00,   0.00,   kA,    -kD,    kA,      //
-    3.00, -3.00, 3.00,  -3.00, -6.00,  -6.00,  18.00,  -3.00, -6.00,  3.00,    //
-    2.00, 6.00,  2.00,  6.00,  -6.00,  -6.00,  -6.00,  6.00,  -6.00,  2.00};
-
-constexpr std::array<double, 144> kMoleculeWeights = {
-    1, 1,  1,  1,  1,  1,  1,  1,  1,  1,  1,  1,  2, -2, -1, 1,  -2, 2,  -1, 1,  -1, 1,  -1, 1,
-    2, 2,  -1, 1,  -2, -2, 1,  -1, -1, 1,  1,  -1, 2, -2, -1, -1, 2,  -2, 1,  1,  -1, -1, 1,  1,
-    0, 0,  1,  1,  0,  0,  -1, 1,  -1, -1, 1,  -1, 0, 0,  1,  1,  0,  0,  1,  -1, -1, -1, -1, 1,
-    0, 0,  -1, 1,  0,  0,  -1, -1, 1,  -1, 1,  1,  1, 1,  1,  -1, -1, -1, -1, 1,  1,  -1, -1, 1,
-    1, -1, 1,  -1, -1, 1,  1,  -1, 1,  -1, 1,  -1, 1, -1, 1,  1,  1,  -1, -1, -1, 1,  1,  -1, -1,
-    2, 2,  -1, -1, 2,  2,  -1, -1, -1, -1, -1, -1, 0, 0,  1,  -1, 0,  0,  -1, -1, -1, 1,  1,  1};
-
-template <std::size_t N>
-constexpr std::array<double, N> scaled(const std::array<double, N>& w, double divisor) {
-  std::array<double, N> t{};
-  for (std::size_t i = 0; i < N; ++i) t[i] = w[i] / divisor;
-  return t;
-}
-
-constexpr std::array<double, 144> molecule_table() {
-  constexpr double r1 = kUsq3 / 4.00;
-  constexpr double r2 = 1.00 / 36.00;
-  std::array<double, 144> t{};
-  for (int j = 0; j < 12; ++j) {
-    double factor = 3.0 * r2;
-    if (j >= 4 && j <= 6) factor = r1;
-    if (j == 10) factor = r2;
-    if (j == 11) factor = r1 / 3.0;
-    for (int i = 0; i < 12; ++i) t[i + 12 * j] = factor * kMoleculeWeights[i + 12 * j];
-  }
-  return t;
-}
-
-constexpr std::array<double, 49> kAtomC6vTable = scaled(kAtomC6vWeights, 36.00);
-constexpr std::array<double, 100> kAtomC3vTable = scaled(kAtomC3vWeights, 108.00);
-constexpr std::array<double, 144> kMoleculeTable = molecule_table();
-
 // The C6v cluster: crystal coordinates of 30 top-layer atoms (Fortran DATA ivec of MURII).
 constexpr std::array<int, 30> kC6vI = {0,  1,  2, 3, -1, 0, 1,  2,  3, -2, -1, 0,  1,  2, 3,
                                        -2, -1, 0, 1, 2,  3, -2, -1, 0, 1,  2,  -2, -1, 0, 1};
@@ -1215,6 +1160,50 @@ std::optional<MoleculeValue> MoleculeSurface::evaluate(const Vec3& a, const Vec3
                        {fb.x * sw.f + (1.00 - sw.f) * gp.d_r * (-rxa), fb.y * sw.f + (1.00 - sw.f) * gp.d_r * (-rya),
                         fb.z * sw.f + near * dfz - dfz * gp.v +
                             (1.00 - sw.f) * (gp.d_z * 0.500 + (-rza) * gp.d_r)}};
+}
+
+// ---------------------------------------------------------------------------------------------------------------
+// Flat arrays for the device
+
+Packed pack(const QdSurface& q) {
+  const MoleculeSurface& m = q.surface();
+  const AtomSurface& at = m.atom_;
+  Packed p;
+  auto spline = [](const Spline1D& s, std::vector<double>& out) {
+    out.insert(out.end(), s.x_.begin(), s.x_.end());
+    out.insert(out.end(), s.y_.begin(), s.y_.end());
+    out.insert(out.end(), s.m_.begin(), s.m_.end());
+  };
+  p.c3v = at.c3v_ ? 1 : 0;
+  p.lattice_constant = m.a_;
+  p.zso_min = m.zso_min_;
+  p.zso_max = m.zso_max_;
+  p.r_min = m.r_min_;
+  p.r_max = m.r_max_;
+  p.z_min = m.z_min_;
+  p.h2_asymptote = m.h2_asymptote();
+  spline(at.corrugation_, p.corrugation);
+  for (const auto& c : at.cluster_) p.cluster.insert(p.cluster.end(), {c.x, c.y, c.z, static_cast<double>(c.layer)});
+  p.atom_offsets.push_back(0);
+  for (const Spline1D& c : at.cuts_) {
+    spline(c, p.atom_splines);
+    p.atom_offsets.push_back(p.atom_offsets.back() + static_cast<int>(c.x_.size()));
+  }
+  const int used = molecule_cuts_used(m.symmetry_);
+  for (int k = 0; k < used; ++k) {
+    const Spline2D& c = m.cuts_[k];
+    int group = k;
+    for (int g = 0; g < k && group == k; ++g)
+      if (m.cuts_[g].x_ == c.x_ && m.cuts_[g].y_ == c.y_) group = g;
+    p.cut_shape.insert(p.cut_shape.end(), {static_cast<int>(c.x_.size()), static_cast<int>(c.y_.size()), group});
+    p.cut_offsets.push_back(static_cast<long>(p.cut_data.size()));
+    for (const auto* v : {&c.x_, &c.y_, &c.f_, &c.fxx_, &c.fyy_, &c.fxxyy_})
+      p.cut_data.insert(p.cut_data.end(), v->begin(), v->end());
+  }
+  p.cut_offsets.push_back(static_cast<long>(p.cut_data.size()));
+  spline(m.h2_, p.h2);
+  spline(m.z_dependence_, p.z_dependence);
+  return p;
 }
 
 // ---------------------------------------------------------------------------------------------------------------

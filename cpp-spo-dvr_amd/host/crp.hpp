@@ -91,6 +91,10 @@ DataSet read_data_set(const std::string& directory);
 // The molecular cuts the interpolation uses (src/crplib/dataset.cpp:33-40).
 int molecule_cuts_used(Symmetry s);
 
+struct Packed;
+class QdSurface;
+Packed pack(const QdSurface& surface);
+
 // A cubic spline in one variable with F'' at the knots (src/crplib/spline.cpp:81-140).
 class Spline1D {
  public:
@@ -110,6 +114,7 @@ class Spline1D {
   const std::vector<double>& y() const { return y_; }
 
  private:
+  friend Packed pack(const QdSurface&);
   std::vector<double> x_, y_, m_;
 };
 
@@ -134,6 +139,7 @@ class Spline2D {
   const std::vector<double>& y() const { return y_; }
 
  private:
+  friend Packed pack(const QdSurface&);
   std::vector<double> x_, y_, f_, fxx_, fyy_, fxxyy_;
 };
 
@@ -156,6 +162,7 @@ class AtomSurface {
   double lattice_constant() const { return a_; }
 
  private:
+  friend Packed pack(const QdSurface&);
   struct ClusterAtom {
     double x, y, z;
     int layer;
@@ -202,6 +209,7 @@ class MoleculeSurface {
   const AtomSurface& atoms() const { return atom_; }
 
  private:
+  friend Packed pack(const QdSurface&);
   Symmetry symmetry_;
   double a_;
   AtomSurface atom_;
@@ -229,6 +237,21 @@ class QdSurface {
 
  private:
   MoleculeSurface surface_;
+};
+
+// A surface as flat arrays for the device evaluation (csrc/kernels_crp.hpp; the layout of spo_hip_crp_surface).
+struct Packed {
+  int c3v = 0;
+  double lattice_constant = 0.0, zso_min = 0.0, zso_max = 0.0, r_min = 0.0, r_max = 0.0, z_min = 0.0;
+  double h2_asymptote = 0.0;                // hartree
+  std::vector<double> corrugation;         // x[n] y[n] F''[n]
+  std::vector<double> cluster;             // x y z layer per atom
+  std::vector<int> atom_offsets;           // site s: knots [off[s], off[s + 1]) of atom_splines' x, y, F'' blocks
+  std::vector<double> atom_splines;        // per site at 3 off[s]: x[k] y[k] F''[k]
+  std::vector<int> cut_shape;              // per used cut: nr, nz, the first cut with the same (r, Z) grid
+  std::vector<long> cut_offsets;           // per used cut: start of r[nr] Z[nz] f fxx fyy fxxyy[nr nz each]
+  std::vector<double> cut_data;
+  std::vector<double> h2, z_dependence;    // x[n] y[n] F''[n]
 };
 
 }  // namespace spo::crp

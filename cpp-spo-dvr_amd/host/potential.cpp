@@ -143,12 +143,46 @@ void clamp(std::vector<double>& v, const Config& c) {
 
 }  // namespace
 
+spo_hip_crp_surface crp_surface_view(const crp::Packed& p) {
+  spo_hip_crp_surface v{};
+  v.c3v = p.c3v;
+  v.lattice_constant = p.lattice_constant;
+  v.zso_min = p.zso_min;
+  v.zso_max = p.zso_max;
+  v.r_min = p.r_min;
+  v.r_max = p.r_max;
+  v.z_min = p.z_min;
+  v.h2_asymptote = p.h2_asymptote;
+  v.n_corrugation = static_cast<int>(p.corrugation.size() / 3);
+  v.corrugation = p.corrugation.data();
+  v.n_cluster = static_cast<int>(p.cluster.size() / 4);
+  v.cluster = p.cluster.data();
+  v.atom_offsets = p.atom_offsets.data();
+  v.atom_splines = p.atom_splines.data();
+  v.n_cuts = static_cast<int>(p.cut_shape.size() / 3);
+  v.cut_shape = p.cut_shape.data();
+  v.cut_offsets = p.cut_offsets.data();
+  v.cut_data = p.cut_data.data();
+  v.n_h2 = static_cast<int>(p.h2.size() / 3);
+  v.h2 = p.h2.data();
+  v.n_z_dependence = static_cast<int>(p.z_dependence.size() / 3);
+  v.z_dependence = p.z_dependence.data();
+  return v;
+}
+
 PotentialGrid compute_potential_grid_on_device(const Pes& pes, const Config& c, const SpatialGrids& g,
                                                const AngularGrid& a) {
   int id = 0;
   if (c.input.system_id == "HardWallPES") id = SPO_HIP_PES_HARDWALL;
   if (c.input.system_id == "H2Cu111DaiLight") id = SPO_HIP_PES_DAILIGHT;
+  if (c.input.system_id == "CRPlibrary") id = SPO_HIP_PES_CRP;
   if (id == 0) throw NotSupportedError("SystemID " + c.input.system_id + " has no device surface");
+  crp::Packed packed;
+  spo_hip_crp_surface crp_view{};
+  if (id == SPO_HIP_PES_CRP) {
+    packed = crp::pack(*load_crp_surface(c.input.crp_data_set));
+    crp_view = crp_surface_view(packed);
+  }
   PotentialGrid grid;
   grid.symmetry = make_grid_symmetry(c, a.n_phi());
   const int nr = g.r.size();
@@ -183,6 +217,7 @@ PotentialGrid compute_potential_grid_on_device(const Pes& pes, const Config& c, 
   d.impose_v_max = c.input.impose_v_max ? 1 : 0;
   d.v_min = c.derived.v_min;
   d.v_max = c.derived.v_max;
+  d.crp = id == SPO_HIP_PES_CRP ? &crp_view : nullptr;
   if (spo_hip_potential_grid(&d, grid.values.data()) != SPO_HIP_OK)
     throw std::runtime_error(std::string("device potential grid: ") + spo_hip_last_error());
   grid.asymptotic = gas_phase_potential(pes, c.derived.z_asymptotic, g);

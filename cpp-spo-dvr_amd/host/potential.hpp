@@ -6,6 +6,7 @@
 #include <string>
 #include <vector>
 
+#include "../../include/spo_hip.h"
 #include "config.hpp"
 #include "crp.hpp"
 #include "grids.hpp"
@@ -51,6 +52,9 @@ class CrpPes final : public Pes {
 std::shared_ptr<const crp::QdSurface> load_crp_surface(const std::string& directory);
 
 std::unique_ptr<Pes> make_pes(const std::string& system_id, const std::string& crp_data_set = "");
+
+// The device library's view (spo_hip_crp_surface) of a packed CRP surface; points into `packed`.
+spo_hip_crp_surface crp_surface_view(const crp::Packed& packed);
 
 // V in hartree on (itp, irep, izr), itp = iphi + nphi * itheta, izr = ir + nr * iz < nz_potential * nr, and
 // the gas-phase V(r) (src/potential/potential_grid.hpp:14-31).

@@ -100,7 +100,7 @@ def host_library() -> ctypes.CDLL:
         for fn in ("spo_crp_molecule", "spo_crp_atom"):
             getattr(lib, fn).argtypes = [ctypes.c_void_p, ctypes.c_long, ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
                                          ctypes.c_void_p]
-        for fn in ("spo_crp_qd", "spo_crp_gas_phase"):
+        for fn in ("spo_crp_qd", "spo_crp_gas_phase", "spo_crp_device_values"):
             getattr(lib, fn).argtypes = [ctypes.c_void_p, ctypes.c_long, ctypes.c_void_p, ctypes.c_void_p]
         lib.spo_crp_geometry.argtypes = [ctypes.c_long, ctypes.c_void_p, ctypes.c_void_p]
         lib.spo_crp_parameters.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
@@ -191,6 +191,13 @@ class CrpSurface:
         pts = _points(points, 6)
         out = np.empty(len(pts))
         _check(host_library().spo_crp_qd(self._handle, len(pts), _ptr(pts), _ptr(out)))
+        return out
+
+    def device_qd(self, points) -> np.ndarray:
+        """qd(points) evaluated on the GPU."""
+        pts = _points(points, 6)
+        out = np.empty(len(pts))
+        _check(host_library().spo_crp_device_values(self._handle, len(pts), _ptr(pts), _ptr(out)))
         return out
 
     def gas_phase(self, zr) -> np.ndarray:

@@ -167,7 +167,7 @@ int spo_hip_profile_read(spo_hip_backend* backend, spo_hip_kernel_time* out, int
 
 /* Potential energy surfaces evaluated on the device (the SystemIDs "HardWallPES" and "H2Cu111DaiLight";
  * src/potential/hard_wall_potential.cpp, src/potential/dai_light_cu111_potential.cpp). */
-enum { SPO_HIP_PES_HARDWALL = 1, SPO_HIP_PES_DAILIGHT = 2 };
+enum { SPO_HIP_PES_HARDWALL = 1, SPO_HIP_PES_DAILIGHT = 2, SPO_HIP_PES_CRP = 3 };
 
 /* The PES in eV at n points, points = (x, y, z, r, theta, phi) per point in bohr and radians; replaces
  * Pes::value (src/potential/pes.hpp) for these surfaces. Needs no backend. */
@@ -190,8 +190,32 @@ typedef struct spo_hip_pes_grid {
   const double* gas_phase;
   int impose_v_min, impose_v_max;
   double v_min, v_max; /* hartree */
+  const struct spo_hip_crp_surface* crp; /* SPO_HIP_PES_CRP: the surface; NULL otherwise */
 } spo_hip_pes_grid;
 int spo_hip_potential_grid(const spo_hip_pes_grid* grid, double* values);
+
+/* A CRP surface (SystemID CRPlibrary, C3v or C6v data set) as flat host arrays: the splines and the atom cluster
+ * that the reference's CRP library builds from the data set (src/crplib/hexagonal_atom_surface.cpp,
+ * molecule_surface.cpp), in internal units (bohr, hartree). The device evaluates V alone, the value statements of
+ * the QD wrapper (src/crplib/qd_wrapper.cpp:46-66): clamps, atom positions, V = Va + Vb + I below ZsoMin, the switch
+ * to the gas-phase potential up to ZsoMax, eV, minus the free molecule at its last r. libspo_host.so fills it from
+ * a data set (host/crp.cpp pack). */
+typedef struct spo_hip_crp_surface {
+  int c3v; /* 1: C3v (10 atom sites, 29 molecular cuts); 0: C6v (7, 18) */
+  double lattice_constant, zso_min, zso_max, r_min, r_max, z_min, h2_asymptote;
+  int n_corrugation; const double* corrugation; /* x[n] y[n] F''[n] */
+  int n_cluster; const double* cluster;         /* x y z layer per atom, in summation order */
+  const int* atom_offsets;                      /* n_sites + 1: site s has knots [off[s], off[s+1]) */
+  const double* atom_splines;                   /* site s at 3 off[s]: x[k] y[k] F''[k] */
+  int n_cuts; const int* cut_shape;             /* per cut: nr, nz, first cut with the same (r, Z) grid */
+  const long* cut_offsets;                      /* n_cuts + 1: r[nr] Z[nz] f fxx fyy fxxyy (nr nz each) */
+  const double* cut_data;
+  int n_h2; const double* h2;                   /* x y F'' */
+  int n_z_dependence; const double* z_dependence;
+} spo_hip_crp_surface;
+
+/* The CRP surface in eV at n points (u, v, Z, r, theta, phi: skewed bohr, rad), as the QD wrapper. */
+int spo_hip_crp_values(const spo_hip_crp_surface* surface, long n, const double* points, double* values);
 
 #ifdef __cplusplus
 }

@@ -131,6 +131,8 @@ int spo_crp_molecule(const spo_crp* crp, long n, const double* atoms, int length
 int spo_crp_atom(const spo_crp* crp, long n, const double* positions, int length_unit, int energy_unit, double* out);
 /* points[6k..] = u v Z r theta phi (skewed bohr, rad); out[k] = V in eV (src/crplib/qd_wrapper.cpp:46-66). */
 int spo_crp_qd(const spo_crp* crp, long n, const double* points, double* out);
+/* The same values evaluated on the GPU (spo_hip_crp_values of libspo_hip.so on this surface's packed tables). */
+int spo_crp_device_values(const spo_crp* crp, long n, const double* points, double* out);
 /* zr[2k..] = Z r (bohr); out[5k..] = switch F, dF/dZ, V2D, dV2D/dZ, dV2D/dr (hartree). */
 int spo_crp_gas_phase(const spo_crp* crp, long n, const double* zr, double* out);
 /* atoms as for spo_crp_molecule (bohr); out[6k..] = centre x y Z, r, theta, phi (CAMBCORD). */

@@ -1,0 +1,105 @@
+"""The CRP surface on the GPU (spo_hip_crp_values, and spo_hip_potential_grid with SPO_HIP_PES_CRP, the default
+set-up of a HIP run with SystemID CRPlibrary) against the host restatement (host/crp.cpp, itself pinned to the
+Fortran CRP library in tests/test_crp.py) and against the Fortran program's potential grid (gate 1 on
+crp_cu111_srp48_small).
+
+The device evaluates the host's value statements in the same operand order without FMA contraction; the
+differences left are those of the device's sin / cos / acos / atan2, a few ulp of the terms: held to 1e-13 of
+max(|a|, |b|, 1 eV) as the other device surfaces. Every test prints its case before touching the GPU."""
+
+import json
+import math
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from tests import cases, fixtures
+
+pytestmark = pytest.mark.gpu
+
+CRP_DIR = os.path.join(cases.GOLDEN_DIR, "crp")
+FLOOR_EV = 1.0
+TOL = 1e-13
+
+
+def _note(*what):
+    print("crp:", *what, file=sys.stderr, flush=True)
+
+
+def _relative(a, b, floor):
+    return float(np.max(np.abs(a - b) / np.maximum(np.maximum(np.abs(a), np.abs(b)), floor)))
+
+
+def _points(surface, n, seed):
+    """Geometries over the cell, from inside the data to beyond ZsoMax (r and Z partly clamped)."""
+    p = surface.parameters()
+    rng = np.random.default_rng(seed)
+    a = p["lattice_constant"]
+    return np.column_stack([rng.uniform(0, a, n), rng.uniform(0, a, n),
+                            rng.uniform(1.0, p["zso_max"] + 1.0, n), rng.uniform(0.6, 3.0, n),
+                            rng.uniform(0, math.pi, n), rng.uniform(0, 2 * math.pi, n)])
+
+
+@pytest.mark.parametrize("case", ["cu111_srp48", "c3v_pairwise", "c6v_pairwise", "c3v_impulse_15", "c6v_impulse_5"])
+def test_device_values_match_host(case):
+    import spodvr_amd
+    _note("device vs host values", case)
+    s = spodvr_amd.CrpSurface(os.path.join(CRP_DIR, case, "dataset"))
+    try:
+        f = np.load(os.path.join(CRP_DIR, case, "points.npz"))
+        pts = np.vstack([f["wrapper_in"], _points(s, 20000, 11)])
+        host = s.qd(pts)
+        device = s.device_qd(pts)
+        assert _relative(device, host, FLOOR_EV) <= TOL
+        # the Fortran library's own values at its driver points (wrapper.out), through the device
+        assert _relative(device[:len(f["wrapper_out"])], f["wrapper_out"], FLOOR_EV) <= 3e-13
+    finally:
+        s.close()
+
+
+def _run(where):
+    import spodvr_amd
+    old = os.environ.get("SPO_HIP_POTENTIAL")
+    os.environ["SPO_HIP_POTENTIAL"] = where
+    try:
+        return spodvr_amd.Run(cases.golden_input("crp_cu111_srp48_small"),
+                              {"CRPlibDataSet": os.path.join(CRP_DIR, "cu111_srp48", "dataset")}, device="hip")
+    finally:
+        if old is None:
+            os.environ.pop("SPO_HIP_POTENTIAL", None)
+        else:
+            os.environ["SPO_HIP_POTENTIAL"] = old
+
+
+def test_device_crp_grid_matches_host_grid_and_fortran():
+    _note("device CRP grid, crp_cu111_srp48_small")
+    device = _run("device")
+    try:
+        dv = device.array("potential.values")
+        info = device.info
+        rep = device.array("symmetry.representative_of").astype(int)
+        op = device.array("symmetry.operation_of").astype(int)
+        phi_index = device.array("symmetry.phi_index").astype(int)
+    finally:
+        device.close()
+    _note("host CRP grid, crp_cu111_srp48_small")
+    host = _run("host")
+    try:
+        hv = host.array("potential.values")
+    finally:
+        host.close()
+    assert dv.shape == hv.shape
+    assert _relative(dv, hv, FLOOR_EV / 27.2114) <= TOL
+    # gate 1 on the device grid (3e-13 for this data set of total energies, reference values clipped at VMax)
+    f = np.load(fixtures.case_dir("crp_cu111_srp48_small") + "/pes.npz")
+    nzpot, nr, nx, ny, nth, nph = (int(x) for x in f["dims"])
+    assert (info.nz_potential, info.nr, info.nx, info.ny, info.n_theta, info.n_phi) == (nzpot, nr, nx, ny, nth, nph)
+    v_max = json.load(open(cases.golden_input("crp_cu111_srp48_small")))["VMax"] / 27.2114
+    idx = f["index"]
+    it, ip, ix, iy, iz, ir = (idx[:, k] for k in range(6))
+    ixy = iy + ny * ix
+    n_tp, n_rep = nth * nph, info.n_representatives
+    mine = dv[phi_index.reshape(-1, nph)[op[ixy], ip] + nph * it + n_tp * (rep[ixy] + n_rep * (ir + nr * iz))]
+    assert _relative(mine, np.minimum(f["value"], v_max), 1.0 / 27.2114) <= 3e-13
