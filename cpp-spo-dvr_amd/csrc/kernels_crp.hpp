@@ -355,11 +355,99 @@ __device__ inline double crp_qd(const CrpDev& s, double u, double v, double z, d
   return m * (1.0 * ev) - s.h2_asymptote * (1.0 * ev);
 }
 
-__global__ __launch_bounds__(256) void crp_points_kernel(CrpDev s, long n, const double* points, double* values) {
+// The SCM correction (spo_hip_scm_correction): the pair functions of the slab's top two layers
+// (src/potential/scm/scm_correction.cpp; Fortran SCM_Potential.f90), the same operand order as host/scm.cpp.
+struct ScmDev {
+  int on, rescale;
+  double scale, cos_skew, sin_skew;
+  double layers[2][18];
+  int n_atoms;
+  const double* atoms;
+};
+
+// -(p3 + q (p4 + q (p5 + q p6))) exp(-p2 q); t = p1..p8, C
+__device__ inline double scm_unswitched(const double* t, double q) {
+  const double cubic = t[2] + q * (t[3] + q * (t[4] + q * t[5]));
+  return -(cubic * exp(-(t[1] * q)));
+}
+
+__device__ inline void scm_terms(const double* layer, double r, double* t) {
+  double c = r;
+  if (r < layer[16]) {
+    c = layer[16];
+  } else if (r > layer[17]) {
+    c = layer[17];
+  }
+  for (int k = 0; k < 8; ++k) t[k] = layer[8 + k] * c + layer[k];
+  t[8] = scm_unswitched(t, t[7] - t[0]);
+}
+
+__device__ inline double scm_pair(const double* t, double d) {
+  if (d > t[7]) return t[8];
+  const double f = scm_unswitched(t, d - t[0]);
+  if (!(d > t[6])) return f;
+  const double phase = ((d - t[7]) * spo::crp_tables::kPi) / (t[7] - t[6]);
+  const double rho = 0.5 * cos(phase) + 0.5;
+  return (1.0 - rho) * f + rho * t[8];
+}
+
+__device__ inline double scm_distance(const double* a, const double* b) {
+  const double dx = a[0] - b[0], dy = a[1] - b[1], dz = a[2] - b[2];
+  return sqrt((dx * dx + dy * dy) + dz * dz);
+}
+
+// dV (hartree) at the Cartesian centre (x, y), Z, r, theta, phi
+__device__ inline double scm_delta(const ScmDev& m, double x, double y, double z, double r, double theta, double phi) {
+  const double st = sin(theta), ct = cos(theta), sp = sin(phi), cp = cos(phi);
+  const double h = 0.5 * r;
+  const double o[3] = {(h * st) * cp, (h * st) * sp, h * ct};
+  double hd[2][3] = {{x - o[0], y - o[1], z - o[2]}, {x + o[0], y + o[1], z + o[2]}};
+  double hi[2][3];
+  if (m.rescale) {
+    const double bond[3] = {hd[1][0] - hd[0][0], hd[1][1] - hd[0][1], hd[1][2] - hd[0][2]};
+    const double c[3] = {((hd[0][0] + hd[1][0]) / 2.0) / m.scale, ((hd[0][1] + hd[1][1]) / 2.0) / m.scale,
+                         (hd[0][2] + hd[1][2]) / 2.0};
+    for (int k = 0; k < 3; ++k) {
+      hi[0][k] = c[k] - 0.5 * bond[k];
+      hi[1][k] = c[k] + 0.5 * bond[k];
+    }
+  } else {
+    for (int k = 0; k < 3; ++k) {
+      hi[0][k] = hd[0][k];
+      hi[1][k] = hd[1][k];
+    }
+  }
+  double t[2][9];
+  scm_terms(m.layers[0], r, t[0]);
+  scm_terms(m.layers[1], r, t[1]);
+  double dv = +0.0;
+  for (int i = 0; i < m.n_atoms; ++i) {
+    const double* at = m.atoms + 7 * i;
+    const double* tl = t[at[6] == 1.0 ? 0 : 1];
+    for (int a = 0; a < 2; ++a) {
+      const double f_ideal = scm_pair(tl, scm_distance(hi[a], at + 3));
+      const double f_distorted = scm_pair(tl, scm_distance(hd[a], at));
+      dv = (dv - f_ideal) + f_distorted;
+    }
+  }
+  return dv;
+}
+
+// The CRP surface, with the SCM correction when m.on, in eV at skewed (u, v)
+__device__ inline double crp_scm_qd(const CrpDev& s, const ScmDev& m, double u, double v, double z, double r,
+                                    double theta, double phi) {
+  if (!m.on) return crp_qd(s, u, v, z, r, theta, phi);
+  const double x = u + m.cos_skew * v, y = m.sin_skew * v;
+  const double v_crp = crp_qd(s, u / m.scale, v / m.scale, z, r, theta, phi);
+  return v_crp + scm_delta(m, x, y, z, r, theta, phi) / 0.036749308286;
+}
+
+__global__ __launch_bounds__(256) void crp_points_kernel(CrpDev s, ScmDev m, long n, const double* points,
+                                                         double* values) {
   const long i = static_cast<long>(blockIdx.x) * 256 + threadIdx.x;
   if (i >= n) return;
   const double* p = points + 6 * i;
-  values[i] = crp_qd(s, p[0], p[1], p[2], p[3], p[4], p[5]);
+  values[i] = crp_scm_qd(s, m, p[0], p[1], p[2], p[3], p[4], p[5]);
 }
 
 }  // namespace spo_dev

@@ -120,6 +120,7 @@ struct PesGrid {
   int impose_v_min, impose_v_max;
   double v_min, v_max;
   CrpDev crp;  // pes == 3 (CRP) only
+  ScmDev scm;  // pes == 3 with a distorted slab (scm.on)
 };
 
 // One value per thread, in the table's own order: index = iphi + n_phi (itheta + n_theta (irep + n_rep izr)).
@@ -136,7 +137,7 @@ __global__ __launch_bounds__(256) void potential_grid_kernel(PesGrid g, double* 
   const int ix = ixy / g.ny, iy = ixy % g.ny;
   const double z = g.z[iz];
   // eV -> hartree by a multiplication with 1 / 27.2114, as the reference's grid builder does
-  const double ev = g.pes == 3 ? crp_qd(g.crp, g.x[ix], g.y[iy], z, g.r[ir], g.theta[it], g.phi[ip])
+  const double ev = g.pes == 3 ? crp_scm_qd(g.crp, g.scm, g.x[ix], g.y[iy], z, g.r[ir], g.theta[it], g.phi[ip])
                                : pes_value(g.pes, g.x[ix], g.y[iy], z, g.r[ir], g.theta[it], g.phi[ip]);
   double v = ev * (1.0 / kHartreeInEv);
   if (g.use_switching && tol_gt(z, g.z_min_switch)) {

@@ -1329,6 +1329,31 @@ struct CrpUpload {
   }
 };
 
+// An SCM correction checked and uploaded (the atoms' layers 1 or 2, a positive scale).
+struct ScmUpload {
+  Buffer atoms;
+  spo_dev::ScmDev view{};
+
+  explicit ScmUpload(const spo_hip_scm_correction* m) {
+    if (!m) return;
+    if (m->n_atoms < 1 || !m->atoms) throw Status(SPO_HIP_E_ARGUMENT, "SCM correction: no atoms");
+    if (!(m->scale > 0.0)) throw Status(SPO_HIP_E_ARGUMENT, "SCM correction: the scale must be positive");
+    for (int i = 0; i < m->n_atoms; ++i)
+      if (m->atoms[7 * i + 6] != 1.0 && m->atoms[7 * i + 6] != 2.0)
+        throw Status(SPO_HIP_E_ARGUMENT, "SCM correction: an atom outside layers 1 and 2");
+    atoms = upload(m->atoms, 7 * std::size_t(m->n_atoms) * sizeof(double));
+    view.on = 1;
+    view.rescale = m->rescale;
+    view.scale = m->scale;
+    view.cos_skew = m->cos_skew;
+    view.sin_skew = m->sin_skew;
+    for (int l = 0; l < 2; ++l)
+      for (int k = 0; k < 18; ++k) view.layers[l][k] = m->layers[l][k];
+    view.n_atoms = m->n_atoms;
+    view.atoms = atoms.as<double>();
+  }
+};
+
 void check_finite(const double* v, long n, const char* where) {
   for (long i = 0; i < n; ++i)
     if (std::isnan(v[i]))
@@ -1341,16 +1366,22 @@ void check_finite(const double* v, long n, const char* where) {
 extern "C" {
 
 int spo_hip_crp_values(const spo_hip_crp_surface* surface, long n, const double* points, double* values) {
+  return spo_hip_scm_values(surface, nullptr, n, points, values);
+}
+
+int spo_hip_scm_values(const spo_hip_crp_surface* surface, const spo_hip_scm_correction* scm, long n,
+                       const double* points, double* values) {
   return guarded([&] {
     if (!surface) throw Status(SPO_HIP_E_ARGUMENT, "spo_hip_crp_values: NULL surface");
     if (n < 0 || (n > 0 && (!points || !values))) throw Status(SPO_HIP_E_ARGUMENT, "spo_hip_crp_values: bad arrays");
     if (n == 0) return;
     require_device();
     CrpUpload crp(*surface);
+    ScmUpload correction(scm);
     Buffer p = upload(points, std::size_t(n) * 6 * sizeof(double));
     Buffer out(std::size_t(n) * sizeof(double));
-    hipLaunchKernelGGL(spo_dev::crp_points_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, nullptr, crp.view, n,
-                       p.as<double>(), out.as<double>());
+    hipLaunchKernelGGL(spo_dev::crp_points_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, nullptr, crp.view,
+                       correction.view, n, p.as<double>(), out.as<double>());
     HIP_CHECK(hipGetLastError());
     HIP_CHECK(hipMemcpy(values, out.as<void>(), std::size_t(n) * sizeof(double), hipMemcpyDeviceToHost));
     check_finite(values, n, "spo_hip_crp_values");
@@ -1386,11 +1417,12 @@ int spo_hip_potential_grid(const spo_hip_pes_grid* grid, double* values) {
     Buffer out(std::size_t(n) * sizeof(double));
     std::optional<CrpUpload> crp;
     if (g.pes == SPO_HIP_PES_CRP) crp.emplace(*g.crp);
+    const ScmUpload correction(g.pes == SPO_HIP_PES_CRP ? g.scm : nullptr);
     spo_dev::PesGrid d{g.pes, g.ny, g.nr, g.n_theta, g.n_phi, g.n_representatives, static_cast<int>(n_zr),
                        x.as<double>(), y.as<double>(), z.as<double>(), r.as<double>(), th.as<double>(),
                        ph.as<double>(), reps.as<int>(), g.use_switching, g.z_min_switch, g.z_max_switch,
                        g.use_switching ? gas.as<double>() : nullptr, g.impose_v_min, g.impose_v_max, g.v_min,
-                       g.v_max, crp ? crp->view : spo_dev::CrpDev{}};
+                       g.v_max, crp ? crp->view : spo_dev::CrpDev{}, correction.view};
     hipLaunchKernelGGL(spo_dev::potential_grid_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, nullptr, d,
                        out.as<double>());
     HIP_CHECK(hipGetLastError());

@@ -16,6 +16,10 @@ struct spo_crp {
   std::shared_ptr<const spo::crp::QdSurface> qd;
 };
 
+struct spo_scm {
+  std::shared_ptr<const spo::scm::ScmSurface> surface;
+};
+
 struct spo_run {
   std::shared_ptr<spo::RunSetup> setup;
   std::unique_ptr<spo::Propagation> propagation;
@@ -472,6 +476,63 @@ int spo_crp_device_values(const spo_crp* crp, long n, const double* points, doub
     const spo_hip_crp_surface view = spo::crp_surface_view(packed);
     if (spo_hip_crp_values(&view, n, points, out) != SPO_HIP_OK)
       throw std::runtime_error(std::string("device CRP values: ") + spo_hip_last_error());
+  });
+}
+
+int spo_scm_open(const char* crp_data_set, const char* parameter_file, const char* position_file, int configuration,
+                 double cutoff, int rescale, spo_scm** out) {
+  return guarded([&] {
+    spo::InputParameters p;
+    p.system_id = "CRPDistrtSurface";
+    p.crp_data_set = crp_data_set;
+    p.scm.parameter_file = parameter_file;
+    p.scm.position_file = position_file;
+    p.scm.configuration = configuration;
+    p.scm.cutoff = cutoff;
+    p.scm.rescale = rescale != 0;
+    *out = new spo_scm{spo::load_scm_surface(p)};
+  });
+}
+
+int spo_scm_close(spo_scm* scm) {
+  delete scm;
+  return SPO_OK;
+}
+
+int spo_scm_values(const spo_scm* scm, long n, const double* points, double* out) {
+  return guarded([&] {
+    const spo::scm::ScmSurface& s = *scm->surface;
+    for_points(n, [&](long k) {
+      const double* p = points + 6 * k;
+      const auto xy = s.correction().to_cartesian(p[0], p[1]);
+      double* o = out + 5 * k;
+      o[0] = xy[0];
+      o[1] = xy[1];
+      o[2] = s.correction().value_hartree(xy[0], xy[1], p[2], p[3], p[4], p[5]);
+      o[3] = s.crp().potential_ev(p[0] / s.crp_scale(), p[1] / s.crp_scale(), p[2], p[3], p[4], p[5]);
+      o[4] = s.value_ev(p[0], p[1], p[2], p[3], p[4], p[5]);
+    });
+  });
+}
+
+int spo_scm_device_values(const spo_scm* scm, long n, const double* points, double* out) {
+  return guarded([&] {
+    const spo::crp::Packed packed = spo::crp::pack(scm->surface->crp());
+    const spo_hip_crp_surface crp_view = spo::crp_surface_view(packed);
+    std::vector<double> atoms;
+    const spo_hip_scm_correction scm_view = spo::scm_correction_view(*scm->surface, atoms);
+    if (spo_hip_scm_values(&crp_view, &scm_view, n, points, out) != SPO_HIP_OK)
+      throw std::runtime_error(std::string("device SCM values: ") + spo_hip_last_error());
+  });
+}
+
+int spo_scm_parameters(const spo_scm* scm, double* out) {
+  return guarded([&] {
+    const spo::scm::ScmSurface& s = *scm->surface;
+    const auto one = s.correction().to_cartesian(0.0, 1.0);
+    const double v[5] = {s.crp_scale(), s.lattice_constant(), one[0], one[1],
+                         static_cast<double>(s.correction().atoms().size())};
+    for (int i = 0; i < 5; ++i) out[i] = v[i];
   });
 }
 

@@ -38,6 +38,16 @@ SystemParameters system_parameters_for(const std::string& id, const std::string&
   throw NotSupportedError("SystemID " + id + " is not supported by this backend's set-up");
 }
 
+SystemParameters system_parameters_for(const InputParameters& p) {
+  if (p.system_id != "CRPDistrtSurface") return system_parameters_for(p.system_id, p.crp_data_set);
+  // the CRP's parameters with the slab's lattice constant when it is rescaled (src/potential/scm/
+  // scm_crp_potential.cpp:46-54)
+  const auto scm = load_scm_surface(p);
+  const crp::MoleculeSurface& s = scm->crp().surface();
+  return {scm->lattice_constant(), scm->lattice_constant(), s.z_asymptotic(), scm->crp().skewing_angle(),
+          s.symmetry() == crp::Symmetry::c3v ? "C3v" : "C6v"};
+}
+
 namespace {
 
 void get(const json::Value& d, const char* key, double& out) {
@@ -129,6 +139,16 @@ InputParameters read_input(const json::Value& d) {
   get(d, "Title", p.title);
   get(d, "SystemID", p.system_id);
   get(d, "CRPlibDataSet", p.crp_data_set);
+  get(d, "DistortedPESMode", p.scm.mode);
+  get(d, "SCMParFilepath", p.scm.parameter_file);
+  get(d, "PosFilepath", p.scm.position_file);
+  get(d, "PosFileSlabNum", p.scm.configuration);
+  get(d, "EnableSurfExpan", p.scm.rescale);
+  get(d, "SCMCutoff", p.scm.cutoff);
+  if (p.system_id != "CRPDistrtSurface")
+    for (const char* key : {"DistortedPESMode", "SCMParFilepath", "PosFilepath", "PosFileSlabNum", "EnableSurfExpan",
+                            "SCMCutoff"})
+      if (d.find(key)) throw InputError(key, std::string(key) + " is only used with SystemID CRPDistrtSurface");
   get(d, "XDoF", p.x_dof);
   get(d, "YDoF", p.y_dof);
   get(d, "Mass1", p.mass1);
@@ -251,7 +271,15 @@ Config build_config(const InputParameters& input) {
   Config c{input, {}};
   const InputParameters& p = c.input;
   DerivedParameters& d = c.derived;
-  d.system = system_parameters_for(p.system_id, p.crp_data_set);
+  if (p.system_id == "CRPDistrtSurface") {  // src/input/json_input.cpp:379-405
+    if (p.scm.mode != 1) throw InputError("DistortedPESMode", "only mode 1 (SCM) exists");
+    if (p.scm.parameter_file.empty()) throw InputError("SCMParFilepath", "SystemID CRPDistrtSurface needs SCMParFilepath");
+    if (p.scm.position_file.empty()) throw InputError("PosFilepath", "SystemID CRPDistrtSurface needs PosFilepath");
+    if (p.use_symmetry)
+      throw InputError("UseSymmetry", "UseSymmetry = true is not supported with SystemID CRPDistrtSurface");
+    if (!p.homonuclear) throw InputError("HomoNuc", "HomoNuc = false is not supported with SystemID CRPDistrtSurface");
+  }
+  d.system = system_parameters_for(p);
   d.z_asymptotic_unswitched = d.system.z_asymptotic;
   d.z_asymptotic = p.use_switching ? p.z_max_switch : d.system.z_asymptotic;
 

@@ -37,7 +37,16 @@ struct LowAbsorber {
 struct InputParameters {
   std::string title;
   std::string system_id;
-  std::string crp_data_set;  // CRPlibDataSet: the CRP data set directory of SystemID CRPlibrary
+  std::string crp_data_set;  // CRPlibDataSet: the CRP data set directory of SystemID CRPlibrary / CRPDistrtSurface
+  // SystemID CRPDistrtSurface: the SCM correction's files and choices (src/input/config.hpp:29-36)
+  struct Scm {
+    int mode = 1;                 // DistortedPESMode: 1 = SCM, the only mode
+    std::string parameter_file;   // SCMParFilepath (params.dat)
+    std::string position_file;    // PosFilepath (slab snapshots)
+    int configuration = 1;        // PosFileSlabNum, 1-based
+    bool rescale = true;          // EnableSurfExpan: the slab's lattice constant for the grid, CRP at (u/s, v/s)
+    double cutoff = 16.0;         // SCMCutoff (bohr)
+  } scm;
   bool x_dof = true, y_dof = true;
   double mass1 = 1.0078, mass2 = 1.0078;
   bool homonuclear = true;
@@ -119,6 +128,7 @@ struct Config {
 
 // The system parameters of a potential by SystemID (HardWallPES, H2Cu111DaiLight; CRPlibrary from its data set).
 SystemParameters system_parameters_for(const std::string& system_id, const std::string& crp_data_set = "");
+SystemParameters system_parameters_for(const struct InputParameters& input);
 
 // Reads the keys from a parsed input document; unknown keys are ignored, keys of other subsystems are not read.
 InputParameters read_input(const json::Value& document);

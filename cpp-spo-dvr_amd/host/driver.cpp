@@ -23,7 +23,7 @@ std::shared_ptr<RunSetup> make_run_setup(const InputParameters& input, bool devi
   s->grids = make_spatial_grids(s->config);
   s->angular = make_angular_grid(s->config);
   s->basis = make_rotational_basis(s->angular, s->config.input.j_in, s->config.input.mj_in);
-  const auto pes = make_pes(s->config.input.system_id, s->config.input.crp_data_set);
+  const auto pes = make_pes(s->config.input);
   s->potential = device_potential ? compute_potential_grid_on_device(*pes, s->config, s->grids, s->angular)
                                   : compute_potential_grid(*pes, s->config, s->grids, s->angular);
   s->gas = compute_gas_phase(s->config, s->angular, s->potential.asymptotic);

@@ -117,6 +117,42 @@ std::shared_ptr<const crp::QdSurface> load_crp_surface(const std::string& direct
   return slot;
 }
 
+double ScmPes::value(double x, double y, double z, double r, double theta, double phi) const {
+  try {
+    return surface_->value_ev(x, y, z, r, theta, phi);
+  } catch (const crp::EvaluationError& e) {
+    throw InputError("CRPlibDataSet", std::string("the CRP PES cannot be evaluated on this grid: ") + e.what());
+  }
+}
+
+std::shared_ptr<const scm::ScmSurface> load_scm_surface(const InputParameters& p) {
+  static std::mutex lock;
+  static std::map<std::string, std::shared_ptr<const scm::ScmSurface>> loaded;
+  const std::string key = p.crp_data_set + "\n" + p.scm.parameter_file + "\n" + p.scm.position_file + "\n" +
+                          std::to_string(p.scm.configuration) + "\n" + std::to_string(p.scm.rescale) + "\n" +
+                          std::to_string(p.scm.cutoff);
+  const auto crp = load_crp_surface(p.crp_data_set);
+  const std::lock_guard<std::mutex> guard(lock);
+  auto& slot = loaded[key];
+  if (!slot) {
+    try {
+      slot = std::make_shared<const scm::ScmSurface>(crp, scm::read_parameters(p.scm.parameter_file),
+                                                     scm::read_slab(p.scm.position_file, p.scm.configuration,
+                                                                    p.scm.cutoff),
+                                                     p.scm.rescale);
+    } catch (const crp::DataError& e) {
+      loaded.erase(key);
+      throw InputError("PosFilepath", e.what());
+    }
+  }
+  return slot;
+}
+
+std::unique_ptr<Pes> make_pes(const InputParameters& p) {
+  if (p.system_id == "CRPDistrtSurface") return std::make_unique<ScmPes>(load_scm_surface(p));
+  return make_pes(p.system_id, p.crp_data_set);
+}
+
 std::unique_ptr<Pes> make_pes(const std::string& id, const std::string& crp_data_set) {
   if (id == "HardWallPES") return std::make_unique<HardWallPes>();
   if (id == "H2Cu111DaiLight") return std::make_unique<DaiLightPes>();
@@ -170,19 +206,49 @@ spo_hip_crp_surface crp_surface_view(const crp::Packed& p) {
   return v;
 }
 
+spo_hip_scm_correction scm_correction_view(const scm::ScmSurface& s, std::vector<double>& atoms) {
+  spo_hip_scm_correction v{};
+  const scm::Correction& m = s.correction();
+  v.rescale = m.rescaling() ? 1 : 0;
+  v.scale = s.crp_scale();
+  const auto unit_v = m.to_cartesian(0.0, 1.0);  // (cos, sin) of the skewing angle, as the host computes them
+  v.cos_skew = unit_v[0];
+  v.sin_skew = unit_v[1];
+  for (int l = 0; l < 2; ++l) {
+    const scm::Layer& layer = m.layers()[l];
+    for (int k = 0; k < 8; ++k) {
+      v.layers[l][k] = layer.intercept[k];
+      v.layers[l][8 + k] = layer.slope[k];
+    }
+    v.layers[l][16] = layer.r_clamp_min;
+    v.layers[l][17] = layer.r_clamp_max;
+  }
+  atoms.clear();
+  for (const scm::Atom& at : m.atoms())
+    atoms.insert(atoms.end(), {at.distorted.x, at.distorted.y, at.distorted.z, at.ideal.x, at.ideal.y, at.ideal.z,
+                               static_cast<double>(at.layer)});
+  v.n_atoms = static_cast<int>(m.atoms().size());
+  v.atoms = atoms.data();
+  return v;
+}
+
 PotentialGrid compute_potential_grid_on_device(const Pes& pes, const Config& c, const SpatialGrids& g,
                                                const AngularGrid& a) {
   int id = 0;
   if (c.input.system_id == "HardWallPES") id = SPO_HIP_PES_HARDWALL;
   if (c.input.system_id == "H2Cu111DaiLight") id = SPO_HIP_PES_DAILIGHT;
-  if (c.input.system_id == "CRPlibrary") id = SPO_HIP_PES_CRP;
+  if (c.input.system_id == "CRPlibrary" || c.input.system_id == "CRPDistrtSurface") id = SPO_HIP_PES_CRP;
   if (id == 0) throw NotSupportedError("SystemID " + c.input.system_id + " has no device surface");
   crp::Packed packed;
   spo_hip_crp_surface crp_view{};
+  std::vector<double> scm_atoms;
+  spo_hip_scm_correction scm_view{};
+  const bool distorted = c.input.system_id == "CRPDistrtSurface";
   if (id == SPO_HIP_PES_CRP) {
     packed = crp::pack(*load_crp_surface(c.input.crp_data_set));
     crp_view = crp_surface_view(packed);
   }
+  if (distorted) scm_view = scm_correction_view(*load_scm_surface(c.input), scm_atoms);
   PotentialGrid grid;
   grid.symmetry = make_grid_symmetry(c, a.n_phi());
   const int nr = g.r.size();
@@ -218,6 +284,7 @@ PotentialGrid compute_potential_grid_on_device(const Pes& pes, const Config& c, 
   d.v_min = c.derived.v_min;
   d.v_max = c.derived.v_max;
   d.crp = id == SPO_HIP_PES_CRP ? &crp_view : nullptr;
+  d.scm = distorted ? &scm_view : nullptr;
   if (spo_hip_potential_grid(&d, grid.values.data()) != SPO_HIP_OK)
     throw std::runtime_error(std::string("device potential grid: ") + spo_hip_last_error());
   grid.asymptotic = gas_phase_potential(pes, c.derived.z_asymptotic, g);

@@ -9,6 +9,7 @@
 #include "../../include/spo_hip.h"
 #include "config.hpp"
 #include "crp.hpp"
+#include "scm.hpp"
 #include "grids.hpp"
 #include "symmetry.hpp"
 
@@ -51,10 +52,27 @@ class CrpPes final : public Pes {
 // a missing or unusable data set.
 std::shared_ptr<const crp::QdSurface> load_crp_surface(const std::string& directory);
 
+// The CRP surface plus the SCM correction of a distorted slab (SystemID CRPDistrtSurface,
+// src/potential/scm/scm_crp_potential.cpp).
+class ScmPes final : public Pes {
+ public:
+  explicit ScmPes(std::shared_ptr<const scm::ScmSurface> surface) : surface_(std::move(surface)) {}
+  double value(double x, double y, double z, double r, double theta, double phi) const override;
+
+ private:
+  std::shared_ptr<const scm::ScmSurface> surface_;
+};
+
+// The SCM surface of an input (its CRP data set, parameter and position files), read once per process.
+std::shared_ptr<const scm::ScmSurface> load_scm_surface(const InputParameters& input);
+
 std::unique_ptr<Pes> make_pes(const std::string& system_id, const std::string& crp_data_set = "");
+std::unique_ptr<Pes> make_pes(const InputParameters& input);
 
 // The device library's view (spo_hip_crp_surface) of a packed CRP surface; points into `packed`.
 spo_hip_crp_surface crp_surface_view(const crp::Packed& packed);
+// The device library's view of an SCM correction; `atoms` holds its atom table.
+spo_hip_scm_correction scm_correction_view(const scm::ScmSurface& surface, std::vector<double>& atoms);
 
 // V in hartree on (itp, irep, izr), itp = iphi + nphi * itheta, izr = ir + nr * iz < nz_potential * nr, and
 // the gas-phase V(r) (src/potential/potential_grid.hpp:14-31).

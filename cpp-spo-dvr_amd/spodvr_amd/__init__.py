@@ -104,6 +104,12 @@ def host_library() -> ctypes.CDLL:
             getattr(lib, fn).argtypes = [ctypes.c_void_p, ctypes.c_long, ctypes.c_void_p, ctypes.c_void_p]
         lib.spo_crp_geometry.argtypes = [ctypes.c_long, ctypes.c_void_p, ctypes.c_void_p]
         lib.spo_crp_parameters.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+        lib.spo_scm_open.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int, ctypes.c_double,
+                                     ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]
+        lib.spo_scm_close.argtypes = [ctypes.c_void_p]
+        lib.spo_scm_values.argtypes = [ctypes.c_void_p, ctypes.c_long, ctypes.c_void_p, ctypes.c_void_p]
+        lib.spo_scm_device_values.argtypes = [ctypes.c_void_p, ctypes.c_long, ctypes.c_void_p, ctypes.c_void_p]
+        lib.spo_scm_parameters.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
         _host = lib
     return _host
 
@@ -220,6 +226,50 @@ class CrpSurface:
         d = dict(zip(keys, (float(x) for x in out[:8])))
         d["point_group"] = "C3v" if out[8] == 3.0 else "C6v"
         return d
+
+
+class ScmSurface:
+    """The CRP surface plus the SCM correction of a distorted slab snapshot (SystemID CRPDistrtSurface).
+
+    values(points (n, 6) = u v Z r theta phi) -> (n, 5): Cartesian x, y, dV (hartree), the CRP at (u/s, v/s) (eV)
+    and the total (eV)."""
+
+    def __init__(self, crp_data_set: str, parameter_file: str, position_file: str, configuration: int = 1,
+                 cutoff: float = 16.0, rescale: bool = True):
+        handle = ctypes.c_void_p()
+        _check(host_library().spo_scm_open(crp_data_set.encode(), parameter_file.encode(), position_file.encode(),
+                                           configuration, cutoff, int(rescale), ctypes.byref(handle)))
+        self._handle = handle
+
+    def close(self) -> None:
+        if self._handle:
+            host_library().spo_scm_close(self._handle)
+            self._handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def values(self, points) -> np.ndarray:
+        pts = _points(points, 6)
+        out = np.empty((len(pts), 5))
+        _check(host_library().spo_scm_values(self._handle, len(pts), _ptr(pts), _ptr(out)))
+        return out
+
+    def device_values(self, points) -> np.ndarray:
+        """The total (eV) of values(points)[:, 4], evaluated on the GPU."""
+        pts = _points(points, 6)
+        out = np.empty(len(pts))
+        _check(host_library().spo_scm_device_values(self._handle, len(pts), _ptr(pts), _ptr(out)))
+        return out
+
+    def parameters(self) -> dict:
+        out = np.empty(5)
+        _check(host_library().spo_scm_parameters(self._handle, _ptr(out)))
+        return dict(scale=float(out[0]), lattice_constant=float(out[1]), cos_skew=float(out[2]),
+                    sin_skew=float(out[3]), atoms=int(out[4]))
 
 
 class Run:

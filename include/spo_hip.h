@@ -191,6 +191,7 @@ typedef struct spo_hip_pes_grid {
   int impose_v_min, impose_v_max;
   double v_min, v_max; /* hartree */
   const struct spo_hip_crp_surface* crp; /* SPO_HIP_PES_CRP: the surface; NULL otherwise */
+  const struct spo_hip_scm_correction* scm; /* SPO_HIP_PES_CRP with a distorted slab: the SCM correction, or NULL */
 } spo_hip_pes_grid;
 int spo_hip_potential_grid(const spo_hip_pes_grid* grid, double* values);
 
@@ -214,8 +215,22 @@ typedef struct spo_hip_crp_surface {
   int n_z_dependence; const double* z_dependence;
 } spo_hip_crp_surface;
 
+/* The SCM correction of a distorted slab snapshot (SystemID CRPDistrtSurface, src/potential/scm/): with it the
+ * surface is V = CRP(u / scale, v / scale) + dV / 0.036749308286 (eV), dV in hartree the sum over the slab atoms of
+ * pair functions at the distorted minus at the ideal positions (scm_correction.cpp:105-125). */
+typedef struct spo_hip_scm_correction {
+  int rescale;                     /* the H atoms are taken to the ideal lattice's frame (centre x, y divided by scale) */
+  double scale;                    /* a_T / a_CRP, 1 without rescaling */
+  double cos_skew, sin_skew;       /* Cartesian (u + cos v, sin v) of skewed (u, v) */
+  double layers[2][18];            /* params.dat P1-P18 of layers 1 and 2 */
+  int n_atoms; const double* atoms; /* distorted x y z, ideal x y z, layer (1 or 2) per atom, in summation order */
+} spo_hip_scm_correction;
+
 /* The CRP surface in eV at n points (u, v, Z, r, theta, phi: skewed bohr, rad), as the QD wrapper. */
 int spo_hip_crp_values(const spo_hip_crp_surface* surface, long n, const double* points, double* values);
+/* The same with the SCM correction (scm may be NULL). */
+int spo_hip_scm_values(const spo_hip_crp_surface* surface, const spo_hip_scm_correction* scm, long n,
+                       const double* points, double* values);
 
 #ifdef __cplusplus
 }

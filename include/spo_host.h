@@ -141,6 +141,22 @@ int spo_crp_geometry(long n, const double* atoms, double* out);
  * C3v or C6v. */
 int spo_crp_parameters(const spo_crp* crp, double* out);
 
+/* The CRP surface plus the SCM correction of a distorted slab snapshot (SystemID CRPDistrtSurface; the reference's
+ * src/potential/scm/): params.dat, the slab position file, the 1-based snapshot, the cut-off (bohr) and whether the
+ * slab is rescaled to the CRP lattice (EnableSurfExpan). */
+typedef struct spo_scm spo_scm;
+int spo_scm_open(const char* crp_data_set, const char* parameter_file, const char* position_file, int configuration,
+                 double cutoff, int rescale, spo_scm** out);
+int spo_scm_close(spo_scm* scm);
+/* points[6k..] = u v Z r theta phi; out[5k..] = Cartesian x, y of (u, v), dV (hartree), the CRP at (u/s, v/s) (eV)
+ * and their sum (eV), as the reference's SCM driver prints them. */
+int spo_scm_values(const spo_scm* scm, long n, const double* points, double* out);
+/* The total (eV) evaluated on the GPU (spo_hip_scm_values). */
+int spo_scm_device_values(const spo_scm* scm, long n, const double* points, double* out);
+/* out[0..4] = rescaling factor s (1 without rescaling), the grid's lattice constant, cos and sin of the skewing
+ * angle, the number of slab atoms in use. */
+int spo_scm_parameters(const spo_scm* scm, double* out);
+
 #ifdef __cplusplus
 }
 #endif

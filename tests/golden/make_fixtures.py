@@ -37,7 +37,7 @@ BK_INDICES = 96          # coefficient indices kept per case
 FORT25_BLOCKS = 5        # energy blocks kept per case (evenly spread)
 PES_POINTS = 1500        # subsample of the PES fixture
 # set-up only cases (the PES and the gas-phase states; no propagation output)
-SETUP_CASES = ["crp_cu111_srp48_small"]
+SETUP_CASES = ["crp_cu111_srp48_small", "scm_cu111_srp48_small", "scm_cu111_srp48_small_rescaling_off"]
 
 
 def fortran_records(path: str) -> list[bytes]:

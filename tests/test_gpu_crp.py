@@ -103,3 +103,76 @@ def test_device_crp_grid_matches_host_grid_and_fortran():
     n_tp, n_rep = nth * nph, info.n_representatives
     mine = dv[phi_index.reshape(-1, nph)[op[ixy], ip] + nph * it + n_tp * (rep[ixy] + n_rep * (ir + nr * iz))]
     assert _relative(mine, np.minimum(f["value"], v_max), 1.0 / 27.2114) <= 3e-13
+
+
+# ---------------------------------------------------------------------------------------------------------------
+# The SCM correction (SystemID CRPDistrtSurface) on the device: spo_hip_scm_values and the grid build.
+
+SCM_RUNS = {"example/rescaling_on": ("example/params.dat", "example/examplePosfile.dat"),
+            "example/rescaling_off": ("example/params.dat", "example/examplePosfile.dat"),
+            "synthetic/v3_slab3_rescaling_on": ("synthetic/params.in", "synthetic/slab_v3.in"),
+            "synthetic/v2_slab2_rescaling_off": ("synthetic/params.in", "synthetic/slab_v2.in")}
+SCM_DIR = os.path.join(cases.GOLDEN_DIR, "scm")
+
+
+@pytest.mark.parametrize("run", sorted(SCM_RUNS))
+def test_device_scm_values_match_host(run):
+    import spodvr_amd
+    _note("device vs host SCM values", run)
+    f = np.load(os.path.join(SCM_DIR, run + ".npz"))
+    par, slab = SCM_RUNS[run]
+    s = spodvr_amd.ScmSurface(os.path.join(CRP_DIR, "cu111_srp48", "dataset"), os.path.join(SCM_DIR, par),
+                              os.path.join(SCM_DIR, slab), int(f["slab_number"]), float(f["cutoff"]),
+                              bool(f["rescaling"]))
+    try:
+        crp = spodvr_amd.CrpSurface(os.path.join(CRP_DIR, "cu111_srp48", "dataset"))
+        pts = np.vstack([f["points"], _points(crp, 20000, 13)])
+        crp.close()
+        host = s.values(pts)[:, 4]
+        device = s.device_values(pts)
+        assert _relative(device, host, FLOOR_EV) <= TOL
+        assert _relative(device[:len(f["out"])], f["out"][:, 4], FLOOR_EV) <= 3e-13
+    finally:
+        s.close()
+
+
+def test_device_scm_grid_matches_host_grid_and_fortran():
+    import spodvr_amd
+    case = "scm_cu111_srp48_small"
+    overrides = {"CRPlibDataSet": os.path.join(CRP_DIR, "cu111_srp48", "dataset"),
+                 "SCMParFilepath": os.path.join(SCM_DIR, "example", "params.dat"),
+                 "PosFilepath": os.path.join(SCM_DIR, "example", "examplePosfile.dat")}
+    values = {}
+    for where in ("device", "host"):
+        _note(where, "SCM grid,", case)
+        old = os.environ.get("SPO_HIP_POTENTIAL")
+        os.environ["SPO_HIP_POTENTIAL"] = where
+        try:
+            run = spodvr_amd.Run(cases.golden_input(case), overrides, device="hip")
+        finally:
+            if old is None:
+                os.environ.pop("SPO_HIP_POTENTIAL", None)
+            else:
+                os.environ["SPO_HIP_POTENTIAL"] = old
+        try:
+            values[where] = run.array("potential.values")
+            info = run.info
+            maps = (run.array("symmetry.representative_of").astype(int), run.array("symmetry.operation_of").astype(int),
+                    run.array("symmetry.phi_index").astype(int))
+        finally:
+            run.close()
+    dv = values["device"]
+    # dV is the difference of two sums of ~300 pair energies of up to several hartree: an ulp of the device's exp in
+    # each term moves it by ~1e-14 hartree (measured 3.5e-13 of max(|V|, 1 eV) on this grid), the reason the
+    # reference's own gate for these cases is 1e-12 (tests/test_regression_pes.cpp:74)
+    assert _relative(dv, values["host"], FLOOR_EV / 27.2114) <= 1e-12
+    f = np.load(fixtures.case_dir(case) + "/pes.npz")
+    nzpot, nr, nx, ny, nth, nph = (int(x) for x in f["dims"])
+    rep, op, phi_index = maps
+    v_max = json.load(open(cases.golden_input(case)))["VMax"] / 27.2114
+    idx = f["index"]
+    it, ip, ix, iy, iz, ir = (idx[:, k] for k in range(6))
+    ixy = iy + ny * ix
+    n_tp, n_rep = nth * nph, info.n_representatives
+    mine = dv[phi_index.reshape(-1, nph)[op[ixy], ip] + nph * it + n_tp * (rep[ixy] + n_rep * (ir + nr * iz))]
+    assert _relative(mine, np.minimum(f["value"], v_max), 1.0 / 27.2114) <= 1e-12
