@@ -24,6 +24,7 @@
 #include "kernels_ct.hpp"
 #include "kernels_mfma.hpp"
 #include "kernels_pes.hpp"
+#include "kernels_mlp.hpp"
 
 using spo_dev::cx;
 
@@ -1266,6 +1267,204 @@ int spo_hip_accumulate(spo_hip_backend* b, double time, const double* coefficien
                        b->energies.as<double>(), time, a.analysis_time_step);
     HIP_CHECK(hipGetLastError());
     HIP_CHECK(hipStreamSynchronize(b->stream));
+  });
+}
+
+}  // extern "C"
+
+// The potential surrogate's device state: FP32 master parameters, gradient, Adam moments, and activation buffers
+// for `capacity` rows.
+struct spo_hip_mlp {
+  int batch = 0;
+  long capacity = 0;
+  int adam_step = 0;
+  Buffer params, grad, m, v;
+  Buffer x, t, y, dy, z1, a1, z2, a2, z3, a3, d1, d2;
+};
+
+namespace {
+
+using spo_dev::kMlpWidth;
+
+void mlp_forward(spo_hip_mlp& n, long rows, const float* x) {
+  const float* p = n.params.as<float>();
+  const long cells = rows * kMlpWidth;
+  hipLaunchKernelGGL(spo_dev::mlp_input_kernel, dim3(blocks_for(cells, 256)), dim3(256), 0, nullptr, x, p, rows,
+                     n.z1.as<float>(), n.a1.as<float>());
+  const int tiles_j = kMlpWidth / 16;
+  const dim3 grid(static_cast<unsigned>(rows / 16 * tiles_j));
+  hipLaunchKernelGGL(spo_dev::mlp_gemm_kernel, grid, dim3(64), 0, nullptr, n.a1.as<float>(), long(kMlpWidth), 1L,
+                     p + spo_dev::kMlpW2, long(kMlpWidth), 1L, kMlpWidth, tiles_j, n.z2.as<float>(), long(kMlpWidth), 1,
+                     p + spo_dev::kMlpB2, n.a2.as<float>());
+  hipLaunchKernelGGL(spo_dev::mlp_gemm_kernel, grid, dim3(64), 0, nullptr, n.a2.as<float>(), long(kMlpWidth), 1L,
+                     p + spo_dev::kMlpW3, long(kMlpWidth), 1L, kMlpWidth, tiles_j, n.z3.as<float>(), long(kMlpWidth), 1,
+                     p + spo_dev::kMlpB3, n.a3.as<float>());
+  hipLaunchKernelGGL(spo_dev::mlp_output_kernel, dim3(blocks_for(rows, 256)), dim3(256), 0, nullptr, n.a3.as<float>(),
+                     p, rows, n.y.as<float>());
+  HIP_CHECK(hipGetLastError());
+}
+
+// The gradient of the mean squared error over `rows` rows into n.grad (after mlp_forward on the same rows).
+void mlp_backward(spo_hip_mlp& n, long rows, const float* x, const float* t, float* loss) {
+  const float* p = n.params.as<float>();
+  float* g = n.grad.as<float>();
+  const long cells = rows * kMlpWidth;
+  const int tiles = kMlpWidth / 16;
+  const long H = kMlpWidth;
+  hipLaunchKernelGGL(spo_dev::mlp_loss_kernel, dim3(1), dim3(256), 0, nullptr, n.y.as<float>(), t, rows,
+                     n.dy.as<float>(), loss);
+  hipLaunchKernelGGL(spo_dev::mlp_column_sum_kernel, dim3(kMlpWidth), dim3(256), 0, nullptr, n.a3.as<float>(), H,
+                     n.dy.as<float>(), rows, g + spo_dev::kMlpW4);
+  hipLaunchKernelGGL(spo_dev::mlp_column_sum_kernel, dim3(1), dim3(256), 0, nullptr, n.dy.as<float>(), 1L,
+                     static_cast<const float*>(nullptr), rows, g + spo_dev::kMlpB4);
+  float* dz3 = n.d2.as<float>();  // dZ3, then dA2 / dZ2 in d1, then dA1 / dZ1 in d2
+  hipLaunchKernelGGL(spo_dev::mlp_output_back_kernel, dim3(blocks_for(cells, 256)), dim3(256), 0, nullptr,
+                     n.dy.as<float>(), p, n.z3.as<float>(), rows, dz3);
+  // layer 3: dW3 = A2^T dZ3, db3, dA2 = dZ3 W3^T, dZ2 = dA2 SiLU'(z2)
+  hipLaunchKernelGGL(spo_dev::mlp_gemm_kernel, dim3(tiles * tiles), dim3(64), 0, nullptr, n.a2.as<float>(), 1L, H,
+                     dz3, H, 1L, static_cast<int>(rows), tiles, g + spo_dev::kMlpW3, H, 0,
+                     static_cast<const float*>(nullptr), static_cast<float*>(nullptr));
+  hipLaunchKernelGGL(spo_dev::mlp_column_sum_kernel, dim3(kMlpWidth), dim3(256), 0, nullptr, dz3, H,
+                     static_cast<const float*>(nullptr), rows, g + spo_dev::kMlpB3);
+  float* dz2 = n.d1.as<float>();
+  hipLaunchKernelGGL(spo_dev::mlp_gemm_kernel, dim3(static_cast<unsigned>(rows / 16 * tiles)), dim3(64), 0, nullptr,
+                     dz3, H, 1L, p + spo_dev::kMlpW3, 1L, H, kMlpWidth, tiles, dz2, H, 0,
+                     static_cast<const float*>(nullptr), static_cast<float*>(nullptr));
+  hipLaunchKernelGGL(spo_dev::mlp_silu_back_kernel, dim3(blocks_for(cells, 256)), dim3(256), 0, nullptr, dz2,
+                     n.z2.as<float>(), cells, dz2);
+  // layer 2
+  hipLaunchKernelGGL(spo_dev::mlp_gemm_kernel, dim3(tiles * tiles), dim3(64), 0, nullptr, n.a1.as<float>(), 1L, H,
+                     dz2, H, 1L, static_cast<int>(rows), tiles, g + spo_dev::kMlpW2, H, 0,
+                     static_cast<const float*>(nullptr), static_cast<float*>(nullptr));
+  hipLaunchKernelGGL(spo_dev::mlp_column_sum_kernel, dim3(kMlpWidth), dim3(256), 0, nullptr, dz2, H,
+                     static_cast<const float*>(nullptr), rows, g + spo_dev::kMlpB2);
+  float* dz1 = n.d2.as<float>();
+  hipLaunchKernelGGL(spo_dev::mlp_gemm_kernel, dim3(static_cast<unsigned>(rows / 16 * tiles)), dim3(64), 0, nullptr,
+                     dz2, H, 1L, p + spo_dev::kMlpW2, 1L, H, kMlpWidth, tiles, dz1, H, 0,
+                     static_cast<const float*>(nullptr), static_cast<float*>(nullptr));
+  hipLaunchKernelGGL(spo_dev::mlp_silu_back_kernel, dim3(blocks_for(cells, 256)), dim3(256), 0, nullptr, dz1,
+                     n.z1.as<float>(), cells, dz1);
+  // layer 1
+  hipLaunchKernelGGL(spo_dev::mlp_input_grad_kernel, dim3(spo_dev::kMlpIn * kMlpWidth), dim3(256), 0, nullptr, x, dz1,
+                     rows, g);
+  hipLaunchKernelGGL(spo_dev::mlp_column_sum_kernel, dim3(kMlpWidth), dim3(256), 0, nullptr, dz1, H,
+                     static_cast<const float*>(nullptr), rows, g + spo_dev::kMlpB1);
+  HIP_CHECK(hipGetLastError());
+}
+
+void mlp_adam(spo_hip_mlp& n, float lr) {
+  constexpr float beta1 = 0.9f, beta2 = 0.999f, eps = 1e-8f;
+  ++n.adam_step;
+  const float c1 = 1.0f - std::pow(beta1, static_cast<float>(n.adam_step));
+  const float c2 = 1.0f - std::pow(beta2, static_cast<float>(n.adam_step));
+  hipLaunchKernelGGL(spo_dev::mlp_adam_kernel, dim3(blocks_for(spo_dev::kMlpParams, 256)), dim3(256), 0, nullptr,
+                     n.params.as<float>(), n.grad.as<float>(), n.m.as<float>(), n.v.as<float>(), spo_dev::kMlpParams,
+                     lr, beta1, beta2, eps, c1, c2);
+  HIP_CHECK(hipGetLastError());
+}
+
+}  // namespace
+
+extern "C" {
+
+int spo_hip_mlp_create(int batch, spo_hip_mlp** out) {
+  return guarded([&] {
+    if (!out) throw Status(SPO_HIP_E_ARGUMENT, "spo_hip_mlp_create: NULL out");
+    if (batch < 16 || batch % 16 != 0 || batch > (1 << 22))
+      throw Status(SPO_HIP_E_ARGUMENT, "spo_hip_mlp_create: batch must be a multiple of 16 (16 .. 4194304)");
+    require_device();
+    auto n = std::make_unique<spo_hip_mlp>();
+    n->batch = batch;
+    n->capacity = std::max<long>(batch, 1L << 16);
+    const std::size_t pb = spo_dev::kMlpParams * sizeof(float);
+    n->params = Buffer(pb);
+    n->grad = Buffer(pb);
+    n->m = Buffer(pb);
+    n->v = Buffer(pb);
+    HIP_CHECK(hipMemset(n->params.as<void>(), 0, pb));
+    HIP_CHECK(hipMemset(n->m.as<void>(), 0, pb));
+    HIP_CHECK(hipMemset(n->v.as<void>(), 0, pb));
+    const std::size_t rows = static_cast<std::size_t>(n->capacity);
+    n->x = Buffer(rows * spo_dev::kMlpIn * sizeof(float));
+    n->t = Buffer(rows * sizeof(float));
+    n->y = Buffer(rows * sizeof(float));
+    n->dy = Buffer(rows * sizeof(float));
+    for (Buffer* b : {&n->z1, &n->a1, &n->z2, &n->a2, &n->z3, &n->a3, &n->d1, &n->d2})
+      *b = Buffer(rows * kMlpWidth * sizeof(float));
+    *out = n.release();
+  });
+}
+
+int spo_hip_mlp_destroy(spo_hip_mlp* mlp) {
+  delete mlp;
+  return SPO_HIP_OK;
+}
+
+int spo_hip_mlp_set_params(spo_hip_mlp* mlp, const float* params) {
+  return guarded([&] {
+    if (!mlp || !params) throw Status(SPO_HIP_E_ARGUMENT, "spo_hip_mlp_set_params: NULL argument");
+    HIP_CHECK(hipMemcpy(mlp->params.as<void>(), params, spo_dev::kMlpParams * sizeof(float), hipMemcpyHostToDevice));
+    const std::size_t pb = spo_dev::kMlpParams * sizeof(float);
+    HIP_CHECK(hipMemset(mlp->m.as<void>(), 0, pb));
+    HIP_CHECK(hipMemset(mlp->v.as<void>(), 0, pb));
+    mlp->adam_step = 0;
+  });
+}
+
+int spo_hip_mlp_get_params(const spo_hip_mlp* mlp, float* params) {
+  return guarded([&] {
+    if (!mlp || !params) throw Status(SPO_HIP_E_ARGUMENT, "spo_hip_mlp_get_params: NULL argument");
+    HIP_CHECK(hipMemcpy(params, mlp->params.as<void>(), spo_dev::kMlpParams * sizeof(float), hipMemcpyDeviceToHost));
+  });
+}
+
+int spo_hip_mlp_gradient(spo_hip_mlp* mlp, const float* x, const float* t, float* loss, float* grad) {
+  return guarded([&] {
+    if (!mlp || !x || !t || !loss || !grad) throw Status(SPO_HIP_E_ARGUMENT, "spo_hip_mlp_gradient: NULL argument");
+    const long rows = mlp->batch;
+    HIP_CHECK(hipMemcpy(mlp->x.as<void>(), x, rows * spo_dev::kMlpIn * sizeof(float), hipMemcpyHostToDevice));
+    HIP_CHECK(hipMemcpy(mlp->t.as<void>(), t, rows * sizeof(float), hipMemcpyHostToDevice));
+    Buffer l(sizeof(float));
+    mlp_forward(*mlp, rows, mlp->x.as<float>());
+    mlp_backward(*mlp, rows, mlp->x.as<float>(), mlp->t.as<float>(), l.as<float>());
+    HIP_CHECK(hipMemcpy(loss, l.as<void>(), sizeof(float), hipMemcpyDeviceToHost));
+    HIP_CHECK(hipMemcpy(grad, mlp->grad.as<void>(), spo_dev::kMlpParams * sizeof(float), hipMemcpyDeviceToHost));
+  });
+}
+
+int spo_hip_mlp_train(spo_hip_mlp* mlp, long n, const float* x, const float* t, int steps, float lr, float* losses) {
+  return guarded([&] {
+    if (!mlp || !x || !t || !losses) throw Status(SPO_HIP_E_ARGUMENT, "spo_hip_mlp_train: NULL argument");
+    if (n < mlp->batch || n % mlp->batch != 0 || steps < 0)
+      throw Status(SPO_HIP_E_ARGUMENT, "spo_hip_mlp_train: n must be a positive multiple of the batch");
+    Buffer dx = upload(x, std::size_t(n) * spo_dev::kMlpIn * sizeof(float));
+    Buffer dt = upload(t, std::size_t(n) * sizeof(float));
+    Buffer dl(std::size_t(std::max(steps, 1)) * sizeof(float));
+    const long rows = mlp->batch;
+    for (int s = 0; s < steps; ++s) {
+      const long off = (static_cast<long>(s) * rows) % n;
+      const float* xs = dx.as<float>() + off * spo_dev::kMlpIn;
+      mlp_forward(*mlp, rows, xs);
+      mlp_backward(*mlp, rows, xs, dt.as<float>() + off, dl.as<float>() + s);
+      mlp_adam(*mlp, lr);
+    }
+    if (steps > 0) HIP_CHECK(hipMemcpy(losses, dl.as<void>(), steps * sizeof(float), hipMemcpyDeviceToHost));
+  });
+}
+
+int spo_hip_mlp_predict(spo_hip_mlp* mlp, long n, const float* x, float* y) {
+  return guarded([&] {
+    if (!mlp || (n > 0 && (!x || !y)) || n < 0) throw Status(SPO_HIP_E_ARGUMENT, "spo_hip_mlp_predict: bad arguments");
+    for (long done = 0; done < n;) {
+      const long chunk = std::min(mlp->capacity, n - done);
+      const long rows = (chunk + 15) / 16 * 16;  // the tail padded with zero rows
+      HIP_CHECK(hipMemset(mlp->x.as<void>(), 0, rows * spo_dev::kMlpIn * sizeof(float)));
+      HIP_CHECK(hipMemcpy(mlp->x.as<void>(), x + done * spo_dev::kMlpIn, chunk * spo_dev::kMlpIn * sizeof(float),
+                          hipMemcpyHostToDevice));
+      mlp_forward(*mlp, rows, mlp->x.as<float>());
+      HIP_CHECK(hipMemcpy(y + done, mlp->y.as<void>(), chunk * sizeof(float), hipMemcpyDeviceToHost));
+      done += chunk;
+    }
   });
 }
 

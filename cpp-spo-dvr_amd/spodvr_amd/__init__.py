@@ -272,6 +272,84 @@ class ScmSurface:
                     sin_skew=float(out[3]), atoms=int(out[4]))
 
 
+def _hip_library() -> ctypes.CDLL:
+    host_library()  # loads libspo_hip.so with it
+    lib = ctypes.CDLL(library_path("libspo_hip.so"))
+    lib.spo_hip_last_error.restype = ctypes.c_char_p
+    lib.spo_hip_mlp_create.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]
+    lib.spo_hip_mlp_destroy.argtypes = [ctypes.c_void_p]
+    lib.spo_hip_mlp_set_params.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+    lib.spo_hip_mlp_get_params.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+    lib.spo_hip_mlp_gradient.argtypes = [ctypes.c_void_p] + [ctypes.c_void_p] * 4
+    lib.spo_hip_mlp_train.argtypes = [ctypes.c_void_p, ctypes.c_long, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
+                                      ctypes.c_float, ctypes.c_void_p]
+    lib.spo_hip_mlp_predict.argtypes = [ctypes.c_void_p, ctypes.c_long, ctypes.c_void_p, ctypes.c_void_p]
+    return lib
+
+
+class Mlp:
+    """The potential surrogate on the GPU (spo_hip_mlp_*): 6 features -> 128 -> 128 -> 128 -> 1, SiLU, hidden layers
+    on bf16 MFMA. Parameters are a float32 vector of MLP_PARAMS values (include/spo_hip.h)."""
+
+    PARAMS = 34049
+
+    def __init__(self, batch: int):
+        self._lib = _hip_library()
+        handle = ctypes.c_void_p()
+        self._check(self._lib.spo_hip_mlp_create(batch, ctypes.byref(handle)))
+        self._handle = handle
+        self.batch = batch
+
+    def _check(self, status: int) -> None:
+        if status != 0:
+            raise SpoError(status, self._lib.spo_hip_last_error().decode(errors="replace"))
+
+    def close(self) -> None:
+        if self._handle:
+            self._lib.spo_hip_mlp_destroy(self._handle)
+            self._handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def params(self) -> np.ndarray:
+        out = np.empty(self.PARAMS, dtype=np.float32)
+        self._check(self._lib.spo_hip_mlp_get_params(self._handle, _ptr(out)))
+        return out
+
+    @params.setter
+    def params(self, p) -> None:
+        a = np.ascontiguousarray(p, dtype=np.float32)
+        if a.size != self.PARAMS:
+            raise ValueError(f"expected {self.PARAMS} parameters")
+        self._check(self._lib.spo_hip_mlp_set_params(self._handle, _ptr(a)))
+
+    def gradient(self, x, t):
+        xs = np.ascontiguousarray(x, dtype=np.float32).reshape(self.batch, 6)
+        ts = np.ascontiguousarray(t, dtype=np.float32).reshape(self.batch)
+        loss = np.zeros(1, dtype=np.float32)
+        grad = np.empty(self.PARAMS, dtype=np.float32)
+        self._check(self._lib.spo_hip_mlp_gradient(self._handle, _ptr(xs), _ptr(ts), _ptr(loss), _ptr(grad)))
+        return float(loss[0]), grad
+
+    def train(self, x, t, steps: int, lr: float) -> np.ndarray:
+        xs = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, 6)
+        ts = np.ascontiguousarray(t, dtype=np.float32).reshape(-1)
+        losses = np.empty(max(steps, 1), dtype=np.float32)
+        self._check(self._lib.spo_hip_mlp_train(self._handle, len(xs), _ptr(xs), _ptr(ts), steps, lr, _ptr(losses)))
+        return losses[:steps]
+
+    def predict(self, x) -> np.ndarray:
+        xs = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, 6)
+        out = np.empty(len(xs), dtype=np.float32)
+        self._check(self._lib.spo_hip_mlp_predict(self._handle, len(xs), _ptr(xs), _ptr(out)))
+        return out
+
+
 class Run:
     """A propagation run: the set-up from a JSON input (plus overrides) and a backend.
 

@@ -232,6 +232,24 @@ int spo_hip_crp_values(const spo_hip_crp_surface* surface, long n, const double*
 int spo_hip_scm_values(const spo_hip_crp_surface* surface, const spo_hip_scm_correction* scm, long n,
                        const double* points, double* values);
 
+/* The potential surrogate: an MLP of 6 features -> 128 -> 128 -> 128 -> 1 with SiLU whose hidden layers run on
+ * bf16 MFMA with FP32 accumulation, trained with Adam on the mean squared error. Parameters (FP32, this order):
+ * W1 (6 x 128, row per input), b1 (128), W2 (128 x 128, row per input unit), b2, W3, b3, W4 (128), b4 (1).
+ * x: 6 floats per row; batch: the rows of one training step, a multiple of 16. No counterpart in the reference:
+ * the surrogate of potential grids for batches of distorted surfaces. */
+enum { SPO_HIP_MLP_PARAMS = 34049 };
+typedef struct spo_hip_mlp spo_hip_mlp;
+int spo_hip_mlp_create(int batch, spo_hip_mlp** out); /* parameters 0: set them before use */
+int spo_hip_mlp_destroy(spo_hip_mlp* mlp);
+int spo_hip_mlp_set_params(spo_hip_mlp* mlp, const float* params); /* also clears Adam's moments */
+int spo_hip_mlp_get_params(const spo_hip_mlp* mlp, float* params);
+/* loss and gradient of the mean squared error over one batch of rows */
+int spo_hip_mlp_gradient(spo_hip_mlp* mlp, const float* x, const float* t, float* loss, float* grad);
+/* `steps` Adam steps over consecutive batches of the n rows (n a multiple of the batch; wraps around); losses[s]:
+ * the batch loss of step s before its update */
+int spo_hip_mlp_train(spo_hip_mlp* mlp, long n, const float* x, const float* t, int steps, float lr, float* losses);
+int spo_hip_mlp_predict(spo_hip_mlp* mlp, long n, const float* x, float* y);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,112 @@
+"""The potential surrogate's network on the GPU (csrc/kernels_mlp.hpp through spo_hip_mlp_*): the gradient of one
+batch against the CPU oracle (oracle/mlp.py, which rounds the hidden layers' matrix operands to bf16 as the
+matrix cores take them, so only FP32 accumulation separates the two: 2e-3 of each block's largest entry), the
+prediction against the oracle's forward pass, and training on CRP energies. Every test prints its case first."""
+
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import oracle.mlp as mlp
+from tests import cases
+
+pytestmark = pytest.mark.gpu
+
+
+def _note(*what):
+    print("mlp:", *what, file=sys.stderr, flush=True)
+
+
+def test_gradient_matches_oracle():
+    import spodvr_amd
+    _note("gradient, batch 256")
+    net = spodvr_amd.Mlp(256)
+    try:
+        p = mlp.initial_parameters(3)
+        p[mlp.COUNT - 1] = 0.25
+        net.params = p
+        rng = np.random.default_rng(4)
+        x = rng.normal(size=(256, 6)).astype(np.float32)
+        t = rng.normal(size=256).astype(np.float32)
+        loss, grad = net.gradient(x, t)
+        ref_loss, ref_grad = mlp.loss_and_gradient(p.astype(np.float64), x, t)
+        assert abs(loss - ref_loss) <= 1e-5 * abs(ref_loss)
+        offsets = np.cumsum([0] + [int(np.prod(s)) for s in mlp.SHAPES])
+        for k in range(len(mlp.SHAPES)):
+            a, b = grad[offsets[k]:offsets[k + 1]], ref_grad[offsets[k]:offsets[k + 1]]
+            scale = np.max(np.abs(b))
+            assert np.max(np.abs(a - b)) <= 2e-3 * scale, (k, np.max(np.abs(a - b)) / scale)
+        y = net.predict(x[:100])
+        ref_y, _ = mlp.forward(p.astype(np.float64), x[:100])
+        assert np.max(np.abs(y - ref_y)) <= 1e-4 * max(1.0, np.max(np.abs(ref_y)))
+    finally:
+        net.close()
+
+
+def test_training_on_crp_energies_converges():
+    import spodvr_amd
+    from spodvr_amd import surrogate
+    _note("training on CRP energies, SRP48 data set")
+    crp = spodvr_amd.CrpSurface(os.path.join(cases.GOLDEN_DIR, "crp", "cu111_srp48", "dataset"))
+    net = spodvr_amd.Mlp(1024)
+    try:
+        pts = surrogate.sample_points(crp.parameters(), 65536, seed=1)
+        v = crp.device_qd(pts)
+        x = surrogate.features(pts, crp.parameters()["lattice_constant"])
+        t = surrogate.target(v)
+        net.params = mlp.initial_parameters(5)
+        losses = net.train(x, t, steps=4000, lr=2e-3)
+        assert np.all(np.isfinite(losses))
+        first, last = float(np.mean(losses[:50])), float(np.mean(losses[-50:]))
+        _note(f"loss {first:.4g} -> {last:.4g}")
+        assert last < 0.35 * first
+    finally:
+        net.close()
+        crp.close()
+
+
+@pytest.mark.parametrize("tolerance", [0.01, 0.3])
+def test_surrogate_grid_with_crp_fallback(tolerance):
+    """The surrogate on the potential grid of crp_cu111_srp48_small's shape (6 x 6 (x, y), 3 x 10 orientations,
+    86 Z x 48 r) against the full CRP grid: the slabs it does not trust hold the CRP's values exactly, and the
+    values it serves stay near the trust tolerance (a slab is trusted from every 16th point). At 10 meV it serves
+    nothing (validation RMS ~0.4 eV with six features); at 0.3 eV it serves part of the grid."""
+    import math
+    import time
+
+    import spodvr_amd
+    from spodvr_amd import surrogate
+    _note("surrogate grid with CRP fallback, SRP48 data set, tolerance", tolerance, "eV")
+    crp = spodvr_amd.CrpSurface(os.path.join(cases.GOLDEN_DIR, "crp", "cu111_srp48", "dataset"))
+    net = spodvr_amd.Mlp(4096)
+    try:
+        a = crp.parameters()["lattice_constant"]
+        net.params = mlp.initial_parameters(5)
+        surrogate.train(net, crp)
+        u = np.arange(6) * a / 6
+        thetas = np.array([0.436634949225522206, 1.00217680364312156, 1.57079632679489656])
+        phis = np.arange(10) * 2 * math.pi / 10
+        z = -1.0 + 0.15 * np.arange(86)
+        r = 0.4 + 0.2 * np.arange(48)
+        zz, rr, uu, vv, tt, pp = np.meshgrid(z, r, u, u, thetas, phis, indexing="ij")
+        pts = np.column_stack([uu.ravel(), vv.ravel(), zz.ravel(), rr.ravel(), tt.ravel(), pp.ravel()])
+        slab = 6 * 6 * 3 * 10
+        t0 = time.perf_counter()
+        full = crp.device_qd(pts)
+        t_crp = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        values, served, trusted = surrogate.grid_values(pts, slab, a, net, crp.device_qd, tolerance=tolerance)
+        t_sur = time.perf_counter() - t0
+        err = np.abs(values - full)
+        _note(f"served {served.mean():.3f} of the points, {trusted.mean():.3f} of the slabs; max error served "
+              f"{err[served].max() if served.any() else 0:.4g} eV; CRP grid {t_crp:.3f} s, surrogate grid {t_sur:.3f} s")
+        assert np.all(err[~served] == 0.0)
+        if tolerance >= 0.3:
+            assert served.any()
+        if served.any():
+            assert np.quantile(err[served], 0.99) <= 2 * tolerance
+    finally:
+        net.close()
+        crp.close()
