@@ -1110,6 +1110,29 @@ int spo_hip_propagate_analysis_step(spo_hip_backend* b) {
   });
 }
 
+int spo_hip_propagate_batch(spo_hip_backend* const* backends, int n) {
+  return guarded([&] {
+    if (n < 0 || (n > 0 && !backends)) throw Status(SPO_HIP_E_ARGUMENT, "spo_hip_propagate_batch: bad arguments");
+    for (int i = 0; i < n; ++i) {
+      if (!backends[i]) throw Status(SPO_HIP_E_ARGUMENT, "spo_hip_propagate_batch: NULL backend");
+      for (int j = 0; j < i; ++j)
+        if (backends[j] == backends[i]) throw Status(SPO_HIP_E_ARGUMENT, "spo_hip_propagate_batch: a backend twice");
+    }
+    // Block r of every run's schedule [T/2 V], (n - 1) [T V], [T/2], round robin over the runs: each run's launches
+    // stay in its own order on its own streams, and the runs' kernels overlap where one alone does not fill the GPU.
+    int longest = 0;
+    for (int i = 0; i < n; ++i) longest = std::max(longest, backends[i]->shape.steps_per_analysis + 1);
+    for (int r = 0; r < longest; ++r)
+      for (int i = 0; i < n; ++i) {
+        spo_hip_backend* b = backends[i];
+        const int blocks = b->shape.steps_per_analysis + 1;
+        if (r >= blocks) continue;
+        b->kinetic_and_potential(r == 0 || r == blocks - 1, r == blocks - 1);
+      }
+    for (int i = 0; i < n; ++i) HIP_CHECK(hipStreamSynchronize(backends[i]->stream));
+  });
+}
+
 int spo_hip_get_specular(spo_hip_backend* b, double* out) {
   return guarded([&] {
     HIP_CHECK(hipMemcpyAsync(out, b->spec.as<void>(), std::size_t(b->s.nr) * b->s.nzsp * sizeof(cx<double>),

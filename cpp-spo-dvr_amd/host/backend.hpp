@@ -104,4 +104,8 @@ std::vector<cplx> exp_potential_table(const PotentialGrid& grid, double dt);
 // The backend on the GPU (hip_backend.cpp); throws DeviceError / NotSupportedError when no device is usable.
 std::unique_ptr<PropagationBackend> make_hip_backend(const BackendInput& input);
 
+// One analysis step of each backend: the HIP backends' steps enqueued together on their own streams
+// (spo_hip_propagate_batch), the others one after another.
+void propagate_batch(const std::vector<PropagationBackend*>& backends);
+
 }  // namespace spo

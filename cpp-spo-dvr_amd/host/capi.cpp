@@ -141,6 +141,18 @@ int spo_run_steps(spo_run* run, int n) {
   return guarded([&] { run->propagation->run(n); });
 }
 
+int spo_run_steps_batch(spo_run* const* runs, int n, int n_analysis_steps) {
+  return guarded([&] {
+    if (n < 0 || (n > 0 && !runs)) throw spo::InputError("runs", "bad arguments");
+    std::vector<spo::Propagation*> p;
+    for (int i = 0; i < n; ++i) {
+      if (!runs[i]) throw spo::InputError("runs", "NULL run");
+      p.push_back(runs[i]->propagation.get());
+    }
+    spo::Propagation::run_batch(p, n_analysis_steps);
+  });
+}
+
 int spo_run_propagate(spo_run* run) {
   return guarded([&] { run->propagation->backend().propagate_analysis_step(); });
 }

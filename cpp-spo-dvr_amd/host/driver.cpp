@@ -150,6 +150,25 @@ void Propagation::record_averages(double t) {
 
 void Propagation::step() {
   backend_->propagate_analysis_step();
+  finish_step();
+}
+
+void Propagation::run_batch(const std::vector<Propagation*>& runs, int max_steps) {
+  for (int i = 0; i < max_steps; ++i) {
+    std::vector<Propagation*> live;
+    std::vector<PropagationBackend*> backends;
+    for (Propagation* p : runs)
+      if (p->done_ < p->analysis_steps_total()) {
+        live.push_back(p);
+        backends.push_back(p->backend_.get());
+      }
+    if (live.empty()) return;
+    propagate_batch(backends);
+    for (Propagation* p : live) p->finish_step();
+  }
+}
+
+void Propagation::finish_step() {
   done_ += 1;
   const double t = time();
   if (surface_.update(backend_->specular()) && !fixed_time_) fixed_time_ = t;

@@ -53,6 +53,9 @@ class Propagation {
   // and, every dtAve, the averages.
   void step();
   void run(int max_analysis_steps);
+  // Up to max_analysis_steps analysis steps of every run not yet done, in lockstep: the propagations of one step
+  // together (propagate_batch), then each run's analysis. Each run's results equal those of run().
+  static void run_batch(const std::vector<Propagation*>& runs, int max_analysis_steps);
 
   // Averages at tStart (before any step) and every dtAve.
   const std::vector<Averages>& averages() const { return averages_; }
@@ -74,6 +77,8 @@ class Propagation {
   void energy_resolved(std::vector<cplx>* sums, std::vector<cplx>* specular_sums);
 
  private:
+  // the analysis part of step(), after the backend has propagated
+  void finish_step();
   void record_averages(double time);
   void restore(const std::string& dir);
 

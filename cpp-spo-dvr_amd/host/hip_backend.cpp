@@ -36,6 +36,8 @@ class HipBackend final : public PropagationBackend {
                                 reinterpret_cast<double*>(host_.specular.data())));
     return host_;
   }
+  spo_hip_backend* handle() const { return handle_; }
+  void refresh_specular() { check(spo_hip_get_specular(handle_, reinterpret_cast<double*>(specular_.data()))); }
   void propagate_analysis_step() override {
     check(spo_hip_propagate_analysis_step(handle_));
     check(spo_hip_get_specular(handle_, reinterpret_cast<double*>(specular_.data())));
@@ -237,6 +239,22 @@ HipBackend::HipBackend(const BackendInput& in) {
 }
 
 }  // namespace
+
+void propagate_batch(const std::vector<PropagationBackend*>& backends) {
+  std::vector<HipBackend*> hip;
+  std::vector<spo_hip_backend*> handles;
+  for (PropagationBackend* b : backends) {
+    if (auto* h = dynamic_cast<HipBackend*>(b)) {
+      hip.push_back(h);
+      handles.push_back(h->handle());
+    } else {
+      b->propagate_analysis_step();
+    }
+  }
+  if (handles.empty()) return;
+  check(spo_hip_propagate_batch(handles.data(), static_cast<int>(handles.size())));
+  for (HipBackend* h : hip) h->refresh_specular();
+}
 
 std::unique_ptr<PropagationBackend> make_hip_backend(const BackendInput& input) {
   return std::make_unique<HipBackend>(input);

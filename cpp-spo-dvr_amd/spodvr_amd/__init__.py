@@ -69,6 +69,7 @@ def host_library() -> ctypes.CDLL:
         for fn in ("spo_run_steps",):
             getattr(lib, fn).argtypes = [ctypes.c_void_p, ctypes.c_int]
         lib.spo_run_propagate.argtypes = [ctypes.c_void_p]
+        lib.spo_run_steps_batch.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_int, ctypes.c_int]
         lib.spo_run_kinetic_and_potential.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]
         lib.spo_run_analyse.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_void_p]
         lib.spo_run_get_wave.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
@@ -348,6 +349,13 @@ class Mlp:
         out = np.empty(len(xs), dtype=np.float32)
         self._check(self._lib.spo_hip_mlp_predict(self._handle, len(xs), _ptr(xs), _ptr(out)))
         return out
+
+
+def run_batch(runs, n_analysis_steps: int) -> None:
+    """Up to n_analysis_steps analysis steps of every run in lockstep (spo_run_steps_batch): on the GPU the runs'
+    time steps are enqueued together, each run's results unchanged."""
+    handles = (ctypes.c_void_p * len(runs))(*[r._h for r in runs])
+    _check(host_library().spo_run_steps_batch(handles, len(runs), n_analysis_steps))
 
 
 class Run:

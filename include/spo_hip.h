@@ -113,6 +113,11 @@ int spo_hip_download_wave(spo_hip_backend* backend, double* scattering, double* 
 /* T(dt/2) V, (T V) x (n-1), T(dt/2) (src/propagation/split_operator_schedule.cpp:7-13). Replaces
  * propagate_analysis_step (src/propagation/propagation_backend.hpp:47-48). */
 int spo_hip_propagate_analysis_step(spo_hip_backend* backend);
+/* One analysis step of each of n distinct backends (independent runs on this device): their time-step blocks are
+ * enqueued round robin, each run's in its own order on its own streams, so that runs too small to fill the GPU
+ * overlap; returns when all are done. Each run's results are those of spo_hip_propagate_analysis_step. No
+ * counterpart in the reference: the batch of initial states or distorted surfaces of one device. */
+int spo_hip_propagate_batch(spo_hip_backend* const* backends, int n);
 /* One T block: specular grid, T_Zr, T_xy, then V unless kinetic_only (src/propagation/split_operator.cpp:17-21);
  * the unit the reference's CUDA tests check stage by stage (tests/cuda/test_cuda_backend.cpp:94-106). */
 int spo_hip_kinetic_and_potential(spo_hip_backend* backend, int half_step, int kinetic_only);

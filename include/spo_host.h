@@ -56,6 +56,10 @@ void* spo_run_backend(spo_run* run);
 
 /* Analysis steps of the time loop: propagate, move the analysis surface, project and accumulate. */
 int spo_run_steps(spo_run* run, int n_analysis_steps);
+/* Up to n_analysis_steps analysis steps of n runs in lockstep: the propagations of one step together (the HIP
+ * runs' launches interleaved on their own streams, spo_hip_propagate_batch), then each run's analysis; each run's
+ * results equal spo_run_steps'. No counterpart in the reference: a batch of independent runs on one device. */
+int spo_run_steps_batch(spo_run* const* runs, int n, int n_analysis_steps);
 /* The backend's propagate_analysis_step() alone (no analysis). */
 int spo_run_propagate(spo_run* run);
 /* One T block of the schedule (specular grid, T_Zr, T_xy, then V unless kinetic_only) on the backend. */
