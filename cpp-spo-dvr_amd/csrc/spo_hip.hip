@@ -262,6 +262,10 @@ struct spo_hip_backend {
   // scattering grid; `stream` waits for spec_ready before anything that touches the specular wave again.
   hipStream_t side = nullptr;
   hipEvent_t main_ready = nullptr, spec_ready = nullptr;
+  // Graph mode (spo_hip_graph_enable): one analysis step captured from `stream` (with `side` joined through the
+  // events) at its first use, then replayed. The step's launches and arguments do not change between steps.
+  bool use_graph = false;
+  hipGraphExec_t step_graph = nullptr;
   int max_lds = 65536;
   spo_hip_analysis analysis{};
 
@@ -319,6 +323,7 @@ struct spo_hip_backend {
       (void)hipEventDestroy(t.stop);
     }
     for (hipEvent_t e : event_pool) (void)hipEventDestroy(e);
+    if (step_graph) (void)hipGraphExecDestroy(step_graph);
     if (side) {
       (void)hipStreamSynchronize(side);
       (void)hipStreamDestroy(side);
@@ -1101,11 +1106,56 @@ int spo_hip_kinetic_and_potential(spo_hip_backend* b, int half_step, int kinetic
   });
 }
 
+}  // extern "C"
+
+namespace {
+
+// The blocks of one analysis step, [T/2 V], (n - 1) [T V], [T/2], enqueued on the backend's streams.
+void enqueue_blocks(spo_hip_backend* b) {
+  b->kinetic_and_potential(true, false);
+  for (int k = 1; k < b->shape.steps_per_analysis; ++k) b->kinetic_and_potential(false, false);
+  b->kinetic_and_potential(true, true);
+}
+
+// One analysis step on the backend's stream: the blocks, or in graph mode the captured step (captured here the
+// first time).
+void enqueue_analysis_step(spo_hip_backend* b) {
+  if (!b->use_graph) {
+    enqueue_blocks(b);
+    return;
+  }
+  if (!b->step_graph) {
+    hipGraph_t graph = nullptr;
+    HIP_CHECK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+    enqueue_blocks(b);
+    HIP_CHECK(hipStreamEndCapture(b->stream, &graph));
+    const hipError_t e = hipGraphInstantiate(&b->step_graph, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    HIP_CHECK(e);
+  }
+  HIP_CHECK(hipGraphLaunch(b->step_graph, b->stream));
+}
+
+}  // namespace
+
+extern "C" {
+
+int spo_hip_graph_enable(spo_hip_backend* b, int on) {
+  return guarded([&] {
+    if (!b) throw Status(SPO_HIP_E_ARGUMENT, "spo_hip_graph_enable: NULL backend");
+    if (on && b->profiling) throw Status(SPO_HIP_E_ARGUMENT, "spo_hip_graph_enable: kernel timing is on");
+    HIP_CHECK(hipStreamSynchronize(b->stream));
+    if (b->step_graph) {
+      HIP_CHECK(hipGraphExecDestroy(b->step_graph));
+      b->step_graph = nullptr;
+    }
+    b->use_graph = on != 0;
+  });
+}
+
 int spo_hip_propagate_analysis_step(spo_hip_backend* b) {
   return guarded([&] {
-    b->kinetic_and_potential(true, false);
-    for (int k = 1; k < b->shape.steps_per_analysis; ++k) b->kinetic_and_potential(false, false);
-    b->kinetic_and_potential(true, true);
+    enqueue_analysis_step(b);
     HIP_CHECK(hipStreamSynchronize(b->stream));
   });
 }
@@ -1120,13 +1170,17 @@ int spo_hip_propagate_batch(spo_hip_backend* const* backends, int n) {
     }
     // Block r of every run's schedule [T/2 V], (n - 1) [T V], [T/2], round robin over the runs: each run's launches
     // stay in its own order on its own streams, and the runs' kernels overlap where one alone does not fill the GPU.
+    // Runs in graph mode launch their captured step; the others' blocks go round robin.
+    for (int i = 0; i < n; ++i)
+      if (backends[i]->use_graph) enqueue_analysis_step(backends[i]);
     int longest = 0;
-    for (int i = 0; i < n; ++i) longest = std::max(longest, backends[i]->shape.steps_per_analysis + 1);
+    for (int i = 0; i < n; ++i)
+      if (!backends[i]->use_graph) longest = std::max(longest, backends[i]->shape.steps_per_analysis + 1);
     for (int r = 0; r < longest; ++r)
       for (int i = 0; i < n; ++i) {
         spo_hip_backend* b = backends[i];
         const int blocks = b->shape.steps_per_analysis + 1;
-        if (r >= blocks) continue;
+        if (b->use_graph || r >= blocks) continue;
         b->kinetic_and_potential(r == 0 || r == blocks - 1, r == blocks - 1);
       }
     for (int i = 0; i < n; ++i) HIP_CHECK(hipStreamSynchronize(backends[i]->stream));
@@ -1248,6 +1302,7 @@ int spo_hip_energy_resolved_read(spo_hip_backend* b, double* sums, double* specu
 
 int spo_hip_profile_enable(spo_hip_backend* b, int on) {
   return guarded([&] {
+    if (on && b->use_graph) throw Status(SPO_HIP_E_ARGUMENT, "spo_hip_profile_enable: graph mode is on");
     HIP_CHECK(hipStreamSynchronize(b->stream));
     for (const auto& t : b->timed_launches) {
       b->event_pool.push_back(t.start);

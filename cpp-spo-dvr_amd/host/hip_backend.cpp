@@ -1,6 +1,7 @@
 // The HIP backend as a PropagationBackend: packs the set-up into the plain tables of include/spo_hip.h and
 // forwards every seam call to libspo_hip.so. Status codes of the C ABI become the exceptions the reference's
 // driver expects (device errors -> DeviceError, no device / too little memory -> NotSupportedError).
+#include <cstdlib>
 #include <cmath>
 #include <cstring>
 
@@ -225,6 +226,8 @@ HipBackend::HipBackend(const BackendInput& in) {
   an.total_energy_hartree = in.energies.total_hartree.data();
 
   check(spo_hip_create(&shape, &tb, &an, d(in.initial_specular), &handle_));
+  // SPO_HIP_GRAPH=1: analysis steps replayed from a captured hipGraph (spo_hip_graph_enable)
+  if (const char* g = std::getenv("SPO_HIP_GRAPH"); g && std::string(g) == "1") check(spo_hip_graph_enable(handle_, 1));
   host_.n_rot = n_rot;
   host_.n_xy = n_xy;
   host_.n_zr = nr * p.nz;

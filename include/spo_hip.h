@@ -118,6 +118,10 @@ int spo_hip_propagate_analysis_step(spo_hip_backend* backend);
  * overlap; returns when all are done. Each run's results are those of spo_hip_propagate_analysis_step. No
  * counterpart in the reference: the batch of initial states or distorted surfaces of one device. */
 int spo_hip_propagate_batch(spo_hip_backend* const* backends, int n);
+/* Graph mode on (1) or off (0): an analysis step is captured into a hipGraph at its next call (the T/V blocks on the
+ * backend's two streams; the step's launches and arguments are the same every step) and replayed from then on,
+ * with the results of the uncaptured step. Not with kernel timing. No counterpart in the reference. */
+int spo_hip_graph_enable(spo_hip_backend* backend, int on);
 /* One T block: specular grid, T_Zr, T_xy, then V unless kinetic_only (src/propagation/split_operator.cpp:17-21);
  * the unit the reference's CUDA tests check stage by stage (tests/cuda/test_cuda_backend.cpp:94-106). */
 int spo_hip_kinetic_and_potential(spo_hip_backend* backend, int half_step, int kinetic_only);
