@@ -552,7 +552,8 @@ int spo_crp_parameters(const spo_crp* crp, double* out) {
   return guarded([&] {
     const spo::crp::MoleculeSurface& s = crp->qd->surface();
     const double v[9] = {s.lattice_constant(), s.z_asymptotic(), crp->qd->skewing_angle(), s.zso_min(), s.zso_max(),
-                         s.r_min(), s.r_max(), s.z_min(), s.symmetry() == spo::crp::Symmetry::c3v ? 3.0 : 6.0};
+                         s.r_min(), s.r_max(), s.z_min(),
+                         s.symmetry() == spo::crp::Symmetry::c3v ? 3.0 : (s.symmetry() == spo::crp::Symmetry::c4v ? 4.0 : 6.0)};
     for (int i = 0; i < 9; ++i) out[i] = v[i];
   });
 }

@@ -33,7 +33,7 @@ SystemParameters system_parameters_for(const std::string& id, const std::string&
     const auto crp = load_crp_surface(crp_data_set);
     const crp::MoleculeSurface& s = crp->surface();
     return {s.lattice_constant(), s.lattice_constant(), s.z_asymptotic(), crp->skewing_angle(),
-            s.symmetry() == crp::Symmetry::c3v ? "C3v" : "C6v"};
+            s.symmetry() == crp::Symmetry::c3v ? "C3v" : (s.symmetry() == crp::Symmetry::c4v ? "C4v" : "C6v")};
   }
   throw NotSupportedError("SystemID " + id + " is not supported by this backend's set-up");
 }

@@ -125,7 +125,7 @@ DataSet read_data_set(const std::string& directory) {
     } else if (sym == "C6v") {
       d.symmetry = Symmetry::c6v;
     } else if (sym == "C4v") {
-      throw DataError(crpdata.string() + ": C4v data sets are not supported by this library (C3v, C6v)");
+      d.symmetry = Symmetry::c4v;
     } else {
       throw DataError(crpdata.string() + ": unknown symmetry '" + sym + "'");
     }
@@ -226,10 +226,17 @@ DataSet read_data_set(const std::string& directory) {
   if (d.symmetry == Symmetry::c3v && d.displacement_1 != 0.0)
     throw DataError("CRPData: the C3v atom potential does not support displacements");
   check_grid(d.corrugation.x, "Atom/intrep.dat");
-  if (static_cast<int>(d.atom_cuts.size()) != atom_cuts_needed(d.symmetry))
-    throw DataError("CRPData.3D: needs exactly " + std::to_string(atom_cuts_needed(d.symmetry)) +
-                    " atom-surface cuts, the data set has " + std::to_string(d.atom_cuts.size()));
-  for (const AtomCut& cut : d.atom_cuts) check_grid(cut.v.x, "Atom/" + cut.file);
+  if (d.symmetry == Symmetry::c4v && d.interlayer_spacings.empty())
+    throw DataError("CRPData: C4v needs at least 2 layers (the atom cluster uses the first interlayer spacing)");
+  const int n_atom = static_cast<int>(d.atom_cuts.size()), need = atom_cuts_needed(d.symmetry);
+  if (d.symmetry == Symmetry::c4v ? n_atom < need : n_atom != need)
+    throw DataError("CRPData.3D: needs " + std::string(d.symmetry == Symmetry::c4v ? "at least " : "exactly ") +
+                    std::to_string(need) + " atom-surface cuts, the data set has " + std::to_string(n_atom));
+  for (const AtomCut& cut : d.atom_cuts) {
+    check_grid(cut.v.x, "Atom/" + cut.file);
+    if (d.symmetry == Symmetry::c4v && cut.v.x != d.atom_cuts.front().v.x)
+      throw DataError("Atom/" + cut.file + ": C4v needs the same z values for all sites");
+  }
   if (d.zso_min > d.zso_max) throw DataError("Molecule/extrapol.dat: ZsoMin is larger than ZsoMax");
   check_grid(d.z_dependence.x, "Molecule/extrapol.dat");
   check_grid(d.h2.x, "Molecule/VH2.dat");
@@ -345,6 +352,59 @@ int bisect(const std::vector<double>& x, double t) {
     }
   }
   return lo;
+}
+
+// Cyclic tridiagonal solve (TRICYCLE, src/crplib/spline.cpp:25-44): the tridiagonal system plus the corner elements
+// bottom_left (row n, column 1) and top_right (row 1, column n) by a rank-one correction; diag and rhs overwritten.
+void cyclic_tridiagonal(std::vector<double>& diag, const std::vector<double>& sub, const std::vector<double>& sup,
+                        double bottom_left, double top_right, std::vector<double>& rhs, double* out) {
+  const std::size_t n = diag.size();
+  const double alp = -diag[0];
+  diag[0] = diag[0] - alp;
+  diag[n - 1] = diag[n - 1] - bottom_left * top_right / alp;
+  std::vector<double> z(diag);
+  tridiagonal(diag, sub, sup, rhs, out);
+  diag = z;
+  rhs[0] = alp;
+  rhs[n - 1] = bottom_left;
+  for (std::size_t i = 1; i + 1 < n; ++i) rhs[i] = 0.0;
+  tridiagonal(diag, sub, sup, rhs, z.data());
+  const double fact = (out[0] + top_right * out[n - 1] / alp) / (1.0 + z[0] + top_right * z[n - 1] / alp);
+  for (std::size_t i = 0; i < n; ++i) out[i] = out[i] - fact * z[i];
+}
+
+// F'' of the periodic spline through (x_i, y_i), period x_{n-1} - x_0 (TRISPL, src/crplib/spline.cpp:148-178).
+std::vector<double> periodic_second_derivatives(const double* x, const double* y, std::size_t n) {
+  if (n < 4) throw DataError("a periodic cubic spline needs at least 4 points");
+  const std::size_t m = n - 1;
+  std::vector<double> diag(m), sub(m), sup(m), rhs(m);
+  double h_right = x[1] - x[0];
+  double slope_right = (y[1] - y[0]) / h_right;
+  for (std::size_t i = 2; i < n; ++i) {
+    const double h_left = h_right;
+    h_right = x[i] - x[i - 1];
+    if (h_left * h_right <= 0.0) throw DataError("spline abscissae are not monotonic");
+    diag[i - 2] = (h_left + h_right) / 3.0;
+    sub[i - 2] = h_right / 6.0;
+    sup[i - 2] = sub[i - 2];
+    const double slope_left = slope_right;
+    slope_right = (y[i] - y[i - 1]) / h_right;
+    rhs[i - 2] = slope_right - slope_left;
+  }
+  // the row that closes the period: the last interval followed by the first
+  const double h_left = h_right;
+  h_right = x[1] - x[0];
+  diag[m - 1] = (h_left + h_right) / 3.0;
+  sub[m - 1] = h_right / 6.0;
+  sup[m - 1] = sub[m - 1];
+  const double slope_left = slope_right;
+  slope_right = (y[1] - y[0]) / h_right;
+  rhs[m - 1] = slope_right - slope_left;
+  const double corner = sub[m - 1];
+  std::vector<double> f2(n);
+  cyclic_tridiagonal(diag, sub, sup, corner, corner, rhs, f2.data() + 1);
+  f2[0] = f2[n - 1];
+  return f2;
 }
 
 }  // namespace
@@ -487,6 +547,116 @@ Spline2D::Value Spline2D::at(const Cell& c) const {
   return {x_outer(c.wx, c.wy), y_outer(c.wx, c.sy), x_outer(c.sx, c.wy)};
 }
 
+// SPLIN3D periodic (src/crplib/periodic_spline_3d.cpp:48-105): clamped splines along z for every (x, y), then the
+// periodic 2D second derivatives (along y, along x, along x of the y ones) of F and of F_zz in every z plane.
+PeriodicSpline3D::PeriodicSpline3D(std::vector<double> x, std::vector<double> y, std::vector<double> z,
+                                   std::vector<double> f, const std::vector<double>& dfdz_first,
+                                   const std::vector<double>& dfdz_last)
+    : x_(std::move(x)), y_(std::move(y)), z_(std::move(z)), f_(std::move(f)) {
+  const std::size_t nx = x_.size(), ny = y_.size(), nz = z_.size(), n = f_.size();
+  if (n != nx * ny * nz || dfdz_first.size() != nx * ny || dfdz_last.size() != nx * ny)
+    throw DataError("3D spline: the number of values does not match the grid");
+  auto at = [&](std::size_t i, std::size_t j, std::size_t k) { return i + nx * (j + ny * k); };
+  zz_.resize(n);
+  std::vector<double> line(nz);
+  for (std::size_t i = 0; i < nx; ++i)
+    for (std::size_t j = 0; j < ny; ++j) {
+      for (std::size_t k = 0; k < nz; ++k) line[k] = f_[at(i, j, k)];
+      const auto m = second_derivatives(z_.data(), line.data(), nz,
+                                        {Spline1D::End::first_derivative, dfdz_first[i + nx * j]},
+                                        {Spline1D::End::first_derivative, dfdz_last[i + nx * j]});
+      for (std::size_t k = 0; k < nz; ++k) zz_[at(i, j, k)] = m[k];
+    }
+  for (auto* v : {&xx_, &yy_, &xxyy_, &zzxx_, &zzyy_, &zzxxyy_}) v->resize(n);
+  auto periodic_2d = [&](const std::vector<double>& g, std::vector<double>& gxx, std::vector<double>& gyy,
+                         std::vector<double>& gxxyy) {
+    std::vector<double> col(ny);
+    for (std::size_t i = 0; i < nx; ++i) {
+      for (std::size_t j = 0; j < ny; ++j) col[j] = g[i + nx * j];
+      const auto m = periodic_second_derivatives(y_.data(), col.data(), ny);
+      for (std::size_t j = 0; j < ny; ++j) gyy[i + nx * j] = m[j];
+    }
+    for (std::size_t j = 0; j < ny; ++j) {
+      const auto m = periodic_second_derivatives(x_.data(), g.data() + nx * j, nx);
+      for (std::size_t i = 0; i < nx; ++i) gxx[i + nx * j] = m[i];
+    }
+    for (std::size_t j = 0; j < ny; ++j) {
+      const auto m = periodic_second_derivatives(x_.data(), gyy.data() + nx * j, nx);
+      for (std::size_t i = 0; i < nx; ++i) gxxyy[i + nx * j] = m[i];
+    }
+  };
+  std::vector<double> plane(nx * ny), plane_zz(nx * ny), xx(nx * ny), yy(nx * ny), xxyy(nx * ny), zx(nx * ny),
+      zy(nx * ny), zxy(nx * ny);
+  for (std::size_t k = 0; k < nz; ++k) {
+    for (std::size_t j = 0; j < ny; ++j)
+      for (std::size_t i = 0; i < nx; ++i) {
+        plane[i + nx * j] = f_[at(i, j, k)];
+        plane_zz[i + nx * j] = zz_[at(i, j, k)];
+      }
+    periodic_2d(plane, xx, yy, xxyy);
+    periodic_2d(plane_zz, zx, zy, zxy);
+    for (std::size_t j = 0; j < ny; ++j)
+      for (std::size_t i = 0; i < nx; ++i) {
+        const std::size_t q = at(i, j, k), l = i + nx * j;
+        xx_[q] = xx[l];
+        yy_[q] = yy[l];
+        xxyy_[q] = xxyy[l];
+        zzxx_[q] = zx[l];
+        zzyy_[q] = zy[l];
+        zzxxyy_[q] = zxy[l];
+      }
+  }
+}
+
+namespace {
+
+// SPLINT3D weights in one direction (src/crplib/periodic_spline_3d.cpp:33-46): x**3.0 is a call of pow there.
+void weights_3d(const std::vector<double>& x, int i, double t, Spline2D::Weights& w, Spline2D::Weights& s) {
+  const double h = x[i + 1] - x[i];
+  w.a = (x[i + 1] - t) / h;
+  w.b = 1.0 - w.a;
+  w.c = (std::pow(w.a, 3.0) - w.a) / 6.0 * (h * h);
+  w.d = (std::pow(w.b, 3.0) - w.b) / 6.0 * (h * h);
+  s.a = -1.0 / h;
+  s.b = -s.a;
+  s.c = (3.0 * (w.a * w.a) * s.a - s.a) / 6.0 * (h * h);
+  s.d = (3.0 * (w.b * w.b) * s.b - s.b) / 6.0 * (h * h);
+}
+
+}  // namespace
+
+// SPLINT3D (src/crplib/periodic_spline_3d.cpp:108-165)
+std::array<double, 4> PeriodicSpline3D::evaluate(double x, double y, double z) const {
+  auto inside = [](const std::vector<double>& g, double t) { return !(t < g.front() || t > g.back()); };
+  if (!inside(x_, x) || !inside(y_, y) || !inside(z_, z))
+    throw EvaluationError("C4v atom-surface spline: point outside the interpolation grid");
+  const int i = bisect(x_, x), j = bisect(y_, y), k = bisect(z_, z);
+  Spline2D::Weights wx, sx, wy, sy, wz, sz;
+  weights_3d(x_, i, x, wx, sx);
+  weights_3d(y_, j, y, wy, sy);
+  weights_3d(z_, k, z, wz, sz);
+  const std::size_t nx = x_.size(), ny = y_.size();
+  auto plane_sum = [&](const Spline2D::Weights& ex, const Spline2D::Weights& ey, const std::vector<double>& g,
+                       const std::vector<double>& gxx, const std::vector<double>& gyy,
+                       const std::vector<double>& gxxyy, int kk) {
+    const std::size_t p00 = i + nx * (j + ny * kk), p10 = p00 + 1, p01 = p00 + nx, p11 = p01 + 1;
+    return ey.a * (ex.a * g[p00] + ex.b * g[p10] + ex.c * gxx[p00] + ex.d * gxx[p10]) +
+           ey.b * (ex.a * g[p01] + ex.b * g[p11] + ex.c * gxx[p01] + ex.d * gxx[p11]) +
+           ey.c * (ex.a * gyy[p00] + ex.b * gyy[p10] + ex.c * gxxyy[p00] + ex.d * gxxyy[p10]) +
+           ey.d * (ex.a * gyy[p01] + ex.b * gyy[p11] + ex.c * gxxyy[p01] + ex.d * gxxyy[p11]);
+  };
+  auto terms = [&](const Spline2D::Weights& ex, const Spline2D::Weights& ey) {
+    return std::array<double, 4>{plane_sum(ex, ey, f_, xx_, yy_, xxyy_, k), plane_sum(ex, ey, f_, xx_, yy_, xxyy_, k + 1),
+                                 plane_sum(ex, ey, zz_, zzxx_, zzyy_, zzxxyy_, k),
+                                 plane_sum(ex, ey, zz_, zzxx_, zzyy_, zzxxyy_, k + 1)};
+  };
+  auto combine = [](const Spline2D::Weights& e, const std::array<double, 4>& t) {
+    return e.a * t[0] + e.b * t[1] + e.c * t[2] + e.d * t[3];
+  };
+  const std::array<double, 4> base = terms(wx, wy);
+  return {combine(wz, base), combine(wz, terms(sx, wy)), combine(wz, terms(wx, sy)), combine(sz, base)};
+}
+
 // ---------------------------------------------------------------------------------------------------------------
 // Atom-surface PES (src/crplib/hexagonal_atom_surface.cpp, atom_surface.cpp)
 
@@ -506,6 +676,95 @@ int cell_index(double ori) {
   return ori >= 0.0 ? static_cast<int>(ori) : static_cast<int>(ori - 1.0);
 }
 
+// (x, y) folded into the square unit cell at the origin (ORIGEN, src/crplib/atom_surface.cpp:84-89).
+std::pair<double, double> fold_square(double x, double y, double delta) {
+  const int i1 = cell_index(x / delta);
+  const int i2 = cell_index(y / delta);
+  return {x - i1 * delta, y - i2 * delta};
+}
+
+// A point of the square cell mapped into the triangle 0 <= y <= x <= delta/2 by one of the 8 operations; returns
+// the operation's number 1..8 (REGION, src/crplib/atom_surface.cpp:92-129; the comparisons as written there).
+int fold_into_triangle(double& x, double& y, double delta) {
+  if (y <= delta / 2.0) {
+    if (x <= delta / 2.0) {
+      if (x >= y) return 1;
+      const double t = x;
+      x = y;
+      y = t;
+      return 3;
+    }
+    if (y >= (delta - x)) {
+      const double t = x;
+      x = y;
+      y = delta - t;
+      return 4;
+    }
+    x = delta - x;
+    return 2;
+  }
+  if (x <= delta / 2.0) {
+    if (y > (delta - x)) {
+      y = delta - y;
+      return 7;
+    }
+    const double t = x;
+    x = delta - y;
+    y = t;
+    return 5;
+  }
+  if (x > y) {
+    const double t = x;
+    x = delta - y;
+    y = delta - t;
+    return 6;
+  }
+  x = delta - x;
+  y = delta - y;
+  return 8;
+}
+
+// The gradient of operation `region` from the triangle's (TRANSF, src/crplib/atom_surface.cpp:132-163).
+void unfold_gradient(int region, double& gx, double& gy) {
+  double t = 0.0;
+  switch (region) {
+    case 1: return;
+    case 2: gx = -gx; return;
+    case 3: t = gx; gx = gy; gy = t; return;
+    case 4: t = gx; gx = -gy; gy = t; return;
+    case 5: t = gx; gx = gy; gy = -t; return;
+    case 6: t = gx; gx = -gy; gy = -t; return;
+    case 7: gy = -gy; return;
+    case 8: gx = -gx; gy = -gy; return;
+    default: throw EvaluationError("CRP C4v atom-surface PES: unknown region");
+  }
+}
+
+// MURO: 16 atoms of the top layer and 16 of the second (layer -1), lattice units (src/crplib/square_atom_surface.cpp
+// :18-28, Fortran DATA VEC)
+struct MuroAtom {
+  double x, y, layer;
+};
+constexpr std::array<MuroAtom, 32> kMuro = {{
+    {0., -2., 0.},    {-1., -1., 0.},    {0., -1., 0.},     {1., -1., 0.},    {2., -1., 0.},     {-2., 0., 0.},
+    {-1., 0., 0.},    {0., 0., 0.},      {1., 0., 0.},      {2., 0., 0.},     {-1., 1., 0.},     {0., 1., 0.},
+    {1., 1., 0.},     {2., 1., 0.},      {0., 2., 0.},      {1., 2., 0.},     {-0.5, -1.5, -1.}, {0.5, -1.5, -1.},
+    {1.5, -1.5, -1.}, {-1.5, -0.5, -1.}, {-0.5, -0.5, -1.}, {0.5, -0.5, -1.}, {1.5, -0.5, -1.},  {-1.5, 0.5, -1.},
+    {-0.5, 0.5, -1.}, {0.5, 0.5, -1.},   {1.5, 0.5, -1.},   {2.5, 0.5, -1.},  {-0.5, 1.5, -1.},  {0.5, 1.5, -1.},
+    {1.5, 1.5, -1.},  {0.5, 2.5, -1.},
+}};
+
+// The (i, j) points (1-based) of the 5 x 5 grid each C4v site cut fills, sites in CRPData.3D order: top, hollow,
+// bridge, t2h, b2h, t2b (src/crplib/square_atom_surface.cpp:31-40)
+const std::array<std::vector<std::pair<int, int>>, 6> kSquareSitePoints = {{
+    {{1, 1}, {1, 5}, {5, 1}, {5, 5}},
+    {{3, 3}},
+    {{3, 1}, {1, 3}, {5, 3}, {3, 5}},
+    {{2, 2}, {4, 4}, {4, 2}, {2, 4}},
+    {{3, 2}, {2, 3}, {4, 3}, {3, 4}},
+    {{2, 1}, {4, 1}, {1, 2}, {5, 2}, {1, 4}, {5, 4}, {2, 5}, {4, 5}},
+}};
+
 // (x, y) folded into the skewed unit cell at the origin (ORIGEN2, src/crplib/atom_surface.cpp:73-82).
 std::pair<double, double> fold(double x, double y, double delta) {
   double bigx = x - y * kUsq3;
@@ -521,6 +780,7 @@ std::pair<double, double> fold(double x, double y, double delta) {
 
 AtomSurface::AtomSurface(const DataSet& data)
     : c3v_(data.symmetry == Symmetry::c3v),
+      square_(data.symmetry == Symmetry::c4v),
       a_(to_length(data.lattice_constant, data.units.length, LengthUnit::bohr)),
       table_(data.symmetry == Symmetry::c3v ? kAtomC3vTable.data() : kAtomC6vTable.data()),
       n_(data.symmetry == Symmetry::c3v ? 10 : 7) {
@@ -534,6 +794,10 @@ AtomSurface::AtomSurface(const DataSet& data)
       f.push_back(to_energy(data.corrugation.f[i], data.units.energy, EnergyUnit::hartree));
     }
     corrugation_ = Spline1D(x, f, {Spline1D::End::first_derivative, s1}, {Spline1D::End::first_derivative, s2});
+  }
+  if (square_) {  // MURO cluster, reduced cuts on the 5 x 5 grid of the cell, periodic 3D spline
+    init_square(data);
+    return;
   }
   // MURII cluster in its loop order (src/crplib/hexagonal_atom_surface.cpp:29-60)
   if (!c3v_) {
@@ -580,11 +844,74 @@ AtomSurface::AtomSurface(const DataSet& data)
   }
 }
 
+// MURO positions, the reduced cuts (V minus the cluster sum at the site) and the periodic 3D spline of the cell
+// (src/crplib/square_atom_surface.cpp:112-146). A negative first spacing is in units of a / sqrt(2) (kept).
+void AtomSurface::init_square(const DataSet& data) {
+  const double spacing = spacings_.front();
+  for (const MuroAtom& m : kMuro) {
+    const double z = spacing < 0.0 ? -m.layer * spacing * a_ / std::sqrt(2.0) : m.layer * spacing;
+    cluster_.push_back({m.x * a_, m.y * a_, z, m.layer == 0.0 ? 0 : 1});
+  }
+  const int nz = static_cast<int>(data.atom_cuts.front().v.x.size());
+  std::vector<double> z(nz), reduced(6 * static_cast<std::size_t>(nz)), first(6), last(6);
+  for (int site = 0; site < 6; ++site) {
+    const AtomCut& cut = data.atom_cuts[site];
+    const double s1 = to_force(cut.slope_first, data.units, kInternalUnits);
+    const double s2 = to_force(cut.slope_last, data.units, kInternalUnits);
+    for (int i = 0; i < nz; ++i) {
+      z[i] = to_length(cut.v.x[i], data.units.length, LengthUnit::bohr);
+      const double pot = to_energy(cut.v.f[i], data.units.energy, EnergyUnit::hartree);
+      const std::array<double, 4> rep = repulsion(cut.x * a_, cut.y * a_, z[i]);
+      reduced[site * nz + i] = pot - rep[0];
+      if (i == 0) first[site] = s1 - rep[3];
+      if (i == nz - 1) last[site] = s2 - rep[3];
+    }
+  }
+  std::vector<double> grid(5);
+  for (int i = 0; i < 5; ++i) grid[i] = static_cast<double>(i) * a_ * 0.25;
+  std::vector<double> f(25 * static_cast<std::size_t>(nz)), d1(25), d2(25);
+  for (int site = 0; site < 6; ++site)
+    for (const auto& [gi, gj] : kSquareSitePoints[site]) {
+      const int ij = (gi - 1) + 5 * (gj - 1);
+      for (int k = 0; k < nz; ++k) f[ij + 25 * k] = reduced[site * nz + k];
+      d1[ij] = first[site];
+      d2[ij] = last[site];
+    }
+  spline_ = PeriodicSpline3D(grid, grid, z, f, d1, d2);
+}
+
+// int_PES6CuHATOMCu100 (src/crplib/square_atom_surface.cpp:148-158): 0 above the data; the point folded into the
+// triangle, the cluster sum and the spline there, the gradient mapped back.
+AtomValue AtomSurface::evaluate_square(const Vec3& p) const {
+  if (p.z >= spline_.z().back()) return {0.0, {0.0, 0.0, 0.0}};
+  auto [x, y] = fold_square(p.x, p.y, a_);
+  const int region = fold_into_triangle(x, y, a_);
+  const std::array<double, 4> rep = repulsion(x, y, p.z);
+  const std::array<double, 4> sp = spline_.evaluate(x, y, p.z);
+  double gx = sp[1] + rep[1], gy = sp[2] + rep[2];
+  unfold_gradient(region, gx, gy);
+  return {sp[0] + rep[0], {gx, gy, sp[3] + rep[3]}};
+}
+
 // MURII: the corrugation function of every cluster atom within its reach, summed in cluster order
 // (src/crplib/hexagonal_atom_surface.cpp:109-154).
 std::array<double, 4> AtomSurface::repulsion(double x, double y, double z) const {
   std::array<double, 4> s{0.0, 0.0, 0.0, 0.0};
   const double reach = corrugation_.x().back(), nearest = corrugation_.x().front();
+  if (square_) {  // MURO / VREP1D: no lower limit (SPLINT continues the first cubic), dV/dr (x - x_a) / r
+    for (const ClusterAtom& at : cluster_) {
+      const double r = std::sqrt((x - at.x) * (x - at.x) + (y - at.y) * (y - at.y) + (z - at.z) * (z - at.z));
+      double vr = 0.0, dvdr = 0.0;
+      if (!(r > reach)) corrugation_.splint(r, vr, dvdr);
+      s[0] = s[0] + vr;
+      if (r > 0.0) {
+        s[1] = s[1] + dvdr * (x - at.x) / r;
+        s[2] = s[2] + dvdr * (y - at.y) / r;
+        s[3] = s[3] + dvdr * (z - at.z) / r;
+      }
+    }
+    return s;
+  }
   for (const ClusterAtom& at : cluster_) {
     const double dx = x - at.x, dy = y - at.y, dz = z - at.z;
     const double r = std::sqrt(dx * dx + dy * dy + dz * dz);
@@ -658,6 +985,7 @@ void AtomSurface::basis(double xa, double ya, double* f, double* dfdx, double* d
 // ATII (src/crplib/hexagonal_atom_surface.cpp:253-283): the repulsive sum at the folded position plus the Fourier
 // interpolation of the reduced cuts.
 AtomValue AtomSurface::evaluate(const Vec3& p) const {
+  if (square_) return evaluate_square(p);
   const auto [xf, yf] = fold(p.x, p.y, a_);
   const std::array<double, 4> rep = repulsion(xf, yf, p.z);
   const std::vector<double>& z = cuts_.front().x();
@@ -868,6 +1196,72 @@ void molecule_basis(double x, double y, double a, double* f, double* dfdx, doubl
   for (int i = 0; i < 12; ++i) dfdy[i] = (-dfdx[i] + 2.00 * dfdy[i]) * kUsq3;
 }
 
+// The C4v site functions (src/crplib/site_interpolation.cpp:196-270): top and hollow (theta 0; 90/0; 90/45), bridge
+// (0; 90/0; 90/45; 90/90), t2h (0; 45/225; 45/45; 90/45; 90/135; 45/135).
+Site square_top_site(const Spline2D::Value* c, const Trig& t, double phi) {
+  Site s{};
+  for (int i = 0; i < 3; ++i) {
+    const double v0 = cutc(c, i, 0), c1 = cutc(c, i, 1), c2 = cutc(c, i, 2);
+    const double v90 = 0.5 * (c1 + c2) + 0.5 * (c1 - c2) * std::cos(4.0 * phi);
+    s.v[i] = 0.5 * (v0 + v90) + 0.5 * (v0 - v90) * t.c2;
+    if (i == 0) {
+      const double d0 = 0.0;
+      const double d90 = -4.0 * 0.5 * (c1 - c2) * std::sin(4.0 * phi);
+      s.d_phi = 0.5 * (d0 + d90) + 0.5 * (d0 - d90) * t.c2;
+      s.d_theta = -2.0 * 0.5 * (v0 - v90) * t.s2;
+    }
+  }
+  return s;
+}
+
+Site square_bridge_site(const Spline2D::Value* c, const Trig& t, double phi) {
+  Site s{};
+  for (int i = 0; i < 3; ++i) {
+    const double v0 = cutc(c, i, 0), c1 = cutc(c, i, 1), c2 = cutc(c, i, 2), c3 = cutc(c, i, 3);
+    const double v90 = (0.25 * c1 + 0.5 * c2 + 0.25 * c3) + 0.5 * (c1 - c3) * std::cos(2.0 * phi) +
+                       (0.25 * c1 - 0.5 * c2 + 0.25 * c3) * std::cos(4.0 * phi);
+    s.v[i] = 0.5 * (v0 + v90) + 0.5 * (v0 - v90) * t.c2;
+    if (i == 0) {
+      const double d0 = 0.0;
+      const double d90 =
+          -2.0 * 0.5 * (c1 - c3) * std::sin(2.0 * phi) - 4.0 * (0.25 * c1 - 0.5 * c2 + 0.25 * c3) * std::sin(4.0 * phi);
+      s.d_phi = 0.5 * (d0 + d90) + 0.5 * (d0 - d90) * t.c2;
+      s.d_theta = -2.0 * 0.5 * (v0 - v90) * t.s2;
+    }
+  }
+  return s;
+}
+
+Site square_t2h_site(const Spline2D::Value* c, const Trig& t, double phi) {
+  const double phi45 = phi - kPi / 4.0;
+  const double phi135 = phi + kPi - kPi / 4.0;
+  Site s{};
+  for (int i = 0; i < 3; ++i) {
+    const double v0 = cutc(c, i, 0), c1 = cutc(c, i, 1), c2 = cutc(c, i, 2);
+    const double c3 = cutc(c, i, 3), c4 = cutc(c, i, 4), c5 = cutc(c, i, 5);
+    // V(45, phi) = a0 + b0 cos(phi - 45) + c0 cos(2 (phi - 45)) through the three 45-degree cuts
+    const double cc0 = 0.25 * (c1 + c2 - 2.0 * c5);
+    const double b0 = -c5 + c2 - 2.0 * cc0;
+    const double a0 = c2 - b0 - cc0;
+    const double v45 = a0 + b0 * std::cos(phi45) + cc0 * std::cos(2.0 * phi45);
+    const double v135 = a0 + b0 * std::cos(phi135) + cc0 * std::cos(2.0 * phi135);
+    const double v90 = 0.5 * (c3 + c4) + 0.5 * (c3 - c4) * std::cos(2.0 * phi45);
+    s.v[i] = 0.25 * (v0 + v45 + v90 + v135) + 0.5 * (v0 - v90) * t.c2 + 0.5 * (v45 - v135) * t.s2 +
+             0.25 * (v0 + v90 - v45 - v135) * t.c4;
+    if (i == 0) {
+      const double d0 = 0.0;
+      const double d45 = -b0 * std::sin(phi45) - 2.0 * cc0 * std::sin(2.0 * phi45);
+      const double d135 = -b0 * std::sin(phi135) - 2.0 * cc0 * std::sin(2.0 * phi135);
+      const double d90 = -2.0 * 0.5 * (c3 - c4) * std::sin(2.0 * phi45);
+      s.d_phi = 0.25 * (d0 + d45 + d90 + d135) + 0.5 * (d0 - d90) * t.c2 + 0.5 * (d45 - d135) * t.s2 +
+                0.25 * (d0 + d90 - d45 - d135) * t.c4;
+      s.d_theta = -2.0 * 0.5 * (v0 - v90) * t.s2 + 2.0 * 0.5 * (v45 - v135) * t.c2 -
+                  4.0 * 0.25 * (v0 + v90 - v45 - v135) * t.s4;
+    }
+  }
+  return s;
+}
+
 std::vector<double> lengths(const std::vector<double>& x, const Units& u) {
   std::vector<double> y;
   for (double v : x) y.push_back(to_length(v, u.length, LengthUnit::bohr));
@@ -886,7 +1280,6 @@ MoleculeSurface::MoleculeSurface(const DataSet& data)
     : symmetry_(data.symmetry),
       a_(to_length(data.lattice_constant, data.units.length, LengthUnit::bohr)),
       atom_(data) {
-  if (symmetry_ == Symmetry::c4v) throw DataError("C4v data sets are not supported by this library");
   const Units& in = data.units;
   zso_min_ = to_length(data.zso_min, in.length, LengthUnit::bohr);
   zso_max_ = to_length(data.zso_max, in.length, LengthUnit::bohr);
@@ -898,8 +1291,9 @@ MoleculeSurface::MoleculeSurface(const DataSet& data)
     const std::vector<double> r = lengths(cut.r, in), z = lengths(cut.z, in);
     std::vector<double> v = energies(cut.v, in);
     if (!data.cuts_interpolation) {  // total energies: subtract both atom potentials (corrugation reduction)
-      const double ucm = cut.x * a_, vcm = cut.y * a_;
-      const double xcm = ucm + vcm / 2.0, ycm = std::sqrt(3.0) * vcm / 2.0;
+      const double ucm = cut.x * a_, vcm = cut.y * a_;  // skewed for C3v / C6v, Cartesian for C4v
+      const bool sq = symmetry_ == Symmetry::c4v;
+      const double xcm = sq ? ucm : ucm + vcm / 2.0, ycm = sq ? vcm : std::sqrt(3.0) * vcm / 2.0;
       const double theta = cut.theta * kPi / 180.0, phi = cut.phi * kPi / 180.0;
       for (std::size_t j = 0; j < z.size(); ++j) {
         for (std::size_t i = 0; i < r.size(); ++i) {
@@ -1008,6 +1402,74 @@ std::array<double, 7> MoleculeSurface::interaction(const Spline2D::Value* c, dou
   return out;
 }
 
+// RRINT's site part and SIS8x8 (src/crplib/square_interaction.cpp): 8 Fourier sites (top, the bridges, hollow, the
+// t2h sites) and the Fourier interpolation over the centre (x, y Cartesian).
+std::array<double, 7> MoleculeSurface::square_interaction(const Spline2D::Value* c, double theta, double phi,
+                                                          double x, double y) const {
+  const Trig t{std::cos(2.00 * theta), std::sin(2.00 * theta), std::cos(4.00 * theta), std::sin(4.00 * theta)};
+  std::array<Site, 8> s{};
+  s[0] = square_top_site(c, t, phi);
+  s[1] = square_bridge_site(c + 3, t, phi);
+  s[4] = square_bridge_site(c + 3, t, phi + kPi / 2.0);
+  s[2] = square_top_site(c + 7, t, phi);  // the hollow has the top's cut pattern
+  s[3] = square_t2h_site(c + 10, t, phi);
+  s[5] = square_t2h_site(c + 10, t, phi - kPi / 2.0);
+  s[6] = square_t2h_site(c + 10, t, phi - kPi);
+  s[7] = square_t2h_site(c + 10, t, phi + kPi / 2.0);
+  auto comp = [&](int k, int i) {
+    return i == 0 ? s[k].v[0] : (i == 1 ? s[k].v[1] : (i == 2 ? s[k].v[2] : (i == 3 ? s[k].d_phi : s[k].d_theta)));
+  };
+  // the coefficients of the 8 basis functions from the site values (the inverse of the basis at the sites)
+  auto coefficients = [&](int i) {
+    const double p1 = comp(0, i), p2 = comp(1, i), p3 = comp(2, i), p4 = comp(3, i);
+    const double p5 = comp(4, i), p6 = comp(5, i), p7 = comp(6, i), p8 = comp(7, i);
+    return std::array<double, 8>{1.0 / 8.0 * (p1 + p2 + p5 + p3 + p4 + p6 + p7 + p8),
+                                 1.0 / 4.0 * (p1 - p2 + p5 - p3),
+                                 1.0 / 4.0 * (p1 + p2 - p5 - p3),
+                                 1.0 / 4.0 * (p4 - p6 - p7 + p8),
+                                 1.0 / 4.0 * (p4 + p6 - p7 - p8),
+                                 1.0 / 8.0 * (p1 - p2 - p5 + p3 - p4 + p6 - p7 + p8),
+                                 1.0 / 8.0 * (p1 - p2 - p5 + p3 + p4 - p6 + p7 - p8),
+                                 1.0 / 16.0 * (p1 + p2 + p5 + p3 - p4 - p6 - p7 - p8)};
+  };
+  const double fac = 2.0 * kPi / a_;
+  const double f[8] = {1.0,
+                       std::cos(fac * x),
+                       std::cos(fac * y),
+                       std::sin(fac * x),
+                       std::sin(fac * y),
+                       std::cos(fac * (x + y)),
+                       std::cos(fac * (x - y)),
+                       std::cos(fac * (2.0 * x)) + std::cos(fac * (2.0 * y))};
+  const double fx[8] = {0.0,
+                        -fac * std::sin(fac * x),
+                        0.0,
+                        fac * std::cos(fac * x),
+                        0.0,
+                        -fac * std::sin(fac * (x + y)),
+                        -fac * std::sin(fac * (x - y)),
+                        -2.0 * fac * std::sin(fac * (2.0 * x))};
+  const double fy[8] = {0.0,
+                        0.0,
+                        -fac * std::sin(fac * y),
+                        0.0,
+                        fac * std::cos(fac * y),
+                        -fac * std::sin(fac * (x + y)),
+                        fac * std::sin(fac * (x - y)),
+                        -2.0 * fac * std::sin(fac * (2.0 * y))};
+  auto dot = [](const std::array<double, 8>& a, const double* g) {
+    return a[0] * g[0] + a[1] * g[1] + a[2] * g[2] + a[3] * g[3] + a[4] * g[4] + a[5] * g[5] + a[6] * g[6] +
+           a[7] * g[7];
+  };
+  std::array<double, 7> out{};
+  const std::array<double, 8> a0 = coefficients(0);
+  out[0] = dot(a0, f);
+  out[5] = dot(a0, fx);
+  out[6] = dot(a0, fy);
+  for (int i = 1; i < 5; ++i) out[i] = dot(coefficients(i), f);
+  return out;
+}
+
 // POT6D -> CAMBCORD -> RRINTII (src/crplib/molecule_surface.cpp:249-315)
 std::optional<MoleculeValue> MoleculeSurface::evaluate(const Vec3& a, const Vec3& b) const {
   const Geometry g = molecule_geometry(a, b);
@@ -1026,15 +1488,19 @@ std::optional<MoleculeValue> MoleculeSurface::evaluate(const Vec3& a, const Vec3
     if (!cell) return std::nullopt;  // outside a cut's grid the Fortran gives up silently
     cv[k] = cuts_[k].at(*cell);
   }
-  const std::array<double, 7> vi = interaction(cv.data(), g.theta, g.phi, g.x, g.y);
-  // derivatives of r, theta, phi with respect to atom a (RRINTII)
+  const bool square = symmetry_ == Symmetry::c4v;
+  const std::array<double, 7> vi = square ? square_interaction(cv.data(), g.theta, g.phi, g.x, g.y)
+                                          : interaction(cv.data(), g.theta, g.phi, g.x, g.y);
+  // derivatives of r, theta, phi with respect to atom a (RRINTII; RRINT for C4v in its own operand order)
   const double rxa = -(b.x - a.x) / g.r, rya = -(b.y - a.y) / g.r, rza = -(b.z - a.z) / g.r;
   double txa = 0.0, tya = 0.0, pxa = 0.0, pya = 0.0;
   const double rho2 = (b.x - a.x) * (b.x - a.x) + (b.y - a.y) * (b.y - a.y);
   const double rho = std::sqrt(rho2);
   if (rho2 > 0.00) {
-    txa = -(b.z - a.z) * (b.x - a.x) / (g.r * g.r * rho);
-    tya = -(b.z - a.z) * (b.y - a.y) / (g.r * g.r * rho);
+    txa = square ? -(b.z - a.z) * (b.x - a.x) / (g.r * g.r) / std::sqrt(rho2)
+                 : -(b.z - a.z) * (b.x - a.x) / (g.r * g.r * rho);
+    tya = square ? -(b.z - a.z) * (b.y - a.y) / (g.r * g.r) / std::sqrt(rho2)
+                 : -(b.z - a.z) * (b.y - a.y) / (g.r * g.r * rho);
     pxa = (b.y - a.y) / rho2;
     pya = -(b.x - a.x) / rho2;
   }
@@ -1066,6 +1532,7 @@ std::optional<MoleculeValue> MoleculeSurface::evaluate(const Vec3& a, const Vec3
 
 Packed pack(const QdSurface& q) {
   const MoleculeSurface& m = q.surface();
+  if (m.symmetry() == Symmetry::c4v) throw DataError("the device evaluates C3v and C6v CRP data sets only");
   const AtomSurface& at = m.atom_;
   Packed p;
   auto spline = [](const Spline1D& s, std::vector<double>& out) {
@@ -1110,7 +1577,9 @@ Packed pack(const QdSurface& q) {
 
 QdSurface::QdSurface(const DataSet& data) : surface_(data) {}
 
-double QdSurface::skewing_angle() const { return kPi / 3.0; }
+double QdSurface::skewing_angle() const {
+  return surface_.symmetry() == Symmetry::c4v ? kPi / 2.0 : kPi / 3.0;
+}
 
 double QdSurface::potential_ev(double u, double v, double z, double r, double theta, double phi) const {
   constexpr double delta = 1.0e-12;  // keeps the clamped values inside the data
@@ -1121,8 +1590,11 @@ double QdSurface::potential_ev(double u, double v, double z, double r, double th
     rr = surface_.r_max() - delta;
   }
   if (zz < surface_.z_min() + delta) zz = surface_.z_min() + delta;
-  const double x = 1.0 * u + kCosPi3 * v;
-  const double y = 0.0 * u + kSinPi3 * v;
+  // [[1, cos g], [0, sin g]] (u, v); cos(pi/2) as gfortran folds it, bits 0x3C91A62633145C07
+  const bool sq = surface_.symmetry() == Symmetry::c4v;
+  const double cg = sq ? 0x1.1a62633145c07p-54 : kCosPi3, sg = sq ? 1.0 : kSinPi3;
+  const double x = 1.0 * u + cg * v;
+  const double y = 0.0 * u + sg * v;
   const Vec3 a{x - 0.5 * rr * std::sin(theta) * std::cos(phi), y - 0.5 * rr * std::sin(theta) * std::sin(phi),
                zz - 0.5 * rr * std::cos(theta)};
   const Vec3 b{x + 0.5 * rr * std::sin(theta) * std::cos(phi), y + 0.5 * rr * std::sin(theta) * std::sin(phi),

@@ -152,7 +152,22 @@ struct AtomValue {
   Vec3 gradient;
 };
 
-// The atom-surface PES of a C3v or C6v data set (src/crplib/hexagonal_atom_surface.cpp).
+// A 3D cubic spline periodic in x and y and clamped in z: the C4v atom surface (src/crplib/periodic_spline_3d.cpp).
+class PeriodicSpline3D {
+ public:
+  PeriodicSpline3D() = default;
+  PeriodicSpline3D(std::vector<double> x, std::vector<double> y, std::vector<double> z, std::vector<double> f,
+                   const std::vector<double>& dfdz_first, const std::vector<double>& dfdz_last);
+  // value, d/dx, d/dy, d/dz; EvaluationError outside the grid
+  std::array<double, 4> evaluate(double x, double y, double z) const;
+  const std::vector<double>& z() const { return z_; }
+
+ private:
+  std::vector<double> x_, y_, z_, f_, xx_, yy_, xxyy_, zz_, zzxx_, zzyy_, zzxxyy_;
+};
+
+// The atom-surface PES of a data set (src/crplib/hexagonal_atom_surface.cpp for C3v / C6v,
+// src/crplib/square_atom_surface.cpp for C4v).
 class AtomSurface {
  public:
   explicit AtomSurface(const DataSet& data);
@@ -168,6 +183,7 @@ class AtomSurface {
     int layer;
   };
   bool c3v_;
+  bool square_;
   double a_;
   std::vector<double> spacings_;
   Spline1D corrugation_;
@@ -175,7 +191,10 @@ class AtomSurface {
   std::vector<Spline1D> cuts_;
   const double* table_;
   int n_;
+  PeriodicSpline3D spline_;  // C4v
   void basis(double x, double y, double* f, double* dfdx, double* dfdy) const;
+  AtomValue evaluate_square(const Vec3& p) const;
+  void init_square(const DataSet& data);
 };
 
 struct MoleculeValue {
@@ -218,6 +237,8 @@ class MoleculeSurface {
   std::vector<Spline2D> cuts_;
   double r_min_ = 0.0, r_max_ = 100.0, z_min_ = -100.0;
   std::array<double, 7> interaction(const Spline2D::Value* cuts, double theta, double phi, double x, double y) const;
+  std::array<double, 7> square_interaction(const Spline2D::Value* cuts, double theta, double phi, double x,
+                                           double y) const;
 };
 
 // Centre, bond length and orientation of the molecule (CAMBCORD, src/crplib/molecule_surface.cpp:22-41).

@@ -225,7 +225,7 @@ class CrpSurface:
         _check(host_library().spo_crp_parameters(self._handle, _ptr(out)))
         keys = ("lattice_constant", "z_asymptotic", "skewing_angle", "zso_min", "zso_max", "r_min", "r_max", "z_min")
         d = dict(zip(keys, (float(x) for x in out[:8])))
-        d["point_group"] = "C3v" if out[8] == 3.0 else "C6v"
+        d["point_group"] = {3.0: "C3v", 4.0: "C4v"}.get(float(out[8]), "C6v")
         return d
 
 

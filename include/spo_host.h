@@ -122,7 +122,7 @@ int spo_pes_value(const char* system_id, double x, double y, double z, double r,
                   double* out);
 
 /* The CRP potential of a data set directory (SystemID CRPlibrary; the reference's CRP library, src/crplib/, as
- * src/potential/crp_potential.cpp uses it). C3v and C6v data sets; SPO_E_INPUT for a missing or unusable one.
+ * src/potential/crp_potential.cpp uses it). C3v, C4v and C6v data sets; SPO_E_INPUT for a missing or unusable one.
  * Points are evaluated in parallel; every function takes n points and writes n records. */
 typedef struct spo_crp spo_crp;
 int spo_crp_open(const char* directory, spo_crp** out);
@@ -141,8 +141,8 @@ int spo_crp_device_values(const spo_crp* crp, long n, const double* points, doub
 int spo_crp_gas_phase(const spo_crp* crp, long n, const double* zr, double* out);
 /* atoms as for spo_crp_molecule (bohr); out[6k..] = centre x y Z, r, theta, phi (CAMBCORD). */
 int spo_crp_geometry(long n, const double* atoms, double* out);
-/* out[0..8] = lattice constant, Zasy, skewing angle, ZsoMin, ZsoMax, Rmin, Rmax, Zmin (bohr, rad), and 3 or 6 for
- * C3v or C6v. */
+/* out[0..8] = lattice constant, Zasy, skewing angle, ZsoMin, ZsoMax, Rmin, Rmax, Zmin (bohr, rad), and 3, 4 or 6
+ * for C3v, C4v or C6v. */
 int spo_crp_parameters(const spo_crp* crp, double* out);
 
 /* The CRP surface plus the SCM correction of a distorted slab snapshot (SystemID CRPDistrtSurface; the reference's

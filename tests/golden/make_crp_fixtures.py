@@ -23,7 +23,8 @@ import struct
 import numpy as np
 
 REF = "/root/reference"
-CASES = ["cu111_srp48", "c3v_pairwise", "c6v_pairwise", "c3v_impulse_15", "c6v_impulse_5"]
+CASES = ["cu111_srp48", "c3v_pairwise", "c6v_pairwise", "c3v_impulse_15", "c6v_impulse_5", "c4v_pairwise",
+         "c4v_impulse_6", "c4v_negative_spacing"]
 OUT = os.path.join(os.path.dirname(os.path.abspath(__file__)), "crp")
 
 

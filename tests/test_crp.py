@@ -24,7 +24,8 @@ import spodvr_amd
 from tests import cases, fixtures
 
 CRP_DIR = os.path.join(cases.GOLDEN_DIR, "crp")
-CASES = ["cu111_srp48", "c3v_pairwise", "c6v_pairwise", "c3v_impulse_15", "c6v_impulse_5"]
+CASES = ["cu111_srp48", "c3v_pairwise", "c6v_pairwise", "c3v_impulse_15", "c6v_impulse_5", "c4v_pairwise",
+         "c4v_impulse_6", "c4v_negative_spacing"]
 BOHR_PER_ANGSTROM = 1.0 / 0.529177249  # the CRP library's factor (src/crplib/units.cpp:13-18)
 EV_PER_HARTREE = 27.2113961
 
@@ -115,30 +116,33 @@ def test_molecule_geometry(crp_case):
 # Symmetry of the surface: the C3v (C6v) point group about a top site, the lattice period and the exchange of the
 # two identical atoms leave V unchanged. A wrong site, table column or rotated azimuth breaks one of these.
 
-def _skewed(x, y):
-    v = y / math.sin(math.pi / 3)
-    return x - v * math.cos(math.pi / 3), v
+def _skewed(x, y, gamma=math.pi / 3):
+    v = y / math.sin(gamma)
+    return x - v * math.cos(gamma), v
 
 
-def _cartesian(u, v):
-    return u + v * math.cos(math.pi / 3), v * math.sin(math.pi / 3)
+def _cartesian(u, v, gamma=math.pi / 3):
+    return u + v * math.cos(gamma), v * math.sin(gamma)
 
 
-@pytest.mark.parametrize("case", ["cu111_srp48", "c6v_pairwise"])
+@pytest.mark.parametrize("case", ["cu111_srp48", "c6v_pairwise", "c4v_pairwise"])
 def test_surface_symmetry(case):
     s = spodvr_amd.CrpSurface(data_set(case))
     a = s.parameters()["lattice_constant"]
+    square = s.parameters()["point_group"] == "C4v"
+    gamma = math.pi / 2 if square else math.pi / 3
+    turn = math.pi / 2 if square else 2 * math.pi / 3  # the smallest rotation of the point group's C3 / C4 axis
     rng = np.random.default_rng(7)
     base, images = [], []
     for _ in range(40):
         u, v = rng.uniform(0, a, 2)
         z, r = rng.uniform(1.5, 4.0), rng.uniform(1.0, 2.0)
         theta, phi = rng.uniform(0.1, math.pi - 0.1), rng.uniform(0, 2 * math.pi)
-        x, y = _cartesian(u, v)
+        x, y = _cartesian(u, v, gamma)
         point = (u, v, z, r, theta, phi)
-        c, sn = math.cos(2 * math.pi / 3), math.sin(2 * math.pi / 3)
-        rotated = (*_skewed(c * x - sn * y, sn * x + c * y), z, r, theta, phi + 2 * math.pi / 3)
-        mirrored = (*_skewed(-x, y), z, r, theta, math.pi - phi)  # the plane x = 0 holds the hcp and fcc sites
+        c, sn = math.cos(turn), math.sin(turn)
+        rotated = (*_skewed(c * x - sn * y, sn * x + c * y, gamma), z, r, theta, phi + turn)
+        mirrored = (*_skewed(-x, y, gamma), z, r, theta, math.pi - phi)  # the plane x = 0 (hcp and fcc for C3v)
         shifted = (u + a, v - a, z, r, theta, phi)
         exchanged = (u, v, z, r, math.pi - theta, phi + math.pi)
         for image in (rotated, mirrored, shifted, exchanged):
