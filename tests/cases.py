@@ -101,3 +101,16 @@ def write_case(name: str, params: dict, directory: str | None = None) -> str:
 
 def golden_input(case: str) -> str:
     return os.path.join(GOLDEN_DIR, case, "input.json")
+
+
+def golden_overrides(case: str) -> dict:
+    """The data files of a golden case that names them (the CRP data set, the SCM parameter and slab files), as the
+    fixture copies under tests/golden/."""
+    out = {}
+    doc = json.load(open(golden_input(case)))
+    if doc.get("SystemID") in ("CRPlibrary", "CRPDistrtSurface"):
+        out["CRPlibDataSet"] = os.path.join(GOLDEN_DIR, "crp", "cu111_srp48", "dataset")
+    if doc.get("SystemID") == "CRPDistrtSurface":
+        out["SCMParFilepath"] = os.path.join(GOLDEN_DIR, "scm", "example", "params.dat")
+        out["PosFilepath"] = os.path.join(GOLDEN_DIR, "scm", "example", "examplePosfile.dat")
+    return out

@@ -32,12 +32,12 @@ import numpy as np
 REFERENCE = "/root/reference/tests/reference"
 HERE = os.path.dirname(os.path.abspath(__file__))
 CASES = ["hardwall_short", "cu111_dailight_short", "cu111_dailight_short_sym", "cu111_dailight_oblique",
-         "cu111_dailight_short_lowz"]
+         "cu111_dailight_short_lowz", "scm_cu111_srp48_small"]
 BK_INDICES = 96          # coefficient indices kept per case
 FORT25_BLOCKS = 5        # energy blocks kept per case (evenly spread)
 PES_POINTS = 1500        # subsample of the PES fixture
 # set-up only cases (the PES and the gas-phase states; no propagation output)
-SETUP_CASES = ["crp_cu111_srp48_small", "scm_cu111_srp48_small", "scm_cu111_srp48_small_rescaling_off"]
+SETUP_CASES = ["crp_cu111_srp48_small", "scm_cu111_srp48_small_rescaling_off"]
 
 
 def fortran_records(path: str) -> list[bytes]:

@@ -1,7 +1,8 @@
 """The HIP backend through whole runs of the reference's Fortran-gated cases: the averages of every dtAve, the BK
 coefficients of every analysis step (1e-10 of the record scale), the analysis limits, the fixed specular line and the
 fort.25 S-matrix rows, all against the golden fixtures (tests/test_regression_propagation.cpp of the reference runs
-the same gates on its CPU and CUDA backends). The complex64 storage mode runs the same cases and is held to the
+the same gates on its CPU and CUDA backends). The SCM case runs the CRP surface of the SRP48 data set plus the SCM
+correction of the rescaled example slab, its potential grid built on the GPU. The complex64 storage mode runs the same cases and is held to the
 reaction and channel probabilities of the FP64 run (1e-4 absolute)."""
 
 import numpy as np
@@ -12,7 +13,7 @@ from tests import cases, fixtures
 pytestmark = pytest.mark.gpu
 
 CASES = ["hardwall_short", "cu111_dailight_short", "cu111_dailight_short_sym", "cu111_dailight_oblique",
-         "cu111_dailight_short_lowz"]
+         "cu111_dailight_short_lowz", "scm_cu111_srp48_small"]
 
 _fp64 = {}
 
@@ -21,7 +22,7 @@ _fp64 = {}
 def test_hip_run_passes_the_fortran_gates(case):
     import spodvr_amd
     print(f"case {case} fp64 full run", flush=True)
-    run = spodvr_amd.Run(cases.golden_input(case), device="hip")
+    run = spodvr_amd.Run(cases.golden_input(case), cases.golden_overrides(case) or None, device="hip")
     out = fixtures.full_run(run)
     report = fixtures.check_gates(run, out, case)
     print(f"  {report}", flush=True)
