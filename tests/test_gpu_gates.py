@@ -30,15 +30,17 @@ def test_hip_run_passes_the_fortran_gates(case):
     run.close()
 
 
-@pytest.mark.parametrize("case", ["hardwall_short", "cu111_dailight_short_sym", "cu111_dailight_oblique"])
+@pytest.mark.parametrize("case", ["hardwall_short", "cu111_dailight_short_sym", "cu111_dailight_oblique",
+                                  "scm_cu111_srp48_small"])
 def test_mixed_precision_run_matches_fp64_probabilities(case):
     import spodvr_amd
     print(f"case {case} mixed full run", flush=True)
     if case not in _fp64:
-        run = spodvr_amd.Run(cases.golden_input(case), device="hip")
+        run = spodvr_amd.Run(cases.golden_input(case), cases.golden_overrides(case) or None, device="hip")
         _fp64[case] = fixtures.full_run(run)["results"]
         run.close()
-    run = spodvr_amd.Run(cases.golden_input(case), {"Precision": "mixed"}, device="hip")
+    run = spodvr_amd.Run(cases.golden_input(case), {"Precision": "mixed", **cases.golden_overrides(case)},
+                         device="hip")
     mixed = fixtures.full_run(run)["results"]
     ref = _fp64[case]
     dp = float(np.abs(mixed["p"] - ref["p"]).max())
