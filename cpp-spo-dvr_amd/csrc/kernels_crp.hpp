@@ -33,6 +33,8 @@ struct CrpDev {
   const double* cut_data;
   const double* h2;
   const double* zdep;
+  int square, cell_nz;
+  const double* cell;  // C4v: x[5] y[5] z[nz], f xx yy xxyy zz zzxx zzyy zzxxyy (25 nz each)
 };
 
 // SPLINT / UBICAR bisection: the lower knot of the interval holding t.
@@ -92,10 +94,105 @@ __device__ inline double crp_dpl(const double* s, int n, double t) {
   return (-g * f * (m[i - 1] * (f + e) + m[i] * (g + e)) + 6.0 * (g * y[i] + f * y[i - 1])) / (6.0 * e);
 }
 
+
+// SPLINT3D value weights in one direction (x**3.0 a call of pow, as there)
+__device__ inline void crp_w3(const double* x, int i, double t, double& a, double& b, double& c, double& d) {
+  const double h = x[i + 1] - x[i];
+  a = (x[i + 1] - t) / h;
+  b = 1.0 - a;
+  c = (pow(a, 3.0) - a) / 6.0 * (h * h);
+  d = (pow(b, 3.0) - b) / 6.0 * (h * h);
+}
+
+// The C4v atom-surface V (int_PES6CuHATOMCu100 value): 0 above the data; the point folded into the triangle of the
+// square cell (ORIGEN, REGION), the MURO cluster sum and the cell's periodic 3D spline there.
+__device__ inline double crp_atom_square(const CrpDev& s, double px, double py, double pz, bool& bad) {
+  const int nz = s.cell_nz;
+  const double* gx = s.cell;
+  const double* gz = s.cell + 10;
+  if (pz >= gz[nz - 1]) return 0.0;
+  const double a = s.a;
+  const double o1 = px / a, o2 = py / a;
+  if (!(fabs(o1) < 2.0e9) || !(fabs(o2) < 2.0e9)) {
+    bad = true;
+    return 0.0;
+  }
+  const int i1 = o1 >= 0.0 ? static_cast<int>(o1) : static_cast<int>(o1 - 1.0);
+  const int i2 = o2 >= 0.0 ? static_cast<int>(o2) : static_cast<int>(o2 - 1.0);
+  double x = px - i1 * a, y = py - i2 * a;
+  // REGION (the gradient's region is not needed for V)
+  if (y <= a / 2.0) {
+    if (x <= a / 2.0) {
+      if (!(x >= y)) {
+        const double t = x;
+        x = y;
+        y = t;
+      }
+    } else if (y >= (a - x)) {
+      const double t = x;
+      x = y;
+      y = a - t;
+    } else {
+      x = a - x;
+    }
+  } else if (x <= a / 2.0) {
+    if (y > (a - x)) {
+      y = a - y;
+    } else {
+      const double t = x;
+      x = a - y;
+      y = t;
+    }
+  } else if (x > y) {
+    const double t = x;
+    x = a - y;
+    y = a - t;
+  } else {
+    x = a - x;
+    y = a - y;
+  }
+  // MURO / VREP1D value (no lower limit)
+  const int nc = s.n_corrugation;
+  const double reach = s.corrugation[nc - 1];
+  double rep = 0.0;
+  for (int k = 0; k < s.n_cluster; ++k) {
+    const double* at = s.cluster + 4 * k;
+    const double r = sqrt((x - at[0]) * (x - at[0]) + (y - at[1]) * (y - at[1]) + (pz - at[2]) * (pz - at[2]));
+    double vr = 0.0;
+    if (!(r > reach)) vr = crp_splint(s.corrugation, nc, r);
+    rep = rep + vr;
+  }
+  // SPLINT3D value
+  if (x < gx[0] || x > gx[4] || y < gx[5] || y > gx[9] || pz < gz[0]) {
+    bad = true;
+    return 0.0;
+  }
+  const int i = crp_bisect(gx, 5, x), j = crp_bisect(gx + 5, 5, y), k = crp_bisect(gz, nz, pz);
+  double xa, xb, xc, xd, ya, yb, yc, yd, za, zb, zc, zd;
+  crp_w3(gx, i, x, xa, xb, xc, xd);
+  crp_w3(gx + 5, j, y, ya, yb, yc, yd);
+  crp_w3(gz, k, pz, za, zb, zc, zd);
+  const long plane = 25L * nz;
+  const double* f = s.cell + 10 + nz;
+  auto plane_sum = [&](const double* g, const double* gxx, const double* gyy, const double* gxxyy, int kk) {
+    const long p00 = i + 5L * (j + 5L * kk), p10 = p00 + 1, p01 = p00 + 5, p11 = p01 + 1;
+    return ya * (xa * g[p00] + xb * g[p10] + xc * gxx[p00] + xd * gxx[p10]) +
+           yb * (xa * g[p01] + xb * g[p11] + xc * gxx[p01] + xd * gxx[p11]) +
+           yc * (xa * gyy[p00] + xb * gyy[p10] + xc * gxxyy[p00] + xd * gxxyy[p10]) +
+           yd * (xa * gyy[p01] + xb * gyy[p11] + xc * gxxyy[p01] + xd * gxxyy[p11]);
+  };
+  const double *fxx = f + plane, *fyy = f + 2 * plane, *fxxyy = f + 3 * plane, *fzz = f + 4 * plane,
+               *fzzxx = f + 5 * plane, *fzzyy = f + 6 * plane, *fzzxxyy = f + 7 * plane;
+  const double t0 = plane_sum(f, fxx, fyy, fxxyy, k), t1 = plane_sum(f, fxx, fyy, fxxyy, k + 1);
+  const double t2 = plane_sum(fzz, fzzxx, fzzyy, fzzxxyy, k), t3 = plane_sum(fzz, fzzxx, fzzyy, fzzxxyy, k + 1);
+  return za * t0 + zb * t1 + zc * t2 + zd * t3 + rep;
+}
+
 // The atom-surface V (ATII value): the corrugation function over the cluster at the folded position plus the
 // Fourier interpolation of the reduced cuts. `bad` is set where the host stops.
 __device__ inline double crp_atom(const CrpDev& s, double px, double py, double pz, bool& bad) {
   using namespace spo::crp_tables;
+  if (s.square) return crp_atom_square(s, px, py, pz, bad);
   const double delta = s.a;
   // ORIGEN2
   double bigx = px - py * kUsq3;
@@ -269,9 +366,42 @@ __device__ inline double crp_molecule(const CrpDev& s, const double* a, const do
             wxc * (wya * fxx[p00] + wyb * fxx[p01] + wyc * fxxyy[p00] + wyd * fxxyy[p01]) +
             wxd * (wya * fxx[p10] + wyb * fxx[p11] + wyc * fxxyy[p10] + wyd * fxxyy[p11]);
   }
+  const double c2t = cos(2.00 * theta), s2t = sin(2.00 * theta), c4t = cos(4.00 * theta);
+  double iv = 0.00;
+  if (s.square) {  // the 8 sites of C4v (RRINT) and SIS8x8, values only
+    auto top = [&](const double* c) {
+      const double v90 = 0.5 * (c[1] + c[2]) + 0.5 * (c[1] - c[2]) * cos(4.0 * phi);
+      return 0.5 * (c[0] + v90) + 0.5 * (c[0] - v90) * c2t;
+    };
+    auto bridge = [&](const double* c, double ph) {
+      const double v90 = (0.25 * c[1] + 0.5 * c[2] + 0.25 * c[3]) + 0.5 * (c[1] - c[3]) * cos(2.0 * ph) +
+                         (0.25 * c[1] - 0.5 * c[2] + 0.25 * c[3]) * cos(4.0 * ph);
+      return 0.5 * (c[0] + v90) + 0.5 * (c[0] - v90) * c2t;
+    };
+    auto t2h = [&](const double* c, double ph) {
+      const double p45 = ph - kPi / 4.0, p135 = ph + kPi - kPi / 4.0;
+      const double cc0 = 0.25 * (c[1] + c[2] - 2.0 * c[5]);
+      const double b0 = -c[5] + c[2] - 2.0 * cc0;
+      const double a0 = c[2] - b0 - cc0;
+      const double v45 = a0 + b0 * cos(p45) + cc0 * cos(2.0 * p45);
+      const double v135 = a0 + b0 * cos(p135) + cc0 * cos(2.0 * p135);
+      const double v90 = 0.5 * (c[3] + c[4]) + 0.5 * (c[3] - c[4]) * cos(2.0 * p45);
+      return 0.25 * (c[0] + v45 + v90 + v135) + 0.5 * (c[0] - v90) * c2t + 0.5 * (v45 - v135) * s2t +
+             0.25 * (c[0] + v90 - v45 - v135) * c4t;
+    };
+    const double p1 = top(cv), p2 = bridge(cv + 3, phi), p5 = bridge(cv + 3, phi + kPi / 2.0);
+    const double p3 = top(cv + 7), p4 = t2h(cv + 10, phi), p6 = t2h(cv + 10, phi - kPi / 2.0);
+    const double p7 = t2h(cv + 10, phi - kPi), p8 = t2h(cv + 10, phi + kPi / 2.0);
+    const double fac = 2.0 * kPi / s.a;
+    iv = 1.0 / 8.0 * (p1 + p2 + p5 + p3 + p4 + p6 + p7 + p8) * 1.0 + 1.0 / 4.0 * (p1 - p2 + p5 - p3) * cos(fac * gx) +
+         1.0 / 4.0 * (p1 + p2 - p5 - p3) * cos(fac * gy) + 1.0 / 4.0 * (p4 - p6 - p7 + p8) * sin(fac * gx) +
+         1.0 / 4.0 * (p4 + p6 - p7 - p8) * sin(fac * gy) +
+         1.0 / 8.0 * (p1 - p2 - p5 + p3 - p4 + p6 - p7 + p8) * cos(fac * (gx + gy)) +
+         1.0 / 8.0 * (p1 - p2 - p5 + p3 + p4 - p6 + p7 - p8) * cos(fac * (gx - gy)) +
+         1.0 / 16.0 * (p1 + p2 + p5 + p3 - p4 - p6 - p7 - p8) * (cos(fac * (2.0 * gx)) + cos(fac * (2.0 * gy)));
+  } else {
   // the 12 Fourier sites (RRINTII) and the Fourier interpolation (SISII, FOUR6II)
   constexpr double pis3 = kPi / 3.00, pi2s3 = 2.00 * pis3, pis6 = kPi / 6.00;
-  const double c2t = cos(2.00 * theta), s2t = sin(2.00 * theta), c4t = cos(4.00 * theta);
   double site[12];
   {
     const double c6 = cos(6.00 * phi);
@@ -315,11 +445,11 @@ __device__ inline double crp_molecule(const CrpDev& s, const double* a, const do
   fb[9] = cos(p - q);
   fb[10] = cos(2.00 * p) + cos(2.00 * q) + cos(2.00 * (p + q));
   fb[11] = -sin(2.00 * p) - sin(2.00 * q) + sin(2.00 * (p + q));
-  double iv = 0.00;
   for (int j = 0; j < 12; ++j) {
     double c = 0.00;
     for (int k = 0; k < 12; ++k) c = c + kCrpMolecule[k + 12 * j] * site[k];
     iv = iv + c * fb[j];
+  }
   }
   const double va = crp_atom(s, a[0], a[1], a[2], bad);
   const double vb = crp_atom(s, b[0], b[1], b[2], bad);
@@ -343,8 +473,10 @@ __device__ inline double crp_qd(const CrpDev& s, double u, double v, double z, d
     rr = s.r_max - delta;
   }
   if (zz < s.z_min + delta) zz = s.z_min + delta;
-  const double x = 1.0 * u + kCosPi3 * v;
-  const double y = 0.0 * u + kSinPi3 * v;
+  // [[1, cos g], [0, sin g]] (u, v); cos(pi/2) as gfortran folds it for C4v
+  const double cg = s.square ? 0x1.1a62633145c07p-54 : kCosPi3, sg = s.square ? 1.0 : kSinPi3;
+  const double x = 1.0 * u + cg * v;
+  const double y = 0.0 * u + sg * v;
   const double st = sin(theta), ctt = cos(theta), cph = cos(phi), sph = sin(phi);
   const double a[3] = {x - 0.5 * rr * st * cph, y - 0.5 * rr * st * sph, zz - 0.5 * rr * ctt};
   const double b[3] = {x + 0.5 * rr * st * cph, y + 0.5 * rr * st * sph, zz + 0.5 * rr * ctt};

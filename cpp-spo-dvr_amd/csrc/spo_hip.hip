@@ -1568,17 +1568,18 @@ namespace {
 
 // A packed CRP surface checked against its own offsets and uploaded; dev() gives the kernels' view.
 struct CrpUpload {
-  Buffer corrugation, cluster, atom_offsets, atom_splines, cut_shape, cut_offsets, cut_data, h2, zdep;
+  Buffer corrugation, cluster, atom_offsets, atom_splines, cut_shape, cut_offsets, cut_data, h2, zdep, cell;
   spo_dev::CrpDev view{};
 
   explicit CrpUpload(const spo_hip_crp_surface& s) {
-    const int n_basis = s.c3v ? 10 : 7, n_cuts = s.c3v ? 29 : 18;
+    const int n_basis = s.square ? 0 : (s.c3v ? 10 : 7), n_cuts = s.square ? 16 : (s.c3v ? 29 : 18);
     auto bad = [](const char* what) { throw Status(SPO_HIP_E_ARGUMENT, std::string("CRP surface: ") + what); };
     if (s.n_cuts != n_cuts) bad("the number of molecular cuts does not match the symmetry");
     if (s.n_corrugation < 4 || s.n_cluster < 1 || s.n_h2 < 4 || s.n_z_dependence < 4) bad("too few points");
-    if (!s.corrugation || !s.cluster || !s.atom_offsets || !s.atom_splines || !s.cut_shape || !s.cut_offsets ||
-        !s.cut_data || !s.h2 || !s.z_dependence)
+    if (!s.corrugation || !s.cluster || !s.atom_offsets || !s.cut_shape || !s.cut_offsets || !s.cut_data || !s.h2 ||
+        !s.z_dependence || (!s.square && !s.atom_splines) || (s.square && !s.cell_spline))
       bad("NULL array");
+    if (s.square && s.cell_nz < 4) bad("the C4v cell spline needs at least 4 z points");
     if (s.atom_offsets[0] != 0) bad("atom offsets");
     for (int i = 0; i < n_basis; ++i)
       if (s.atom_offsets[i + 1] - s.atom_offsets[i] < 4) bad("an atom cut with fewer than 4 knots");
@@ -1592,7 +1593,9 @@ struct CrpUpload {
     corrugation = upload(s.corrugation, 3 * std::size_t(s.n_corrugation) * sizeof(double));
     cluster = upload(s.cluster, 4 * std::size_t(s.n_cluster) * sizeof(double));
     atom_offsets = upload(s.atom_offsets, (n_basis + 1) * sizeof(int));
-    atom_splines = upload(s.atom_splines, 3 * std::size_t(s.atom_offsets[n_basis]) * sizeof(double));
+    atom_splines = s.square ? Buffer(sizeof(double))
+                            : upload(s.atom_splines, 3 * std::size_t(s.atom_offsets[n_basis]) * sizeof(double));
+    if (s.square) cell = upload(s.cell_spline, (10 + std::size_t(s.cell_nz) + 8 * 25 * std::size_t(s.cell_nz)) * sizeof(double));
     cut_shape = upload(s.cut_shape, 3 * std::size_t(n_cuts) * sizeof(int));
     cut_offsets = upload(s.cut_offsets, (n_cuts + 1) * sizeof(long));
     cut_data = upload(s.cut_data, std::size_t(s.cut_offsets[n_cuts]) * sizeof(double));
@@ -1602,7 +1605,8 @@ struct CrpUpload {
                            s.lattice_constant, s.zso_min, s.zso_max, s.r_min, s.r_max, s.z_min, s.h2_asymptote,
                            corrugation.as<double>(), cluster.as<double>(), atom_offsets.as<int>(),
                            atom_splines.as<double>(), cut_shape.as<int>(), cut_offsets.as<long>(),
-                           cut_data.as<double>(), h2.as<double>(), zdep.as<double>()};
+                           cut_data.as<double>(), h2.as<double>(), zdep.as<double>(), s.square, s.cell_nz,
+                           s.square ? cell.as<double>() : nullptr};
   }
 };
 

@@ -1532,7 +1532,6 @@ std::optional<MoleculeValue> MoleculeSurface::evaluate(const Vec3& a, const Vec3
 
 Packed pack(const QdSurface& q) {
   const MoleculeSurface& m = q.surface();
-  if (m.symmetry() == Symmetry::c4v) throw DataError("the device evaluates C3v and C6v CRP data sets only");
   const AtomSurface& at = m.atom_;
   Packed p;
   auto spline = [](const Spline1D& s, std::vector<double>& out) {
@@ -1550,6 +1549,13 @@ Packed pack(const QdSurface& q) {
   p.h2_asymptote = m.h2_asymptote();
   spline(at.corrugation_, p.corrugation);
   for (const auto& c : at.cluster_) p.cluster.insert(p.cluster.end(), {c.x, c.y, c.z, static_cast<double>(c.layer)});
+  if (at.square_) {
+    const PeriodicSpline3D& c = at.spline_;
+    p.square = 1;
+    p.cell_nz = static_cast<int>(c.z_.size());
+    for (const auto* v : {&c.x_, &c.y_, &c.z_, &c.f_, &c.xx_, &c.yy_, &c.xxyy_, &c.zz_, &c.zzxx_, &c.zzyy_, &c.zzxxyy_})
+      p.cell_spline.insert(p.cell_spline.end(), v->begin(), v->end());
+  }
   p.atom_offsets.push_back(0);
   for (const Spline1D& c : at.cuts_) {
     spline(c, p.atom_splines);

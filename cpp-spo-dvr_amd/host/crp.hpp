@@ -163,6 +163,7 @@ class PeriodicSpline3D {
   const std::vector<double>& z() const { return z_; }
 
  private:
+  friend Packed pack(const QdSurface&);
   std::vector<double> x_, y_, z_, f_, xx_, yy_, xxyy_, zz_, zzxx_, zzyy_, zzxxyy_;
 };
 
@@ -273,6 +274,9 @@ struct Packed {
   std::vector<long> cut_offsets;           // per used cut: start of r[nr] Z[nz] f fxx fyy fxxyy[nr nz each]
   std::vector<double> cut_data;
   std::vector<double> h2, z_dependence;    // x[n] y[n] F''[n]
+  int square = 0;                          // C4v: the cell's periodic 3D spline instead of the atom splines
+  int cell_nz = 0;                         // its z points
+  std::vector<double> cell_spline;         // x[5] y[5] z[nz], then f xx yy xxyy zz zzxx zzyy zzxxyy (25 nz each)
 };
 
 }  // namespace spo::crp

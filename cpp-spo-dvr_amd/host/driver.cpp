@@ -24,10 +24,6 @@ std::shared_ptr<RunSetup> make_run_setup(const InputParameters& input, bool devi
   s->angular = make_angular_grid(s->config);
   s->basis = make_rotational_basis(s->angular, s->config.input.j_in, s->config.input.mj_in);
   const auto pes = make_pes(s->config.input);
-  // the device has the C3v and C6v CRP surfaces; a C4v data set's grid is built on the host
-  if (s->config.input.system_id == "CRPlibrary" &&
-      load_crp_surface(s->config.input.crp_data_set)->surface().symmetry() == crp::Symmetry::c4v)
-    device_potential = false;
   s->potential = device_potential ? compute_potential_grid_on_device(*pes, s->config, s->grids, s->angular)
                                   : compute_potential_grid(*pes, s->config, s->grids, s->angular);
   s->gas = compute_gas_phase(s->config, s->angular, s->potential.asymptotic);

@@ -203,6 +203,9 @@ spo_hip_crp_surface crp_surface_view(const crp::Packed& p) {
   v.h2 = p.h2.data();
   v.n_z_dependence = static_cast<int>(p.z_dependence.size() / 3);
   v.z_dependence = p.z_dependence.data();
+  v.square = p.square;
+  v.cell_nz = p.cell_nz;
+  v.cell_spline = p.cell_spline.empty() ? nullptr : p.cell_spline.data();
   return v;
 }
 

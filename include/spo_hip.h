@@ -211,7 +211,7 @@ int spo_hip_potential_grid(const spo_hip_pes_grid* grid, double* values);
  * to the gas-phase potential up to ZsoMax, eV, minus the free molecule at its last r. libspo_host.so fills it from
  * a data set (host/crp.cpp pack). */
 typedef struct spo_hip_crp_surface {
-  int c3v; /* 1: C3v (10 atom sites, 29 molecular cuts); 0: C6v (7, 18) */
+  int c3v; /* 1: C3v (10 atom sites, 29 molecular cuts); 0: C6v (7, 18) or C4v (square below) */
   double lattice_constant, zso_min, zso_max, r_min, r_max, z_min, h2_asymptote;
   int n_corrugation; const double* corrugation; /* x[n] y[n] F''[n] */
   int n_cluster; const double* cluster;         /* x y z layer per atom, in summation order */
@@ -222,6 +222,9 @@ typedef struct spo_hip_crp_surface {
   const double* cut_data;
   int n_h2; const double* h2;                   /* x y F'' */
   int n_z_dependence; const double* z_dependence;
+  /* C4v (square = 1; 16 molecular cuts, the 32-atom cluster, no atom splines): the cell's periodic 3D spline,
+   * x[5] y[5] z[cell_nz] then f, f_xx, f_yy, f_xxyy, f_zz, f_zzxx, f_zzyy, f_zzxxyy (25 cell_nz each, x fastest) */
+  int square, cell_nz; const double* cell_spline;
 } spo_hip_crp_surface;
 
 /* The SCM correction of a distorted slab snapshot (SystemID CRPDistrtSurface, src/potential/scm/): with it the

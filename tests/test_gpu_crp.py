@@ -42,7 +42,8 @@ def _points(surface, n, seed):
                             rng.uniform(0, math.pi, n), rng.uniform(0, 2 * math.pi, n)])
 
 
-@pytest.mark.parametrize("case", ["cu111_srp48", "c3v_pairwise", "c6v_pairwise", "c3v_impulse_15", "c6v_impulse_5"])
+@pytest.mark.parametrize("case", ["cu111_srp48", "c3v_pairwise", "c6v_pairwise", "c3v_impulse_15", "c6v_impulse_5",
+                                  "c4v_pairwise", "c4v_impulse_6", "c4v_negative_spacing"])
 def test_device_values_match_host(case):
     import spodvr_amd
     _note("device vs host values", case)
@@ -59,13 +60,13 @@ def test_device_values_match_host(case):
         s.close()
 
 
-def _run(where):
+def _run(where, data_set="cu111_srp48"):
     import spodvr_amd
     old = os.environ.get("SPO_HIP_POTENTIAL")
     os.environ["SPO_HIP_POTENTIAL"] = where
     try:
         return spodvr_amd.Run(cases.golden_input("crp_cu111_srp48_small"),
-                              {"CRPlibDataSet": os.path.join(CRP_DIR, "cu111_srp48", "dataset")}, device="hip")
+                              {"CRPlibDataSet": os.path.join(CRP_DIR, data_set, "dataset")}, device="hip")
     finally:
         if old is None:
             os.environ.pop("SPO_HIP_POTENTIAL", None)
@@ -103,6 +104,22 @@ def test_device_crp_grid_matches_host_grid_and_fortran():
     n_tp, n_rep = nth * nph, info.n_representatives
     mine = dv[phi_index.reshape(-1, nph)[op[ixy], ip] + nph * it + n_tp * (rep[ixy] + n_rep * (ir + nr * iz))]
     assert _relative(mine, np.minimum(f["value"], v_max), 1.0 / 27.2114) <= 3e-13
+
+
+def test_device_c4v_grid_matches_host_grid():
+    """The square cell (C4v data set) through the grid build: the crp_cu111_srp48_small grid (Z from -1 bohr, r to
+    9.8 bohr) on the c4v_pairwise data set, device against host."""
+    grids = {}
+    for where in ("device", "host"):
+        _note(where, "C4v CRP grid, c4v_pairwise on the crp_cu111_srp48_small grid")
+        run = _run(where, "c4v_pairwise")
+        try:
+            grids[where] = run.array("potential.values")
+        finally:
+            run.close()
+    assert grids["device"].shape == grids["host"].shape
+    assert np.isfinite(grids["device"]).all()
+    assert _relative(grids["device"], grids["host"], FLOOR_EV / 27.2114) <= TOL
 
 
 # ---------------------------------------------------------------------------------------------------------------
