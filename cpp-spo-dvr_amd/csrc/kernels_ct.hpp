@@ -29,12 +29,13 @@ constexpr int per_thread() {
 }
 
 // SKIP_ZERO: skip the (exact) multiplications by 1 of j == 0, worth a branch when j is uniform over a wavefront.
-template <typename T, int R, bool SKIP_ZERO = true>
+// Table entry k sits at table[k * TS].
+template <typename T, int R, bool SKIP_ZERO = true, int TS = 1>
 __device__ __forceinline__ void twiddle(cx<T>* v, const cx<T>* table, int j, int sign) {
   if (SKIP_ZERO && j == 0) return;
 #pragma unroll
   for (int r = 1; r < R; ++r) {
-    cx<T> w = table[r * j];
+    cx<T> w = table[r * j * TS];
     if (sign > 0) w = conj(w);
     v[r] = v[r] * w;
   }

@@ -20,7 +20,10 @@
 // one phi (phases 1 and 3), or at one column (the first and last phi passes, lanes along rows), land on 16
 // distinct bank groups without
 // padding, which keeps the FP64 tile of 8 x 13 x 48 values at two workgroups per CU. Otherwise rows are padded to
-// n_phi + 1 (an odd stride spreads the rows over the banks just as well).
+// n_phi + 1 (an odd stride spreads the rows over the banks just as well); with the fused phi step the padding
+// column, never part of a row's data, holds the phi twiddles (rows 0 .. n_phi - 1) and the phi map (rows n_phi ..
+// 2 n_phi - 1, the slot's first int), the tile having at least 2 n_phi rows: the complex64 tile of 8 x 13 x 48
+// values then takes 40768 bytes in all, four workgroups per CU.
 //
 // MFMA 16x16x4 operand layouts (lane l of 64): A[i][k] = lane i + 16 k, B[k][j] = lane j + 16 k; result
 // element v of lane l is D[l / 16 + 4 v][l % 16] for f64 and D[4 (l / 16) + v][l % 16] for f32.
@@ -121,7 +124,8 @@ constexpr int phi_exp_per_thread() {
 // Rows past the tile (row >= rows, or a column cc >= cw past the interaction region) read the last valid row
 // instead: their products are never stored, and every load stays inside the table without a branch. The offsets
 // are 32-bit byte offsets from the uniform base.
-template <typename T, int NPHI, int R1, int R2, int C, int THREADS, int Q>
+// phi map entry k sits at phi_map[k * PS].
+template <typename T, int NPHI, int R1, int R2, int C, int THREADS, int PS, int Q>
 __device__ __forceinline__ void phi_exp_load(cx<T> (&ev)[Q][R2], int rows, int cw, const int* phi_map,
                                              const cx<T>* exp_rows, int exp_stride) {
   static_assert(Q == phi_exp_per_thread<NPHI, R1, C, THREADS>(), "exp values per thread");
@@ -134,14 +138,14 @@ __device__ __forceinline__ void phi_exp_load(cx<T> (&ev)[Q][R2], int rows, int c
     const unsigned off = static_cast<unsigned>(exp_stride * cc + NPHI * th);
 #pragma unroll
     for (int r = 0; r < R2; ++r)
-      ev[q][r] = *reinterpret_cast<const cx<T>*>(base + (off + static_cast<unsigned>(phi_map[j + r * R1])) *
+      ev[q][r] = *reinterpret_cast<const cx<T>*>(base + (off + static_cast<unsigned>(phi_map[(j + r * R1) * PS])) *
                                                             static_cast<unsigned>(sizeof(cx<T>)));
   }
 }
 
 // ev holds the exp(i A) values of phi_exp_load once load_ev() has run; load_ev is called at the start of the second
 // pass (a no-op when the caller loaded them earlier).
-template <typename T, int NPHI, int R1, int R2, int C, int THREADS, class Tile, int QE, class LoadEv>
+template <typename T, int NPHI, int R1, int R2, int C, int THREADS, class Tile, int TS, int QE, class LoadEv>
 __device__ __forceinline__ void phi_step_fused(cx<T>* dvr, int rows, const cx<T>* table, cx<T> (&ev)[QE][R2],
                                                LoadEv load_ev) {
   static_assert(R1 * R2 == NPHI, "phi radices");
@@ -184,7 +188,7 @@ __device__ __forceinline__ void phi_step_fused(cx<T>* dvr, int rows, const cx<T>
       if (row < rows) {
 #pragma unroll
         for (int r = 0; r < R2; ++r) v[q][r] = dvr[Tile::at(row, j + r * R1)];
-        twiddle<T, R2, false>(v[q], table, j, +1);
+        twiddle<T, R2, false, TS>(v[q], table, j, +1);
         rdft<R2, T>(v[q], +1);
 #pragma unroll
         for (int r = 0; r < R2; ++r) v[q][r] = ev[q][r] * v[q][r];
@@ -213,7 +217,7 @@ __device__ __forceinline__ void phi_step_fused(cx<T>* dvr, int rows, const cx<T>
         cx<T> v[R1];
 #pragma unroll
         for (int r = 0; r < R1; ++r) v[r] = dvr[Tile::at(row, j + r * R2)];
-        twiddle<T, R1>(v, table, j, -1);
+        twiddle<T, R1, true, TS>(v, table, j, -1);
         rdft<R1, T>(v, -1);
 #pragma unroll
         for (int r = 0; r < R1; ++r) dvr[Tile::at(row, j + r * R2)] = v[r];
@@ -234,10 +238,15 @@ __global__ __launch_bounds__(64 * WAVES) void interaction_mfma_kernel(
   constexpr int C = 8 * NT;
   constexpr int THREADS = 64 * WAVES;
   extern __shared__ unsigned char smem_raw[];
-  // LDS: phi map, phi twiddles, DVR tile.
-  int* phi_map = reinterpret_cast<int*>(smem_raw);
-  cx<T>* tw_phi = reinterpret_cast<cx<T>*>(smem_raw + ((NPHI * sizeof(int) + 15) / 16) * 16);
-  cx<T>* dvr = tw_phi + NPHI;
+  // LDS: phi map, phi twiddles, DVR tile; or the DVR tile alone with the two tables in its padding column.
+  constexpr bool kPackTables = R1 > 0 && !dvr_swizzled<T, NPHI, R1>();
+  constexpr int TS = kPackTables ? Tile::ld : 1;                                        // twiddle stride
+  constexpr int PS = kPackTables ? Tile::ld * static_cast<int>(sizeof(cx<T>) / sizeof(int)) : 1;  // phi map stride
+  constexpr int kTablesBytes = kPackTables ? 0 : ((NPHI * sizeof(int) + 15) / 16) * 16 + NPHI * sizeof(cx<T>);
+  cx<T>* dvr = reinterpret_cast<cx<T>*>(smem_raw + kTablesBytes);
+  int* phi_map = kPackTables ? reinterpret_cast<int*>(dvr + NPHI * Tile::ld + NPHI) : reinterpret_cast<int*>(smem_raw);
+  cx<T>* tw_phi = kPackTables ? dvr + NPHI
+                              : reinterpret_cast<cx<T>*>(smem_raw + ((NPHI * sizeof(int) + 15) / 16) * 16);
   T* dvr_t = reinterpret_cast<T*>(dvr);
 
   const int n_theta = s.n_theta;
@@ -269,7 +278,7 @@ __global__ __launch_bounds__(64 * WAVES) void interaction_mfma_kernel(
     if constexpr (kLateTwiddles)
       tw_entry = table_phi[tid];
     else
-      tw_phi[tid] = table_phi[tid];
+      tw_phi[tid * TS] = table_phi[tid];
   }
   // Slots of |mJ| > MJMax carry no basis function.
   const int n_unused = NPHI - s.n_blocks;
@@ -380,8 +389,8 @@ __global__ __launch_bounds__(64 * WAVES) void interaction_mfma_kernel(
     }
   }
   if (tid < NPHI) {
-    phi_map[tid] = map_entry;
-    if constexpr (kLateTwiddles) tw_phi[tid] = tw_entry;
+    phi_map[tid * PS] = map_entry;
+    if constexpr (kLateTwiddles) tw_phi[tid * TS] = tw_entry;
   }
   __syncthreads();
 
@@ -392,8 +401,8 @@ __global__ __launch_bounds__(64 * WAVES) void interaction_mfma_kernel(
   if constexpr (R1 > 0) {
     cx<T> ev[phi_exp_per_thread<NPHI, R1, C, THREADS>()][R2];
     const cx<T>* exp_rows = exp_table + n_tp * (rep + static_cast<long>(s.n_rep) * izr0);
-    phi_step_fused<T, NPHI, R1, R2, C, THREADS, Tile>(dvr, rows, tw_phi, ev, [&] {
-      phi_exp_load<T, NPHI, R1, R2, C, THREADS>(ev, rows, cw, phi_map, exp_rows, n_tp * s.n_rep);
+    phi_step_fused<T, NPHI, R1, R2, C, THREADS, Tile, TS>(dvr, rows, tw_phi, ev, [&] {
+      phi_exp_load<T, NPHI, R1, R2, C, THREADS, PS>(ev, rows, cw, phi_map, exp_rows, n_tp * s.n_rep);
     });
   } else {
     using Rows = CtLayout<0, Tile::ld, 0, 1>;
@@ -530,6 +539,8 @@ __global__ __launch_bounds__(256) void specular_coupling_kernel(cx<T>* psi, cons
 template <typename T>
 inline std::size_t interaction_mfma_lds(int C, int n_theta, int nphi, bool fused) {
   const bool swizzled = sizeof(T) == 8 && fused && nphi % 16 == 0;
+  if (fused && !swizzled)  // the tables in the padding column of a tile of at least 2 n_phi rows
+    return std::size_t(std::max(C * n_theta, 2 * nphi)) * (nphi + 1) * sizeof(cx<T>);
   return ((nphi * sizeof(int) + 15) / 16) * 16 + std::size_t(nphi) * sizeof(cx<T>) +
          std::size_t(C) * n_theta * (swizzled ? nphi : nphi + 1) * sizeof(cx<T>);
 }
