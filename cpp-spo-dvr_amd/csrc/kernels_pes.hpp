@@ -140,6 +140,12 @@ __global__ __launch_bounds__(256) void potential_grid_kernel(PesGrid g, double* 
   const double ev = g.pes == 3 ? crp_scm_qd(g.crp, g.scm, g.x[ix], g.y[iy], z, g.r[ir], g.theta[it], g.phi[ip])
                                : pes_value(g.pes, g.x[ix], g.y[iy], z, g.r[ir], g.theta[it], g.phi[ip]);
   double v = ev * (1.0 / kHartreeInEv);
+  // A point the surface cannot evaluate (NaN) stays NaN through the switch and the limits, so that the entry point
+  // refuses the grid where the host build stops, also above ZmaxSwitch.
+  if (isnan(v)) {
+    values[i] = v;
+    return;
+  }
   if (g.use_switching && tol_gt(z, g.z_min_switch)) {
     const double va = g.gas_phase[ir];
     if (tol_ge(z, g.z_max_switch)) {

@@ -324,6 +324,16 @@ int spo_run_array(const spo_run* run, const char* name, double* out, size_t capa
       v = s.energies.total;
     } else if (n == "tables.z_optical") v = s.tables.z_optical;
     else if (n == "tables.r_optical") v = s.tables.r_optical;
+    else if (n == "angular.legendre") v = spo::legendre_matrix(s.angular, s.basis);
+    else if (n == "grids.z") v = s.grids.z.points;
+    else if (n == "grids.r") v = s.grids.r.points;
+    else if (n == "derived.cell") {
+      // lattice_x, lattice_y, skew (rad), total mass (m_e), dx, dy, dt, the Cartesian parallel momentum (x, y)
+      const spo::DerivedParameters& d = s.config.derived;
+      const double k_par = d.k_total * std::sin(d.theta_incidence);
+      v = {d.system.lattice_x, d.system.lattice_y, d.system.skew, d.total_mass, d.dx, d.dy, s.config.input.time_step,
+           k_par * std::cos(d.phi_incidence), k_par * std::sin(d.phi_incidence)};
+    }
     else if (n == "initial.specular") {
       const double* d = reinterpret_cast<const double*>(s.initial_specular.data());
       v.assign(d, d + 2 * s.initial_specular.size());

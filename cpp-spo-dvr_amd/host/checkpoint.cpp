@@ -13,6 +13,8 @@
 #include <type_traits>
 #include <utility>
 
+#include "scm.hpp"
+
 namespace spo {
 
 namespace {
@@ -90,8 +92,53 @@ class In {
 
 using Parameters = std::vector<std::pair<std::string, double>>;
 
+// The numeric identity of a run on a distorted surface (SystemID CRPDistrtSurface; src/io/checkpoint.cpp:87-106,
+// statistics as src/potential/scm/scm_diagnostics.cpp:286-311): the snapshot, the cut-off and the rescaling, the
+// slab's lattice constant a_T, the number of slab atoms the correction uses, their rms displacement, the mean
+// displacement (x, y, z) of the top layer, the highest distorted z, and the 2 x 18 values of the parameter file. A
+// restart on another slab, cut-off or parameter set is then refused, and the list is the reference program's.
+void add_scm_identity(Parameters& list, const InputParameters& p) {
+  const scm::Slab slab = scm::read_slab(p.scm.position_file, p.scm.configuration, p.scm.cutoff);
+  const scm::Parameters parameters = scm::read_parameters(p.scm.parameter_file);
+  double squares = 0.0, highest = 0.0;
+  double shift[3] = {0.0, 0.0, 0.0};
+  int top = 0;
+  for (std::size_t k = 0; k < slab.atoms.size(); ++k) {
+    const scm::Atom& a = slab.atoms[k];
+    const double dx = a.distorted.x - a.ideal.x, dy = a.distorted.y - a.ideal.y, dz = a.distorted.z - a.ideal.z;
+    const double d = std::sqrt((dx * dx + dy * dy) + dz * dz);
+    squares = squares + d * d;
+    if (k == 0 || std::isnan(a.distorted.z) || (!std::isnan(highest) && a.distorted.z > highest)) highest = a.distorted.z;
+    if (a.layer != 1) continue;
+    ++top;
+    shift[0] = shift[0] + dx;
+    shift[1] = shift[1] + dy;
+    shift[2] = shift[2] + dz;
+  }
+  const int n = static_cast<int>(slab.atoms.size());
+  const double rms = n > 0 ? std::sqrt(squares / n) : 0.0;
+  for (double& v : shift) v = top > 0 ? v / top : 0.0;
+  list.emplace_back("PosFileSlabNum", double(p.scm.configuration));
+  list.emplace_back("SCMCutoff", p.scm.cutoff);
+  list.emplace_back("EnableSurfExpan", p.scm.rescale ? 1.0 : 0.0);
+  list.emplace_back("PosFilepath (a_T)", slab.lattice_constant);
+  list.emplace_back("PosFilepath (atoms used)", double(n));
+  list.emplace_back("PosFilepath (rms displacement)", rms);
+  list.emplace_back("PosFilepath (layer 1 mean shift x)", shift[0]);
+  list.emplace_back("PosFilepath (layer 1 mean shift y)", shift[1]);
+  list.emplace_back("PosFilepath (layer 1 mean shift z)", shift[2]);
+  list.emplace_back("PosFilepath (highest atom)", highest);
+  for (int layer = 0; layer < 2; ++layer) {
+    const scm::Layer& l = parameters.layers[layer];
+    for (int k = 0; k < 18; ++k) {
+      const double v = k < 8 ? l.intercept[k] : k < 16 ? l.slope[k - 8] : k == 16 ? l.r_clamp_min : l.r_clamp_max;
+      list.emplace_back("SCMParFilepath (layer " + std::to_string(layer + 1) + ", P" + std::to_string(k + 1) + ")", v);
+    }
+  }
+}
+
 // The input parameters a restart must keep, under the reference's names and in its order (format version 2,
-// src/io/checkpoint.cpp:35-119); the SCM identity of its distorted-surface runs does not arise here.
+// src/io/checkpoint.cpp:35-119).
 Parameters restart_parameters(const Config& c, int n_phi) {
   const InputParameters& p = c.input;
   const DerivedParameters& d = c.derived;
@@ -139,6 +186,7 @@ Parameters restart_parameters(const Config& c, int n_phi) {
   };
   low("ZOptLow", p.z_opt_low, d.z_opt_low_factor);
   low("ZSpOptLow", p.zsp_opt_low, d.zsp_opt_low_factor);
+  if (p.system_id == "CRPDistrtSurface") add_scm_identity(list, p);
   return list;
 }
 
@@ -494,10 +542,12 @@ int read_fortran_bk_file(const std::string& path, const Config& c, const Analysi
                          const std::function<void(double, const std::vector<cplx>&)>& on_record) {
   FortranIn f(path);
   const int n_dif_total = static_cast<int>(basis.channels.size());
+  int n_records = 0;
   {
     const auto b = f.record("record 1 (sizes)");
     Fields r{b, "record 1 (sizes)", path};
-    (void)r.next<std::int32_t>();  // the records the file was meant to hold
+    n_records = r.next<std::int32_t>();  // the records the file holds: exactly these are read
+    if (n_records < 0) throw CheckpointError("'" + path + "': negative record count in record 1");
     expect(r.next<std::int32_t>(), n_dif_total * n_rotations, "nstatw", path);
     expect(r.next<std::int32_t>(), n_dif_total, "ndftow", path);
     expect(r.next<std::int32_t>(), n_rotations, "nrotw", path);
@@ -525,22 +575,16 @@ int read_fortran_bk_file(const std::string& path, const Config& c, const Analysi
   (void)f.record("record 10 (channel energies)");
   const std::size_t n = std::size_t(basis.n_states()) * basis.n_v;
   std::vector<cplx> bk(n);
-  int count = 0;
-  for (;;) {
-    std::vector<unsigned char> b;
-    try {
-      b = f.record("a BK record");
-    } catch (const CheckpointError& e) {
-      if (std::string(e.what()).find("ends before") != std::string::npos) break;
-      throw;
-    }
+  // Exactly the records record 1 announces (src/io/fortran_files.cpp:177-183): a file cut short by a killed run
+  // is refused.
+  for (int k = 1; k <= n_records; ++k) {
+    const std::vector<unsigned char> b = f.record("BK record " + std::to_string(k) + " of " + std::to_string(n_records));
     if (b.size() != n * sizeof(cplx))
-      throw CheckpointError("'" + path + "': BK record " + std::to_string(count + 1) + " has the wrong length");
+      throw CheckpointError("'" + path + "': BK record " + std::to_string(k) + " has the wrong length");
     std::memcpy(bk.data(), b.data(), b.size());
-    ++count;
-    on_record(count * deltat, bk);
+    on_record(k * deltat, bk);
   }
-  return count;
+  return n_records;
 }
 
 }  // namespace spo

@@ -3,7 +3,9 @@
 #include "grids.hpp"
 #include "potential.hpp"
 
+#include <algorithm>
 #include <array>
+#include <cctype>
 #include <cmath>
 #include <numeric>
 #include <tuple>
@@ -133,11 +135,23 @@ double wave_packet_average_energy(double e_low, double e_high, double mass) {
   return std::pow(k0, 2) / (2.0 * mass);
 }
 
+std::string canonical_system_id(const std::string& id) {
+  for (const char* known : {"HardWallPES", "H2Cu111DaiLight", "CRPlibrary", "CRPDistrtSurface"}) {
+    const std::string k(known);
+    if (k.size() == id.size() && std::equal(k.begin(), k.end(), id.begin(), [](char a, char b) {
+          return std::tolower(static_cast<unsigned char>(a)) == std::tolower(static_cast<unsigned char>(b));
+        }))
+      return k;
+  }
+  return id;
+}
+
 InputParameters read_input(const json::Value& d) {
   if (d.kind != json::Value::Kind::Object) throw InputError("input", "the document is not a JSON object");
   InputParameters p;
   get(d, "Title", p.title);
   get(d, "SystemID", p.system_id);
+  p.system_id = canonical_system_id(p.system_id);
   get(d, "CRPlibDataSet", p.crp_data_set);
   get(d, "DistortedPESMode", p.scm.mode);
   get(d, "SCMParFilepath", p.scm.parameter_file);

@@ -130,7 +130,12 @@ struct Config {
 SystemParameters system_parameters_for(const std::string& system_id, const std::string& crp_data_set = "");
 SystemParameters system_parameters_for(const struct InputParameters& input);
 
+// The canonical spelling of a SystemID this build knows, matched without case as the reference compares SystemIDs
+// (src/input/json_input.cpp:419-420); other names are returned unchanged.
+std::string canonical_system_id(const std::string& id);
+
 // Reads the keys from a parsed input document; unknown keys are ignored, keys of other subsystems are not read.
+// SystemID is stored in its canonical spelling.
 InputParameters read_input(const json::Value& document);
 InputParameters read_input_file(const std::string& path);
 

@@ -113,8 +113,10 @@ void Propagation::write_checkpoint(const std::string& dir) {
                           psi.n_xy, psi.n_zr, psi.specular, surface_.state());
   }
   if (format != "native") {
+    // Record 1 holds the records the run is meant to write, the earlier run's plus this run's analysis steps
+    // (src/app/run.cpp:236-240); a reader takes exactly that many.
     FortranBkHeader h;
-    h.n_records = static_cast<int>(bk_.size());
+    h.n_records = earlier_records_ + analysis_steps_total();
     h.n_rotations = s.basis.size();
     h.initial_channel = s.basis.initial;
     h.initial_energy = s.gas.initial_energy;

@@ -219,6 +219,18 @@ AngularGrid make_angular_grid(const Config& c) {
   return a;
 }
 
+std::vector<double> legendre_matrix(const AngularGrid& g, const RotationalBasis& basis) {
+  std::vector<double> out;
+  for (int m = -basis.mj_max; m <= basis.mj_max; ++m)
+    for (int th = 0; th < g.n_theta(); ++th) {
+      const std::vector<double> pl = associated_legendre(g.j_max, m, g.thetas[th]);
+      const double w = std::sqrt(g.theta_weights[th] / g.n_phi());
+      for (int j = g.j_min; j <= g.j_max; j += g.delta_j)
+        if (std::abs(m) <= j) out.push_back(pl[j - std::abs(m)] * w);
+    }
+  return out;
+}
+
 RotationalBasis make_rotational_basis(const AngularGrid& g, int j_in, int mj_in) {
   RotationalBasis b;
   b.mj_max = std::min(g.mj_max, g.j_max);

@@ -70,4 +70,8 @@ struct RotationalBasis {
 };
 RotationalBasis make_rotational_basis(const AngularGrid& grid, int j_in, int mj_in);
 
+// The theta part of the angular transform: sqrt(w_t / nphi) P_J^m(theta_t) for every mJ block of the basis, ordered
+// (mJ block, theta, J) (src/propagation/angular_transform.cpp:31-39).
+std::vector<double> legendre_matrix(const AngularGrid& grid, const RotationalBasis& basis);
+
 }  // namespace spo

@@ -146,18 +146,11 @@ HipBackend::HipBackend(const BackendInput& in) {
   const PotentialStepFactors f = potential_step_factors(n_rot, nr, t);
   const std::vector<cplx> spec_gas = specular_gas_phase_factors(nr, n_rot, in.basis.initial, t);
   // sqrt(w_t / nphi) P_J^m(theta_t) ordered (mJ block, theta, J) (src/propagation/angular_transform.cpp:31-39).
-  std::vector<double> legendre;
+  const std::vector<double> legendre = legendre_matrix(in.angular, in.basis);
   std::vector<int> block_first, block_size;
-  for (int m = -in.basis.mj_max; m <= in.basis.mj_max; ++m) {
-    const Range blk = in.basis.blocks[m + in.basis.mj_max];
+  for (const Range& blk : in.basis.blocks) {
     block_first.push_back(blk.first);
     block_size.push_back(blk.size());
-    for (int th = 0; th < in.angular.n_theta(); ++th) {
-      const std::vector<double> pl = associated_legendre(in.angular.j_max, m, in.angular.thetas[th]);
-      const double w = std::sqrt(in.angular.theta_weights[th] / n_phi);
-      for (int j = in.angular.j_min; j <= in.angular.j_max; j += in.angular.delta_j)
-        if (std::abs(m) <= j) legendre.push_back(pl[j - std::abs(m)] * w);
-    }
   }
   const std::vector<cplx> exp_table = exp_potential_table(in.potential, p.time_step);
   std::vector<int> phi_index(std::size_t(n_phi) * sym.ops.size());

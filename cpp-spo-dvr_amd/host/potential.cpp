@@ -153,7 +153,8 @@ std::unique_ptr<Pes> make_pes(const InputParameters& p) {
   return make_pes(p.system_id, p.crp_data_set);
 }
 
-std::unique_ptr<Pes> make_pes(const std::string& id, const std::string& crp_data_set) {
+std::unique_ptr<Pes> make_pes(const std::string& system_id, const std::string& crp_data_set) {
+  const std::string id = canonical_system_id(system_id);
   if (id == "HardWallPES") return std::make_unique<HardWallPes>();
   if (id == "H2Cu111DaiLight") return std::make_unique<DaiLightPes>();
   if (id == "CRPlibrary") return std::make_unique<CrpPes>(load_crp_surface(crp_data_set));

@@ -107,7 +107,9 @@ int spo_run_energy_resolved(spo_run* run, double* sums, double* specular_sums);
  * (ir, j, v), "potential.values" (itp, irep, izr), "potential.asymptotic", "angular.thetas", "angular.weights",
  * "angular.phis", "basis.j", "basis.mj", "state.j", "state.mj", "state.n", "state.m", "state.order",
  * "level.energies" (E(v, j) in eV, j + (jmax+1) v), "symmetry.representatives", "tables.z_optical",
- * "tables.r_optical", "initial.specular" (complex). count receives the element count (doubles, or complex
+ * "tables.r_optical", "initial.specular" (complex), "angular.legendre" (sqrt(w_t / nphi) P_J^m(theta_t), ordered
+ * (mJ block, theta, J)), "grids.z", "grids.r", "derived.cell" (lattice_x, lattice_y, skew angle, total mass, dx, dy,
+ * dt, Cartesian parallel momentum x, y). count receives the element count (doubles, or complex
  * values for complex arrays); out may be NULL to query the count. */
 int spo_run_array(const spo_run* run, const char* name, double* out, size_t capacity, size_t* count);
 

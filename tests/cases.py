@@ -58,6 +58,21 @@ DZ_ANGULAR = dict(SMOKE_DAILIGHT, **{
     "JMax": 24, "MJMax": 23, "UseSymmetry": True,
 })
 
+# The DZ workload's angular basis and 16 x 16 C6v (x, y) grid on a coarse, short (Z, r) grid: the set-up (Legendre
+# matrix at JMax 24, symmetry maps at 16 x 16) at a size whose unreduced potential grid the CPU builds in seconds.
+DZ_SETUP = dict(SMOKE_DAILIGHT, **{
+    "Title": "coarse Dai-Light grid with the DZ workload's angular basis and (x, y) grid",
+    "dZ": 0.32, "nZ": 32, "nZSpecular": 64, "ZOptEnd": 8.92, "ZSpOptEnd": 19.16,
+    "dR": 0.6, "nR": 8, "ROptEnd": 5.0,
+    "nX": 16, "nY": 16, "JMax": 24, "MJMax": 23, "UseSymmetry": True,
+})
+
+# The same with the HardWall workload's angular basis (JMax 10) and an 8 x 8 (x, y) grid.
+DZ_SETUP_8X8 = dict(DZ_SETUP, **{
+    "Title": "coarse Dai-Light grid with JMax 10 on 8 x 8",
+    "nX": 8, "nY": 8, "JMax": 10, "MJMax": 10,
+})
+
 HARDWALL_6D = dict(_COMMON, **{
     "Title": "HardWallPES 6D workload",
     "SystemID": "HardWallPES",

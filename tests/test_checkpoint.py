@@ -245,3 +245,64 @@ def test_fortran_restart_prefers_fort8_and_fort22(split, tmp_path):
     (tmp_path / "fort.8").write_bytes(struct.pack("<iii", 4, 3, 4))  # nproj = 3
     e = _error(path, tmp_path, RestartFormat="fortran")
     assert e.code == 4 and "ProjOp is 2 but 3 in" in str(e) and "fort.8" in str(e)
+
+
+def test_truncated_fortran_bk_file_is_refused(split, tmp_path):
+    """fort.21 announces its record count in record 1; a file cut short (a killed run) is refused, as the reference
+    reads exactly that many records (src/io/fortran_files.cpp:177-183)."""
+    path, _, _ = split
+    first = oracle.run(path, dict(FIRST, CheckpointFormat="fortran"))
+    first.steps(first.info.analysis_steps_total)
+    first.write_checkpoint(str(tmp_path))
+    first.close()
+    data = (tmp_path / "fort.21").read_bytes()
+    recs = _records(data)
+    assert struct.unpack("<4i", recs[0])[0] == len(recs) - 10 == 4
+    last = len(recs[-1]) + 8
+    (tmp_path / "fort.21").write_bytes(data[:-last])
+    e = _error(path, tmp_path, RestartFormat="fortran")
+    assert e.code == 4 and "BK record 4 of 4" in str(e) and "fort.21" in str(e)
+
+
+SCM_CASE = "scm_cu111_srp48_small"
+
+# The SCM identity the reference appends for SystemID CRPDistrtSurface (src/io/checkpoint.cpp:87-106).
+SCM_IDENTITY = ["PosFileSlabNum", "SCMCutoff", "EnableSurfExpan", "PosFilepath (a_T)", "PosFilepath (atoms used)",
+                "PosFilepath (rms displacement)", "PosFilepath (layer 1 mean shift x)",
+                "PosFilepath (layer 1 mean shift y)", "PosFilepath (layer 1 mean shift z)",
+                "PosFilepath (highest atom)"] + [f"SCMParFilepath (layer {l}, P{k})" for l in (1, 2) for k in range(1, 19)]
+
+
+@pytest.fixture(scope="module")
+def scm_checkpoint(tmp_path_factory):
+    """The native restart files of the SCM case after its first analysis step (t = 50)."""
+    d = tmp_path_factory.mktemp("scm_restart")
+    over = dict(cases.golden_overrides(SCM_CASE), nStep=20, tEnd=50.0)
+    run = oracle.run(cases.golden_input(SCM_CASE), over)
+    run.steps(1)
+    run.write_checkpoint(str(d))
+    run.close()
+    return d, dict(over, NewCalc=False, RestartDir=str(d), tStart=50.0, tEnd=100.0)
+
+
+def test_scm_files_carry_the_scm_identity(scm_checkpoint):
+    d, restart = scm_checkpoint
+    run = oracle.run(cases.golden_input(SCM_CASE), restart)  # the same set-up continues
+    run.close()
+    _, _, system_id, params, _ = _read_header((d / "checkpoint.bin").read_bytes())
+    names = [p[0] for p in params]
+    assert system_id == "CRPDistrtSurface"
+    assert names == FORMAT_2 + SCM_IDENTITY
+    values = dict(params)
+    assert values["PosFileSlabNum"] == 1 and values["SCMCutoff"] == 16.0 and values["EnableSurfExpan"] == 1.0
+    assert values["PosFilepath (atoms used)"] > 0 and values["PosFilepath (rms displacement)"] > 0
+    assert values["PosFilepath (a_T)"] == pytest.approx(4.8896663475)
+
+
+@pytest.mark.parametrize("key,value,text", [("SCMCutoff", 12.0, "SCMCutoff is 12 in the input but 16"),
+                                            ("EnableSurfExpan", False, "EnableSurfExpan is 0 in the input but 1")])
+def test_scm_restart_on_another_slab_set_up_is_refused(scm_checkpoint, key, value, text):
+    d, over = scm_checkpoint
+    with pytest.raises(spodvr_amd.SpoError) as e:
+        oracle.run(cases.golden_input(SCM_CASE), dict(over, **{key: value}))
+    assert e.value.code == 4 and text in str(e.value)

@@ -48,3 +48,15 @@ def test_restart_and_time_keys_are_checked(overrides, key, message):
     with pytest.raises(spodvr_amd.SpoError) as e:
         oracle.run(cases.write_case("invalid_restart_" + key, params))
     assert e.value.code == 2 and key in str(e.value) and message in str(e.value)
+
+
+@pytest.mark.parametrize("spelling", ["h2cu111dailight", "H2CU111DAILIGHT", "H2Cu111Dailight"])
+def test_system_id_is_matched_without_case(spelling):
+    """The reference compares SystemIDs without case (src/input/json_input.cpp:419-420): any spelling runs, with
+    the same potential as the canonical one."""
+    canonical = oracle.run(cases.write_case("sysid_canonical", cases.SMOKE_DAILIGHT))
+    other = oracle.run(cases.write_case("sysid_" + spelling, dict(cases.SMOKE_DAILIGHT, SystemID=spelling)))
+    import numpy as np
+    assert np.array_equal(canonical.array("potential.values"), other.array("potential.values"))
+    canonical.close()
+    other.close()
