@@ -37,6 +37,9 @@ WORKLOADS = {
     "dz": ("DZ_6D", "H2/Cu(111) Dai-Light LEPS 6D: NZ 96 x NR 48 x 16 x 16, JMax 24 / MJMax 23 "
                     "(13 theta x 48 phi), C6v-reduced potential, specular grid 512"),
     "hardwall": ("HARDWALL_6D", "HardWallPES 6D: NZ 128 x NR 32 x 8 x 8, JMax 10 (6 theta x 22 phi)"),
+    "batch12": ("BATCH_12", "12 DZ-LEPS runs batched in complex64: initial states (v, j, mJ) on the 96 x 48 x 16 x 16 "
+                            "grid (JMax 24 / 23) and four at JMax 12 / 18 with NR 32; one hipGraph launch per "
+                            "analysis step for the batch"),
 }
 
 
@@ -99,23 +102,53 @@ def synthetic_wave(run, seed: int = 2024):
     return s, run.specular()
 
 
+def host_description() -> dict:
+    """The CPU the baseline runs on: model, logical CPUs available to this process, NUMA nodes."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    node_dir = "/sys/devices/system/node"
+    numa = len([d for d in os.listdir(node_dir) if d.startswith("node") and d[4:].isdigit()]) \
+        if os.path.isdir(node_dir) else 1
+    available = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    return {"cpu_model": model, "logical_cpus": available, "numa_nodes": numa}
+
+
 def cpu_baseline(path: str, sample_steps: int, wave) -> dict:
     """The CPU oracle (FP64 OpenMP restatement of the reference's CPU backend) on the same grid and wave packet:
-    sample_steps full time steps (specular grid, T_Zr, T_xy, V), timed with a monotonic clock after one warm-up."""
+    whole analysis steps (the reference's schedule, src/propagation/split_operator_schedule.cpp:7-13, followed by
+    the analysis-line projection), at least sample_steps time steps, timed with a monotonic clock after one warm-up
+    analysis step, as the reference's operator timing does (one warm-up call, then the mean of N;
+    src/app/operator_timing.cpp:26-32)."""
     from oracle import oracle
     cpu = oracle.run(path, {"Precision": "fp64"})
+    spa = cpu.info.steps_per_analysis
+    n_analysis = max(1, -(-sample_steps // spa))
     cpu.set_wave(*wave)
-    cpu.kinetic_and_potential(False, False)
+    cpu.propagate()
+    cpu.analyse(cpu.info.time_step * spa)
     t0 = time.perf_counter()
-    for _ in range(sample_steps):
-        cpu.kinetic_and_potential(False, False)
-    dt = (time.perf_counter() - t0) / sample_steps
-    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    for k in range(n_analysis):
+        cpu.propagate()
+        cpu.analyse(cpu.info.time_step * spa * (k + 2))
+    dt = (time.perf_counter() - t0) / (n_analysis * spa)
+    threads = int(os.environ.get("OMP_NUM_THREADS", 0)) or host_description()["logical_cpus"]
     value = dvr_points(cpu.info) / dt
     cpu.close()
-    return {"value": value, "unit": "grid-point-steps/s", "cores": cores, "kind": "port",
-            "sample": f"{sample_steps} full time step(s) of the same grid after one warm-up step "
-                      f"({dt:.2f} s per step, FP64 OpenMP oracle, {cores} threads)"}
+    host = host_description()
+    return {"value": value, "unit": "grid-point-steps/s", "cores": threads, "kind": "port",
+            "sample": f"{n_analysis} analysis step(s) of {spa} time steps (schedule, BK projection and S-matrix sums) "
+                      f"on the same grid and wave packet after one warm-up analysis step ({dt:.2f} s per time step, "
+                      f"FP64 OpenMP oracle with its own Stockham FFT, {threads} threads)",
+            "host": dict(host, omp_threads=threads,
+                         omp_schedule="static for the kinetic passes and the analysis, dynamic,1 for the potential "
+                                      "step (the reference's one-NUMA-node choice, src/core/loop_schedule.hpp:6-18)")}
 
 
 def measure(path: str, workload: str, precision: str, steps: int, warmup: int) -> dict:
@@ -167,6 +200,38 @@ def measure(path: str, workload: str, precision: str, steps: int, warmup: int) -
             "kernels": {k: {"launches": n, "ms_per_time_step": round(ms / n_steps, 4)} for k, (n, ms) in kernels.items()}}
 
 
+def measure_batch(steps: int, warmup: int) -> dict:
+    """The 12-run batch on the GPU in graph mode (SPO_HIP_GRAPH=1: every analysis step of the batch is one launch of
+    a hipGraph captured over all runs' streams); throughput summed over the runs. Each run starts from its own
+    synthetic wave packet."""
+    import spodvr_amd
+    from tests import cases
+    os.environ["SPO_HIP_GRAPH"] = "1"
+    runs, points = [], 0
+    for name, params in cases.BATCH_12:
+        run = spodvr_amd.Run(cases.write_case(f"bench_batch_{name}", params), device="hip")
+        run.set_wave(*synthetic_wave(run, seed=len(runs) + 1))
+        points += dvr_points(run.info)
+        runs.append(run)
+    spa = runs[0].info.steps_per_analysis
+    timed = max(1, -(-steps // spa))
+    try:
+        if warmup > 0:
+            spodvr_amd.run_batch(runs, max(1, -(-warmup // spa)))
+        t0 = time.perf_counter()
+        spodvr_amd.run_batch(runs, timed)
+        elapsed = time.perf_counter() - t0
+    finally:
+        members = [{"name": name, "n_rot": r.info.n_rot, "nr": r.info.nr, "n_theta": r.info.n_theta,
+                    "n_phi": r.info.n_phi, "dvr_points": dvr_points(r.info)} for (name, _), r in zip(cases.BATCH_12, runs)]
+        for r in runs:
+            r.close()
+        os.environ.pop("SPO_HIP_GRAPH", None)
+    n_steps = timed * spa
+    return {"value": points * n_steps / elapsed, "ms_per_step": elapsed / n_steps * 1e3, "steps": n_steps,
+            "members": members, "dvr_points": points}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -177,6 +242,10 @@ def main() -> int:
                     help="both: the mixed-precision line with the FP64 run beside it")
     ap.add_argument("--cpu-steps", type=int, default=10, help="time steps of the CPU baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--hardwall-steps", type=int, default=500,
+                    help="time steps of the HardWall FP64 line beside the DZ line (0: none)")
+    ap.add_argument("--hardwall-cpu-steps", type=int, default=100,
+                    help="time steps of the HardWall CPU sample (its full run is 2000)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run device copies of BYTES (the counter calibration of the traffic figures)")
     args = ap.parse_args()
@@ -189,6 +258,17 @@ def main() -> int:
         return 2
 
     from tests import cases
+
+    if args.workload == "batch12":
+        b = measure_batch(args.steps, args.warmup)
+        print(json.dumps({
+            "metric": METRIC, "value": b["value"], "unit": "grid-point-steps/s", "n_gpus": 1, "steps": b["steps"],
+            "warmup": args.warmup, "ms_per_step": b["ms_per_step"], "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "f32 (complex64 waves, FP64 phases and analysis)", "data": "synthetic",
+            "config": {"workload": WORKLOADS["batch12"][1], "precision": "mixed", "runs": len(b["members"]),
+                       "dvr_points_summed": b["dvr_points"], "members": b["members"]},
+            "roofline": None, "cpu_baseline": None}), flush=True)
+        return 0
 
     name, description = WORKLOADS[args.workload]
     path = cases.write_case(f"bench_{args.workload}", dict(getattr(cases, name)))
@@ -205,6 +285,16 @@ def main() -> int:
     def brief(m):
         return None if m is None else {"value": m["value"], "ms_per_step": m["ms_per_step"], "roofline": m["roofline"],
                                         "step_roofline_frac": m["step_roofline_frac"], "kernels": m["kernels"]}
+
+    # The HardWall workload (FP64, the reference's CPU-runnable case) beside the DZ line, with its own CPU sample.
+    hardwall = None
+    if args.workload == "dz" and args.hardwall_steps > 0:
+        hname, hdesc = WORKLOADS["hardwall"]
+        hpath = cases.write_case("bench_hardwall", dict(getattr(cases, hname)))
+        h = measure(hpath, "hardwall", "fp64", args.hardwall_steps, args.warmup)
+        hardwall = dict(brief(h), workload=hdesc, precision="fp64", steps=h["steps"], dvr_points=dvr_points(h["info"]))
+        if not args.no_cpu_baseline:
+            hardwall["cpu_baseline"] = cpu_baseline(hpath, args.hardwall_cpu_steps, h["wave"])
 
     line = {
         "metric": METRIC,
@@ -228,6 +318,7 @@ def main() -> int:
         "fp64": brief(fp64),
         "mixed": brief(mixed),
         "cpu_baseline": cpu,
+        "hardwall": hardwall,
     }
     print(json.dumps(line), flush=True)
     return 0

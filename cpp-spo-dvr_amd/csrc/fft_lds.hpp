@@ -11,6 +11,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include "rounding.hpp"
+
 namespace spo_dev {
 
 // Aligned to its size, so that a complex value moves as one 16-byte (FP64) or 8-byte (FP32) access: in LDS one
@@ -73,11 +75,10 @@ __device__ __forceinline__ void odd_dft(cx<T>* v, int sign, const double* c, con
     for (int q = 1; q <= H; ++q) {
       const int idx = (k * q) % R;
       const int qq = idx <= H ? idx : R - idx;       // cos(2 pi idx / R) = c[qq]
-      const T sgn = idx <= H ? T(1) : T(-1);         // sin(2 pi idx / R) = sgn * s[qq]
-      const T cc = T(c[qq - 1]);
-      const T ss = T(s[qq - 1]) * sgn;
-      re = re + scale(a[q - 1], cc);
-      im = im + scale(b[q - 1], ss);
+      const double sgn = idx <= H ? 1.0 : -1.0;      // sin(2 pi idx / R) = sgn * s[qq]
+      // the constants take part with split precision in FP32 (fma_const), so the DFT stays unitary to ~1e-14
+      re = {fma_const(c[qq - 1], a[q - 1].x, re.x), fma_const(c[qq - 1], a[q - 1].y, re.y)};
+      im = {fma_const(sgn * s[qq - 1], b[q - 1].x, im.x), fma_const(sgn * s[qq - 1], b[q - 1].y, im.y)};
     }
     // y_k = re + sign i im, y_{R-k} = re - sign i im
     const cx<T> jim = times_i(im, sign);
@@ -104,10 +105,12 @@ __device__ __forceinline__ void dft(cx<T>* v, int sign) {
     v[0] = a + b;
     v[1] = a - b;
   } else if constexpr (R == 3) {
-    const T s = T(0.8660254037844387) * T(sign);
+    // sqrt(3)/2 with split precision in FP32: a rounded float would scale the difference modes by 1 - 1.8e-8 at
+    // every transform.
+    const double s = 0.8660254037844386 * sign;
     const cx<T> t1 = v[1] + v[2], t2 = v[1] - v[2];
     const cx<T> m = {v[0].x - T(0.5) * t1.x, v[0].y - T(0.5) * t1.y};
-    const cx<T> js = {-s * t2.y, s * t2.x};
+    const cx<T> js = {mul_const(-s, t2.y), mul_const(s, t2.x)};
     v[0] = v[0] + t1;
     v[1] = m + js;
     v[2] = m - js;
@@ -124,12 +127,14 @@ __device__ __forceinline__ void dft(cx<T>* v, int sign) {
     cx<T> o[4] = {v[1], v[3], v[5], v[7]};
     dft<4, T>(e, sign);
     dft<4, T>(o, sign);
-    const T h = T(0.7071067811865476);
+    // w^k = exp(sign i pi k / 4): w^1 = h (1 + sign i), w^3 = h (-1 + sign i), h = sqrt(1/2) with split
+    // precision in FP32 (mul_const): one rounded float h gives |w|^2 = 2 h^2 = 1 - 3.4e-8, the same shrink of these
+    // outputs at every transform
+    constexpr double h = 0.7071067811865476;
     const T sg = T(sign);
-    // w^k = exp(sign i pi k / 4): w^1 = h (1 + sign i), w^3 = h (-1 + sign i)
-    o[1] = {h * (o[1].x - sg * o[1].y), h * (o[1].y + sg * o[1].x)};
+    o[1] = {mul_const(h, o[1].x - sg * o[1].y), mul_const(h, o[1].y + sg * o[1].x)};
     o[2] = times_i(o[2], sign);
-    o[3] = {h * (-o[3].x - sg * o[3].y), h * (-o[3].y + sg * o[3].x)};
+    o[3] = {mul_const(h, -o[3].x - sg * o[3].y), mul_const(h, -o[3].y + sg * o[3].x)};
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
       v[k] = e[k] + o[k];

@@ -29,32 +29,64 @@ constexpr int per_thread() {
 }
 
 // SKIP_ZERO: skip the (exact) multiplications by 1 of j == 0, worth a branch when j is uniform over a wavefront.
-// Table entry k sits at table[k * TS].
-template <typename T, int R, bool SKIP_ZERO = true, int TS = 1>
-__device__ __forceinline__ void twiddle(cx<T>* v, const cx<T>* table, int j, int sign) {
+// Table entry k sits at table[k * TS]. In FP32 the table entries cannot all be of modulus 1 (no pair of floats near
+// an eighth turn (h, h) is), so each comes with eps[k * ES] = 1 / |w_k| - 1 (a few 1e-8, from FP64) and the product
+// is corrected as p + eps p: the twiddles then keep the norm to ~1e-15 instead of shrinking the modes they multiply
+// by up to ~3e-8 at every transform. FP64 passes no eps.
+template <typename T, int R, bool SKIP_ZERO = true, int TS = 1, int ES = 1>
+__device__ __forceinline__ void twiddle(cx<T>* v, const cx<T>* table, int j, int sign, const T* eps = nullptr) {
   if (SKIP_ZERO && j == 0) return;
 #pragma unroll
   for (int r = 1; r < R; ++r) {
     cx<T> w = table[r * j * TS];
     if (sign > 0) w = conj(w);
     v[r] = v[r] * w;
+    if constexpr (sizeof(T) == 4) {
+      const T e = eps[r * j * ES];
+      v[r] = {fmaf(v[r].x, e, v[r].x), fmaf(v[r].y, e, v[r].y)};
+    }
   }
 }
+
+// w v for a factor w of a complex64 table whose entry idx has the modulus correction eps[idx] = |w_exact| / |w| - 1
+// (the product's modulus then matches the FP64 factor's to ~1e-15; see twiddle()); FP64 passes no eps.
+template <typename T>
+__device__ __forceinline__ cx<T> mul_factor(cx<T> w, cx<T> v, const T* eps, int idx) {
+  cx<T> p = w * v;
+  if constexpr (sizeof(T) == 4) {
+    const T e = eps[idx];
+    p = {fmaf(p.x, e, p.x), fmaf(p.y, e, p.y)};
+  }
+  return p;
+}
+
+// The complex64 modulus corrections of the tables xy_reg_kernel multiplies with (NULL in FP64).
+template <typename T>
+struct XyEps {
+  const T* factor;
+  const T* shift_f;
+  const T* shift_b;
+  const T* before;
+  const T* gas_step;
+};
 
 template <typename T>
 constexpr bool zr_twiddles_in_lds() {
   return sizeof(T) == 4;
 }
 
-// LDS bytes of zr_reg_kernel.
+// LDS bytes of zr_reg_kernel (complex64: the twiddles and their modulus corrections behind the plane).
 template <typename T>
 inline std::size_t zr_reg_lds(int nz, int nr) {
-  return (std::size_t(nz) * (nr + 1) + (zr_twiddles_in_lds<T>() ? nz + nr : 0)) * sizeof(cx<T>);
+  return std::size_t(nz) * (nr + 1) * sizeof(cx<T>) +
+         (zr_twiddles_in_lds<T>() ? std::size_t(nz + nr) * (sizeof(cx<T>) + sizeof(T)) : 0);
 }
 
+// eps_z, eps_r: the complex64 twiddle tables' modulus corrections (twiddle()); unused in FP64.
 template <typename T, int NZ, int NR, int A1, int A2, int B1, int B2, int THREADS>
 __global__ __launch_bounds__(THREADS) void zr_reg_kernel(cx<T>* psi, const cx<T>* table_z, const cx<T>* table_r,
-                                                         const cx<T>* factor) {
+                                                         const cx<T>* factor, const T* eps_z, const T* eps_r,
+                                                         const T* factor_eps) {
   static_assert(A1 * A2 == NR && B1 * B2 == NZ, "zr_reg_kernel: radices must multiply to the lengths");
   extern __shared__ unsigned char smem_raw[];
   cx<T>* t = reinterpret_cast<cx<T>*>(smem_raw);
@@ -68,10 +100,20 @@ __global__ __launch_bounds__(THREADS) void zr_reg_kernel(cx<T>* psi, const cx<T>
   if constexpr (zr_twiddles_in_lds<T>()) {
     cx<T>* tw_r = t + NZ * LD;
     cx<T>* tw_z = tw_r + NR;
-    for (int e = tid; e < NR; e += THREADS) tw_r[e] = table_r[e];
-    for (int e = tid; e < NZ; e += THREADS) tw_z[e] = table_z[e];
+    T* e_r = reinterpret_cast<T*>(tw_z + NZ);
+    T* e_z = e_r + NR;
+    for (int e = tid; e < NR; e += THREADS) {
+      tw_r[e] = table_r[e];
+      e_r[e] = eps_r[e];
+    }
+    for (int e = tid; e < NZ; e += THREADS) {
+      tw_z[e] = table_z[e];
+      e_z[e] = eps_z[e];
+    }
     table_r = tw_r;
     table_z = tw_z;
+    eps_r = e_r;
+    eps_z = e_z;
   }
 
   // Pass 1, r forward, radix A1: butterfly (iz, j < A2) from HBM.
@@ -102,7 +144,7 @@ __global__ __launch_bounds__(THREADS) void zr_reg_kernel(cx<T>* psi, const cx<T>
       cx<T> v[A2];
 #pragma unroll
       for (int r = 0; r < A2; ++r) v[r] = t[iz * LD + j + r * A1];
-      twiddle<T, A2>(v, table_r, j, -1);
+      twiddle<T, A2>(v, table_r, j, -1, eps_r);
       rdft<A2, T>(v, -1);
 #pragma unroll
       for (int r = 0; r < A2; ++r) t[iz * LD + j + r * A1] = v[r];
@@ -144,10 +186,13 @@ __global__ __launch_bounds__(THREADS) void zr_reg_kernel(cx<T>* psi, const cx<T>
       const int kr = b % NR, j = b / NR;
 #pragma unroll
       for (int r = 0; r < B2; ++r) v[q][r] = t[(j + r * B1) * LD + kr];
-      twiddle<T, B2>(v[q], table_z, j, -1);
+      twiddle<T, B2>(v[q], table_z, j, -1, eps_z);
       rdft<B2, T>(v[q], -1);
 #pragma unroll
-      for (int r = 0; r < B2; ++r) v[q][r] = factor[(j + r * B1) * NR + kr] * v[q][r];
+      for (int r = 0; r < B2; ++r) {
+        const int f = (j + r * B1) * NR + kr;
+        v[q][r] = mul_factor(factor[f], v[q][r], factor_eps, f);
+      }
       rdft<B2, T>(v[q], +1);
     }
     __syncthreads();
@@ -172,7 +217,7 @@ __global__ __launch_bounds__(THREADS) void zr_reg_kernel(cx<T>* psi, const cx<T>
       cx<T> v[B1];
 #pragma unroll
       for (int r = 0; r < B1; ++r) v[r] = t[(j + r * B2) * LD + kr];
-      twiddle<T, B1>(v, table_z, j, +1);
+      twiddle<T, B1>(v, table_z, j, +1, eps_z);
       rdft<B1, T>(v, +1);
 #pragma unroll
       for (int r = 0; r < B1; ++r) t[(j + r * B2) * LD + kr] = v[r];
@@ -214,7 +259,7 @@ __global__ __launch_bounds__(THREADS) void zr_reg_kernel(cx<T>* psi, const cx<T>
       cx<T> v[A1];
 #pragma unroll
       for (int r = 0; r < A1; ++r) v[r] = t[iz * LD + j + r * A2];
-      twiddle<T, A1>(v, table_r, j, +1);
+      twiddle<T, A1>(v, table_r, j, +1, eps_r);
       rdft<A1, T>(v, +1);
 #pragma unroll
       for (int r = 0; r < A1; ++r) plane[iz * NR + j + r * A2] = v[r];
@@ -246,7 +291,7 @@ template <typename T, int NX, int NY, int C, bool SPLIT = false>
 __global__ __launch_bounds__(C * NX) __attribute__((amdgpu_waves_per_eu(3))) void xy_reg_kernel(cx<T>* psi, Shape s, const cx<T>* factor,
                                                          const cx<T>* shift_f, const cx<T>* shift_b, int needs_shift,
                                                          int apply_potential, const T* z_opt, const cx<T>* before,
-                                                         const cx<T>* gas_step) {
+                                                         const cx<T>* gas_step, XyEps<T> eps) {
   static_assert(single_radix<NX>() && single_radix<NY>() && NX == NY, "xy_reg_kernel: one radix per axis, NX = NY");
   static_assert(!SPLIT || NY % 2 == 0, "xy_reg_kernel: halves of the y lines");
   extern __shared__ unsigned char smem_raw[];
@@ -270,7 +315,7 @@ __global__ __launch_bounds__(C * NX) __attribute__((amdgpu_waves_per_eu(3))) voi
   for (int iy = 0; iy < NY; ++iy) {
     const int ixy = a * NY + iy;
     v[iy] = live ? g[xy_stride * ixy] : cx<T>{T(0), T(0)};
-    if (needs_shift) v[iy] = v[iy] * shift_f[ixy];
+    if (needs_shift) v[iy] = mul_factor(shift_f[ixy], v[iy], eps.shift_f, ixy);
   }
   rdft<NY, T>(v, -1);
   cx<T> w[NX];
@@ -296,7 +341,7 @@ __global__ __launch_bounds__(C * NX) __attribute__((amdgpu_waves_per_eu(3))) voi
   }
   rdft<NX, T>(w, -1);
 #pragma unroll
-  for (int ix = 0; ix < NX; ++ix) w[ix] = factor[ix * NY + a] * w[ix];
+  for (int ix = 0; ix < NX; ++ix) w[ix] = mul_factor(factor[ix * NY + a], w[ix], eps.factor, ix * NY + a);
   rdft<NX, T>(w, +1);
   if constexpr (!SPLIT) {
 #pragma unroll
@@ -323,16 +368,23 @@ __global__ __launch_bounds__(C * NX) __attribute__((amdgpu_waves_per_eu(3))) voi
   const int iz = izr / s.nr, ir = izr % s.nr;
   cx<T> f = {T(1), T(0)};
   T zo = T(1);
+  T fe = T(0);  // the modulus correction of f (complex64)
   if (apply_potential) {
-    f = iz >= s.nzpot ? gas_step[irot + s.n_rot * ir] : before[irot + s.n_rot * ir];
+    const int k = irot + s.n_rot * ir;
+    f = iz >= s.nzpot ? gas_step[k] : before[k];
+    if constexpr (sizeof(T) == 4) fe = iz >= s.nzpot ? eps.gas_step[k] : eps.before[k];
     zo = z_opt[iz];
   }
 #pragma unroll
   for (int iy = 0; iy < NY; ++iy) {
     const int ixy = a * NY + iy;
     cx<T> u = v[iy];
-    if (needs_shift) u = u * shift_b[ixy];
-    if (apply_potential) u = scale(f * u, zo);
+    if (needs_shift) u = mul_factor(shift_b[ixy], u, eps.shift_b, ixy);
+    if (apply_potential) {
+      u = f * u;
+      if constexpr (sizeof(T) == 4) u = {fmaf(u.x, fe, u.x), fmaf(u.y, fe, u.y)};
+      u = scale(u, zo);
+    }
     g[xy_stride * ixy] = u;
   }
 }

@@ -144,10 +144,11 @@ __device__ __forceinline__ void phi_exp_load(cx<T> (&ev)[Q][R2], int rows, int c
 }
 
 // ev holds the exp(i A) values of phi_exp_load once load_ev() has run; load_ev is called at the start of the second
-// pass (a no-op when the caller loaded them earlier).
-template <typename T, int NPHI, int R1, int R2, int C, int THREADS, class Tile, int TS, int QE, class LoadEv>
-__device__ __forceinline__ void phi_step_fused(cx<T>* dvr, int rows, const cx<T>* table, cx<T> (&ev)[QE][R2],
-                                               LoadEv load_ev) {
+// pass (a no-op when the caller loaded them earlier). complex64: eps[k * ES] is twiddle k's modulus correction
+// (twiddle(), kernels_ct.hpp); FP64 passes none.
+template <typename T, int NPHI, int R1, int R2, int C, int THREADS, class Tile, int TS, int ES, int QE, class LoadEv>
+__device__ __forceinline__ void phi_step_fused(cx<T>* dvr, int rows, const cx<T>* table, const T* eps,
+                                               cx<T> (&ev)[QE][R2], LoadEv load_ev) {
   static_assert(R1 * R2 == NPHI, "phi radices");
   static_assert(QE == phi_exp_per_thread<NPHI, R1, C, THREADS>(), "exp values per thread");
   constexpr int ROWS_MAX = 16 * C;
@@ -188,7 +189,7 @@ __device__ __forceinline__ void phi_step_fused(cx<T>* dvr, int rows, const cx<T>
       if (row < rows) {
 #pragma unroll
         for (int r = 0; r < R2; ++r) v[q][r] = dvr[Tile::at(row, j + r * R1)];
-        twiddle<T, R2, false, TS>(v[q], table, j, +1);
+        twiddle<T, R2, false, TS, ES>(v[q], table, j, +1, eps);
         rdft<R2, T>(v[q], +1);
 #pragma unroll
         for (int r = 0; r < R2; ++r) v[q][r] = ev[q][r] * v[q][r];
@@ -217,7 +218,7 @@ __device__ __forceinline__ void phi_step_fused(cx<T>* dvr, int rows, const cx<T>
         cx<T> v[R1];
 #pragma unroll
         for (int r = 0; r < R1; ++r) v[r] = dvr[Tile::at(row, j + r * R2)];
-        twiddle<T, R1, true, TS>(v, table, j, -1);
+        twiddle<T, R1, true, TS, ES>(v, table, j, -1, eps);
         rdft<R1, T>(v, -1);
 #pragma unroll
         for (int r = 0; r < R1; ++r) dvr[Tile::at(row, j + r * R2)] = v[r];
@@ -231,7 +232,7 @@ template <typename T, int NPHI, int NT, int WAVES, int R1 = 0, int R2 = 0>
 __global__ __launch_bounds__(64 * WAVES) void interaction_mfma_kernel(
     cx<T>* psi, const cx<double>* spec, Shape s, const cx<T>* table_phi, AngularTables at, MfmaTables mt,
     const T* frag_fwd, const T* frag_bwd, int max_jt, const cx<T>* exp_table, const cx<T>* rot_t,
-    const cx<T>* s2sc) {
+    const cx<T>* s2sc, const T* eps_phi) {
   using M = Mfma<T>;
   using acc_t = typename M::acc_t;
   using Tile = DvrTile<NPHI, dvr_swizzled<T, NPHI, R1>()>;
@@ -272,9 +273,11 @@ __global__ __launch_bounds__(64 * WAVES) void interaction_mfma_kernel(
   constexpr bool kLateTwiddles = sizeof(T) == 4;
   const int op = at.op_of[ixy];
   int map_entry = 0;
+  T eps_entry = T(0);
   cx<T> tw_entry = {};
   if (tid < NPHI) {
     map_entry = at.phi_index[tid + NPHI * op];
+    if constexpr (sizeof(T) == 4) eps_entry = eps_phi[tid];
     if constexpr (kLateTwiddles)
       tw_entry = table_phi[tid];
     else
@@ -390,6 +393,8 @@ __global__ __launch_bounds__(64 * WAVES) void interaction_mfma_kernel(
   }
   if (tid < NPHI) {
     phi_map[tid * PS] = map_entry;
+    // complex64: twiddle tid's modulus correction in the second word of its phi map slot (packed tables only)
+    if constexpr (sizeof(T) == 4 && kPackTables) reinterpret_cast<T*>(phi_map)[tid * PS + 1] = eps_entry;
     if constexpr (kLateTwiddles) tw_phi[tid * TS] = tw_entry;
   }
   __syncthreads();
@@ -401,7 +406,8 @@ __global__ __launch_bounds__(64 * WAVES) void interaction_mfma_kernel(
   if constexpr (R1 > 0) {
     cx<T> ev[phi_exp_per_thread<NPHI, R1, C, THREADS>()][R2];
     const cx<T>* exp_rows = exp_table + n_tp * (rep + static_cast<long>(s.n_rep) * izr0);
-    phi_step_fused<T, NPHI, R1, R2, C, THREADS, Tile, TS>(dvr, rows, tw_phi, ev, [&] {
+    phi_step_fused<T, NPHI, R1, R2, C, THREADS, Tile, TS, PS>(dvr, rows, tw_phi, reinterpret_cast<const T*>(phi_map) + 1,
+                                                               ev, [&] {
       phi_exp_load<T, NPHI, R1, R2, C, THREADS, PS>(ev, rows, cw, phi_map, exp_rows, n_tp * s.n_rep);
     });
   } else {
@@ -518,6 +524,317 @@ __global__ __launch_bounds__(64 * WAVES) void interaction_mfma_kernel(
         }
       }
   }
+}
+
+// ---- buffer addressing ----------------------------------------------------------------------------------------
+// A raw buffer resource (CDNA dword 3 0x00020000: 32-bit elements, no swizzle) over `bytes` bytes. Loads and stores
+// take a 32-bit per-lane offset and a wavefront-uniform offset in an SGPR, so a per-lane offset computed once serves
+// every task of a loop while the uniform part moves on the scalar unit; loads at or past `bytes` return 0 and
+// stores there are dropped.
+using rsrc_t = __amdgpu_buffer_rsrc_t;
+
+__device__ __forceinline__ rsrc_t buffer_rsrc(const void* base, long bytes) {
+  const int n = static_cast<int>(bytes < 0x7fffffffL ? bytes : 0x7fffffffL);
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), 0, n, 0x00020000);
+}
+
+template <typename V>
+__device__ __forceinline__ V buffer_load(rsrc_t r, unsigned voff, unsigned soff) {
+  if constexpr (sizeof(V) == 4) {
+    return __builtin_bit_cast(V, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
+  } else if constexpr (sizeof(V) == 8) {
+    return __builtin_bit_cast(V, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+  } else {
+    static_assert(sizeof(V) == 16, "buffer_load: 4, 8 or 16 bytes");
+    return __builtin_bit_cast(V, __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, 0));
+  }
+}
+
+template <typename V>
+__device__ __forceinline__ void buffer_store(V v, rsrc_t r, unsigned voff, unsigned soff) {
+  if constexpr (sizeof(V) == 4) {
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, voff, soff, 0);
+  } else if constexpr (sizeof(V) == 8) {
+    typedef unsigned u2 __attribute__((ext_vector_type(2)));
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u2, v), r, voff, soff, 0);
+  } else {
+    static_assert(sizeof(V) == 16, "buffer_store: 4, 8 or 16 bytes");
+    typedef unsigned u4 __attribute__((ext_vector_type(4)));
+    __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u4, v), r, voff, soff, 0);
+  }
+}
+
+// The low part of the complex64 Legendre contraction. The FP32 fragments hold L rounded to float, L_hi; summing
+// products of the same rounded matrix forward (L_hi) and back (L_hi^T) makes L_hi^T L_hi differ from the identity
+// by ~1e-8, the same at every time step, which moves the norm of a rotational state linearly with the step count.
+// The remainder L_lo = L - L_hi (|L_lo| <= 2^-24 |L|) is added as one 16 x 16 x 16 bf16 product: L_lo rounded to
+// bf16 (host) times the same four B values of the lane truncated to bf16 (their top 16 bits), accumulated into the
+// FP32 result; its error, ~2^-33 |L| |B|, leaves L_hi + L_lo orthogonal to ~1e-10.
+// Operand layout of v_mfma_f32_16x16x16_bf16: lane l holds A[l % 16][4 (l / 16) + t] and B[4 (l / 16) + t][l % 16]
+// for t = 0 .. 3 (bf16 element t in bits 16 t .. 16 t + 15). The contraction index is relabelled as
+// kappa = 4 (l / 16) + t <-> row 4 t + l / 16 of the FP32 k steps, so B element t is the lane's value of k step t,
+// and the host arranges the fragments of L_lo (512 bytes per mJ block, lane order) the same way.
+__device__ __forceinline__ Mfma<float>::acc_t legendre_low_part(rsrc_t lo, int bb, const float (&b)[4],
+                                                               Mfma<float>::acc_t acc) {
+  typedef short s4 __attribute__((ext_vector_type(4)));
+  typedef unsigned u2 __attribute__((ext_vector_type(2)));
+  const u2 a = buffer_load<u2>(lo, (threadIdx.x % 64) * 8u, static_cast<unsigned>(bb) * 512u);
+  // the top halves of (b0, b1) and (b2, b3): element t = b[t] truncated to bf16
+  const u2 bb16 = {__builtin_amdgcn_perm(__float_as_uint(b[1]), __float_as_uint(b[0]), 0x07060302u),
+                   __builtin_amdgcn_perm(__float_as_uint(b[3]), __float_as_uint(b[2]), 0x07060302u)};
+  return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(__builtin_bit_cast(s4, a), __builtin_bit_cast(s4, bb16), acc, 0,
+                                                   0, 0);
+}
+
+// The split-BF16x3 variant of the complex64 Legendre contraction (SPO_HIP_CONTRACTION=bf16x3): L and the four B
+// values of the lane are each split into bf16 pieces, L = A_hi + A_lo (host) and b = b_hi + b_lo (here, both
+// rounded to nearest), and A_hi b_hi + A_hi b_lo + A_lo b_hi is accumulated in FP32 by three 16 x 16 x 16 bf16
+// products (smallest first; A_lo b_lo is dropped). The contraction index is relabelled as in legendre_low_part; the
+// host arranges the A_hi and A_lo fragments (512 bytes each per mJ block) the same way. Each piece carries 8
+// significant bits, so L takes part with about 16 bits: its relative error, ~2^-17, is ~100 times that of an FP32
+// L, and the same at every step.
+__device__ __forceinline__ Mfma<float>::acc_t legendre_bf16x3(rsrc_t hi, rsrc_t lo, int bb, const float (&b)[4]) {
+  typedef short s4 __attribute__((ext_vector_type(4)));
+  typedef unsigned u2 __attribute__((ext_vector_type(2)));
+  typedef __bf16 bf4 __attribute__((ext_vector_type(4)));
+  typedef float f4 __attribute__((ext_vector_type(4)));
+  const unsigned off = (threadIdx.x % 64) * 8u, base = static_cast<unsigned>(bb) * 512u;
+  const u2 a_hi = buffer_load<u2>(hi, off, base);
+  const u2 a_lo = buffer_load<u2>(lo, off, base);
+  const f4 x = {b[0], b[1], b[2], b[3]};
+  const bf4 x_hi = __builtin_convertvector(x, bf4);
+  const f4 rest = x - __builtin_convertvector(x_hi, f4);
+  const bf4 x_lo = __builtin_convertvector(rest, bf4);
+  Mfma<float>::acc_t acc = {};
+  acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(__builtin_bit_cast(s4, a_lo), __builtin_bit_cast(s4, x_hi), acc,
+                                                  0, 0, 0);
+  acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(__builtin_bit_cast(s4, a_hi), __builtin_bit_cast(s4, x_lo), acc,
+                                                  0, 0, 0);
+  return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(__builtin_bit_cast(s4, a_hi), __builtin_bit_cast(s4, x_hi), acc,
+                                                   0, 0, 0);
+}
+
+// An entry of the sign-split rotation table rot_pm (16 bytes): (Re w, -Im w) for the lane holding Re c and
+// (Re w, Im w) for the lane holding Im c, and in complex64 the modulus correction eps = |w_exact| / |w| - 1
+// (mul_factor, kernels_ct.hpp).
+template <typename T>
+struct alignas(16) RotEntry {
+  T x, y;
+};
+template <>
+struct alignas(16) RotEntry<float> {
+  float x, y, eps, pad;
+};
+
+// The interaction step of one (8 izr, site) tile with buffer addressing, for shapes where every mJ block has at
+// most 16 J, n_theta <= 16 and the blocks fit 8 tasks per wavefront. Same arithmetic as interaction_mfma_kernel's
+// fast path (same MFMA products in the same order, same phi step), organised so that the vector units do little
+// besides the phi FFTs:
+//   - the B operands of phase 1 and the A fragments of both contractions come through buffer loads whose per-lane
+//     offsets are computed once; the block's base moves in an SGPR;
+//   - phase 3 multiplies by rot(dt/2) in two operations per value: the lane holding Re c takes (Re w, -Im w) and
+//     the lane holding Im c takes (Re w, Im w) from a sign-split copy of the table (rot_pm, (irot, ir, lane parity)),
+//     so Re and Im of w c are each one product and one fused multiply-add with the neighbour's component;
+//   - results go straight to HBM through buffer stores at per-lane offsets computed once.
+// The DVR tile has padded rows (n_phi + 1) whose padding column holds the phi twiddles and the phi map, as the
+// complex64 tile of interaction_mfma_kernel; the tile has at least 2 n_phi rows.
+// BF16X3 (complex64 only): the theta contractions as split-BF16x3 products (legendre_bf16x3) instead of FP32 MFMA
+// with the bf16 low part; lo_fwd / lo_bwd then hold the A_lo fragments and hi_fwd / hi_bwd the A_hi ones.
+template <typename T, int NPHI, int WAVES, int R1, int R2, bool BF16X3 = false>
+__global__ __launch_bounds__(64 * WAVES, sizeof(T) == 8 ? 1 : 2) void interaction_buf_kernel(
+    cx<T>* psi, Shape s, const cx<T>* table_phi, AngularTables at, MfmaTables mt, const T* frag_fwd,
+    const T* frag_bwd, int frag_fwd_count, int frag_bwd_count, const cx<T>* exp_table, const cx<T>* rot_pm,
+    const unsigned* lo_fwd, const unsigned* lo_bwd, const T* eps_phi, const unsigned* hi_fwd,
+    const unsigned* hi_bwd) {
+  static_assert(!BF16X3 || sizeof(T) == 4, "split-BF16x3 is a complex64 variant");
+  using M = Mfma<T>;
+  using acc_t = typename M::acc_t;
+  using Tile = DvrTile<NPHI, false>;
+  constexpr int C = 8;
+  constexpr int THREADS = 64 * WAVES;
+  constexpr int TASKS = 8;  // mJ blocks per wavefront in phase 1 (all B operands loaded up front)
+  constexpr int LD = Tile::ld;
+  constexpr int TS = LD;
+  constexpr int PS = LD * static_cast<int>(sizeof(cx<T>) / sizeof(int));
+  constexpr unsigned CB = sizeof(cx<T>);
+  extern __shared__ unsigned char smem_raw[];
+  cx<T>* dvr = reinterpret_cast<cx<T>*>(smem_raw);
+  cx<T>* tw_phi = dvr + NPHI;                                          // row k, padding column
+  int* phi_map = reinterpret_cast<int*>(dvr + NPHI * LD + NPHI);       // row n_phi + k, padding column
+
+  const int n_theta = s.n_theta;
+  int tile, ixy;
+  block_tile_site(blockIdx.x, s.n_xy, mt, tile, ixy);
+  const int izr0 = tile * C;
+  if (izr0 >= s.n_zr_pot) return;  // the padding workgroup of an odd tile count (whole workgroup, no barrier yet)
+  const int cw = min(C, s.n_zr_pot - izr0);
+  const int tid = threadIdx.x;
+  const int lane = tid % 64;
+  const int wave = __builtin_amdgcn_readfirstlane(tid / 64);
+  const int n = lane % 16;   // B / D column: complex column n / 2, component n % 2
+  const int kk = lane / 16;  // B row within a k step
+  const int ri = n % 2;
+  const int col = n / 2;
+  const bool live = col < cw;
+  const int rows = C * n_theta;
+  const unsigned zr_bytes = static_cast<unsigned>(s.n_zr) * CB;
+
+  const rsrc_t site = buffer_rsrc(psi + static_cast<long>(s.n_zr) * s.n_rot * ixy, static_cast<long>(s.n_zr) * s.n_rot * CB);
+  const rsrc_t ffwd = buffer_rsrc(frag_fwd, 64L * frag_fwd_count * sizeof(T));
+  const rsrc_t fbwd = buffer_rsrc(frag_bwd, 64L * frag_bwd_count * sizeof(T));
+  // rot_pm entries are 16 bytes: (Re w, +-Im w) in FP64, (Re w, +-Im w, eps, 0) in complex64 (RotEntry)
+  const rsrc_t rots = buffer_rsrc(rot_pm, 2L * s.n_rot * s.nr * 16);
+  const rsrc_t lo_f = buffer_rsrc(lo_fwd, sizeof(T) == 4 ? 512L * s.n_blocks : 0);
+  const rsrc_t lo_b = buffer_rsrc(lo_bwd, sizeof(T) == 4 ? 512L * s.n_blocks : 0);
+  const rsrc_t hi_f = buffer_rsrc(hi_fwd, BF16X3 ? 512L * s.n_blocks : 0);
+  const rsrc_t hi_b = buffer_rsrc(hi_bwd, BF16X3 ? 512L * s.n_blocks : 0);
+
+  // The phi map (and for complex64 the twiddles) go to LDS after phase 1, so that their loads do not hold up its
+  // first loads; FP64 writes its twiddles at once (holding them through phase 1 would spill).
+  constexpr bool kLateTwiddles = sizeof(T) == 4;
+  const int op = at.op_of[ixy];
+  int map_entry = 0;
+  T eps_entry = T(0);
+  cx<T> tw_entry = {};
+  if (tid < NPHI) {
+    map_entry = at.phi_index[tid + NPHI * op];
+    if constexpr (sizeof(T) == 4) eps_entry = eps_phi[tid];
+    if constexpr (kLateTwiddles)
+      tw_entry = table_phi[tid];
+    else
+      tw_phi[tid * TS] = table_phi[tid];
+  }
+  const int n_unused = NPHI - s.n_blocks;
+  for (int e = tid; e < rows * n_unused; e += THREADS) dvr[Tile::at(e / n_unused, s.mj_max + 1 + e % n_unused)] = {};
+
+  // 1. FBR -> DVR (theta), task = mJ block bb = wave + i WAVES: B[k][n] = psi(J = f0 + 4 k + kk, izr0 + col).ri.
+  // Rows past the block read the next block's values (their fragment entries are zero) or 0 past the column.
+  const unsigned b_lane = static_cast<unsigned>(kk) * zr_bytes + static_cast<unsigned>(n) * sizeof(T);
+  T bv[TASKS][4];
+#pragma unroll
+  for (int i = 0; i < TASKS; ++i) {
+    const int bb = wave + i * WAVES;
+    if (bb < s.n_blocks) {
+      const unsigned base = (static_cast<unsigned>(at.block_first[bb]) * s.n_zr + izr0) * CB;
+#pragma unroll
+      for (int k = 0; k < 4; ++k) bv[i][k] = buffer_load<T>(site, b_lane, base + 4u * k * zr_bytes);
+    }
+  }
+  // This lane's DVR tile position of row theta = M::row(lane, 0), column col, component ri (theta rows of the
+  // other v follow at compile-time distances).
+  const unsigned d_lane = (static_cast<unsigned>(M::row(lane, 0) * C + col) * LD) * CB + ri * sizeof(T);
+  constexpr unsigned d_v = (sizeof(T) == 8 ? 4u : 1u) * C * LD * CB;  // tile bytes between v and v + 1
+  unsigned char* dvr_b = smem_raw;
+#pragma unroll
+  for (int i = 0; i < TASKS; ++i) {
+    const int bb = wave + i * WAVES;
+    if (bb >= s.n_blocks) break;
+    const int k_steps = (at.block_size[bb] + 3) / 4;
+    const unsigned fbase = static_cast<unsigned>(mt.fwd_offset[bb]) * 64u * sizeof(T);
+    const int m = bb - s.mj_max;
+    const int slot = m >= 0 ? m : NPHI + m;
+    acc_t acc = {};
+    if constexpr (BF16X3) {
+      acc = legendre_bf16x3(hi_f, lo_f, bb, bv[i]);
+    } else {
+#pragma unroll
+      for (int k = 0; k < 4; ++k)
+        if (k < k_steps)
+          acc = M::fma(buffer_load<T>(ffwd, lane * sizeof(T), fbase + k * 64u * sizeof(T)), bv[i][k], acc);
+      if constexpr (sizeof(T) == 4) acc = legendre_low_part(lo_f, bb, bv[i], acc);
+    }
+    unsigned char* d = dvr_b + d_lane + slot * CB;
+#pragma unroll
+    for (int v = 0; v < 4; ++v)
+      if (M::row(lane, v) < n_theta) *reinterpret_cast<T*>(d + v * d_v) = acc[v];
+  }
+  if (tid < NPHI) {
+    phi_map[tid * PS] = map_entry;
+    if constexpr (sizeof(T) == 4) reinterpret_cast<T*>(phi_map)[tid * PS + 1] = eps_entry;
+    if constexpr (kLateTwiddles) tw_phi[tid * TS] = tw_entry;
+  }
+  __syncthreads();
+
+  // 2. phi FFTs, the potential phase, phi FFTs back.
+  {
+    const int rep = at.rep_of[ixy];
+    const int n_tp = n_theta * NPHI;
+    cx<T> ev[phi_exp_per_thread<NPHI, R1, C, THREADS>()][R2];
+    const cx<T>* exp_rows = exp_table + n_tp * (rep + static_cast<long>(s.n_rep) * izr0);
+    phi_step_fused<T, NPHI, R1, R2, C, THREADS, Tile, TS, PS>(dvr, rows, tw_phi, reinterpret_cast<const T*>(phi_map) + 1,
+                                                               ev, [&] {
+      phi_exp_load<T, NPHI, R1, R2, C, THREADS, PS>(ev, rows, cw, phi_map, exp_rows, n_tp * s.n_rep);
+    });
+  }
+
+  // 3. DVR -> FBR (theta), x rot(dt/2), to HBM. B[k][n] = DVR(theta = 4 k + kk, col).ri at the block's phi slot;
+  // rows theta >= n_theta read the last theta row instead (written in phase 2, so finite; their fragment entries
+  // are zero). The tile holds no row past it: a read there would meet whatever the LDS holds.
+  const int k_bwd = (n_theta + 3) / 4;
+  unsigned t_lane[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const int th = 4 * k + kk < n_theta ? 4 * k + kk : n_theta - 1;
+    t_lane[k] = (static_cast<unsigned>(th * C + col) * LD) * CB + ri * sizeof(T);
+  }
+  // Per v: the store offset of J = M::row(lane, v) (past the tile's columns: beyond every block, dropped by the
+  // row test below) and the rotation factor's offset in rot_pm.
+  unsigned st_lane[4], rot_lane[4];
+  const int ir0 = izr0 % s.nr;
+#pragma unroll
+  for (int v = 0; v < 4; ++v) {
+    const int j = M::row(lane, v);
+    int ir = ir0 + col;
+    if (ir >= s.nr) ir -= s.nr;
+    st_lane[v] = (static_cast<unsigned>(j) * s.n_zr + col) * CB + ri * sizeof(T);
+    rot_lane[v] = ((static_cast<unsigned>(j) * s.nr + ir) * 2u + ri) * 16u;
+  }
+  for (int bb = wave; bb < s.n_blocks; bb += WAVES) {
+    const int f0 = at.block_first[bb], nb = at.block_size[bb];
+    const int m = bb - s.mj_max;
+    const int slot = m >= 0 ? m : NPHI + m;
+    const unsigned fbase = static_cast<unsigned>(mt.bwd_offset[bb]) * 64u * sizeof(T);
+    T a[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) a[k] = k < k_bwd ? buffer_load<T>(fbwd, lane * sizeof(T), fbase + k * 64u * sizeof(T)) : T(0);
+    RotEntry<T> w[4];
+    const unsigned rbase = static_cast<unsigned>(f0) * s.nr * 2u * 16u;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) w[v] = buffer_load<RotEntry<T>>(rots, rot_lane[v], rbase);
+    T b[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) b[k] = k < k_bwd ? *reinterpret_cast<const T*>(dvr_b + t_lane[k] + slot * CB) : T(0);
+    acc_t acc = {};
+    if constexpr (BF16X3) {
+      acc = legendre_bf16x3(hi_b, lo_b, bb, b);
+    } else {
+#pragma unroll
+      for (int k = 0; k < 4; ++k)
+        if (k < k_bwd) acc = M::fma(a[k], b[k], acc);
+      if constexpr (sizeof(T) == 4) acc = legendre_low_part(lo_b, bb, b, acc);
+    }
+    const unsigned sbase = (static_cast<unsigned>(f0) * s.n_zr + izr0) * CB;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const T mine = acc[v];
+      const T other = swap_adjacent_lanes(mine);
+      T out = w[v].x * mine + w[v].y * other;
+      if constexpr (sizeof(T) == 4) out = fmaf(out, w[v].eps, out);  // the factor's modulus correction
+      if (live && M::row(lane, v) < nb) buffer_store<T>(out, site, st_lane[v], sbase);
+    }
+  }
+}
+
+// Host-side twin of the rot_pm layout: entry ((irot * nr + ir) * 2 + parity) = (Re w, -Im w) for parity 0 and
+// (Re w, Im w) for parity 1, w = rot(dt/2)[irot, ir].
+inline std::size_t rot_pm_index(int irot, int ir, int nr, int parity) {
+  return (static_cast<std::size_t>(irot) * nr + ir) * 2 + parity;
+}
+
+// LDS bytes of interaction_buf_kernel: the padded tile of max(8 n_theta, 2 n_phi) rows.
+template <typename T>
+inline std::size_t interaction_buf_lds(int n_theta, int nphi) {
+  return std::size_t(std::max(8 * n_theta, 2 * nphi)) * (nphi + 1) * sizeof(cx<T>);
 }
 
 // The initial channel's specular coupling term of the interaction region, added to psi before the MFMA interaction

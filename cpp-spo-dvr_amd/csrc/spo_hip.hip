@@ -10,6 +10,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdint>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -97,13 +98,54 @@ Buffer upload(const void* host, std::size_t bytes) {
   return b;
 }
 
+// A complex factor in the storage type: FP64 as is; complex64 rounded so that its modulus is kept (the factors
+// multiply the wave at every step, and plain rounding would change their modulus by the same ~6e-8 every time).
+template <typename T>
+cx<T> to_storage(double x, double y) {
+  if constexpr (sizeof(T) == 8) {
+    return {x, y};
+  } else {
+    cx<float> r;
+    spo_dev::round_keep_modulus(x, y, r.x, r.y);
+    return r;
+  }
+}
+
 // Complex array (interleaved doubles) -> device array of cx<T>, optionally times a real factor (in FP64 first).
+// Large complex64 tables (exp(i A)) are rounded on the device.
 template <typename T>
 Buffer upload_complex(const double* p, std::size_t n, double factor = 1.0) {
+  if (sizeof(T) == 4 && n > (std::size_t(1) << 18)) {
+    Buffer full = upload(p, n * sizeof(cx<double>));
+    Buffer out(n * sizeof(cx<T>));
+    hipLaunchKernelGGL(spo_dev::round_keep_modulus_kernel, dim3(static_cast<unsigned>((n + 255) / 256)), dim3(256), 0,
+                       nullptr, full.as<const double>(), out.as<float>(), static_cast<long>(n), factor);
+    HIP_CHECK(hipGetLastError());
+    HIP_CHECK(hipStreamSynchronize(nullptr));
+    return out;
+  }
   std::vector<cx<T>> v(n);
-  for (std::size_t i = 0; i < n; ++i) v[i] = {T(p[2 * i] * factor), T(p[2 * i + 1] * factor)};
+  for (std::size_t i = 0; i < n; ++i) v[i] = to_storage<T>(p[2 * i] * factor, p[2 * i + 1] * factor);
   return upload(v.data(), n * sizeof(cx<T>));
 }
+// complex64: per entry of a factor table uploaded by upload_complex, |z| / |stored z| - 1 in FP32 (z the FP64 value
+// times factor); the kernels multiply by the stored value and add eps times the product (mul_factor). Empty in FP64.
+template <typename T>
+Buffer factor_eps(const double* p, std::size_t n, double factor = 1.0) {
+  if constexpr (sizeof(T) == 8) {
+    return Buffer();
+  } else {
+    std::vector<float> e(n);
+    for (std::size_t i = 0; i < n; ++i) {
+      const double x = p[2 * i] * factor, y = p[2 * i + 1] * factor;
+      const cx<float> w = to_storage<float>(x, y);
+      const double m2 = static_cast<double>(w.x) * w.x + static_cast<double>(w.y) * w.y;
+      e[i] = m2 > 0.0 ? static_cast<float>(std::sqrt((x * x + y * y) / m2) - 1.0) : 0.0f;
+    }
+    return upload(e.data(), n * sizeof(float));
+  }
+}
+
 template <typename T>
 Buffer upload_real(const double* p, std::size_t n) {
   std::vector<T> v(p, p + n);
@@ -127,22 +169,29 @@ std::vector<int> radices_of(int n) {
 // An FFT length: twiddle table W[m] = exp(-2 pi i m / n) on the device (in T) and the run-time plan.
 template <typename T>
 struct DevicePlan {
-  Buffer table;
+  Buffer table, eps;
   spo_dev::FftPlan plan{};
   explicit DevicePlan(int n) {
     const std::vector<int> r = radices_of(n);
     std::vector<cx<T>> w(n);
+    std::vector<T> e(n, T(0));
     for (int m = 0; m < n; ++m) {
       const long double a = -2.0L * 3.14159265358979323846264338327950288L * m / n;
-      w[m] = {static_cast<T>(std::cos(a)), static_cast<T>(std::sin(a))};
+      w[m] = to_storage<T>(static_cast<double>(std::cos(a)), static_cast<double>(std::sin(a)));
+      // complex64: the correction that makes the product's modulus exact (twiddle(), kernels_ct.hpp)
+      const double m2 = static_cast<double>(w[m].x) * w[m].x + static_cast<double>(w[m].y) * w[m].y;
+      e[m] = static_cast<T>(1.0 / std::sqrt(m2) - 1.0);
     }
     table = upload(w.data(), w.size() * sizeof(cx<T>));
+    eps = upload(e.data(), e.size() * sizeof(T));
     plan.n = n;
     plan.n_stages = static_cast<int>(r.size());
     for (std::size_t i = 0; i < r.size(); ++i) plan.radix[i] = r[i];
     plan.table = table.as<void>();
   }
   const cx<T>* tw() const { return table.as<cx<T>>(); }
+  // 1 / |w_m| - 1 of the stored twiddles (complex64 kernels only)
+  const T* tw_eps() const { return eps.as<T>(); }
 };
 
 unsigned blocks_for(long n, int threads) { return static_cast<unsigned>((n + threads - 1) / threads); }
@@ -158,10 +207,14 @@ void set_lds(const void* kernel, std::size_t bytes) {
 #define SPO_PHI_LENGTHS(X) X(48) X(22) X(10) X(12) X(8) X(56)
 // T_Zr planes with register-resident passes (NZ, NR, A1, A2, B1, B2, threads): NR = A1 A2, NZ = B1 B2. Entries of
 // one shape that differ only in the thread count compute identically; the launch tuner picks among them.
+// Other factorisations of a shape (the radices of each axis) are further entries of that shape: they compute the same
+// transforms with their roundings in another order, and the launch tuner picks among them as among block sizes.
 #define SPO_ZR_REG_SHAPES(X)                                                                                 \
-  X(96, 48, 3, 16, 8, 12, 192) X(96, 48, 3, 16, 8, 12, 384) X(96, 48, 3, 16, 8, 12, 512) X(128, 32, 4, 8, 16, 8, 256)                      \
-  X(128, 32, 4, 8, 16, 8, 512) X(64, 16, 4, 4, 8, 8, 128) X(80, 16, 4, 4, 8, 10, 128)                         \
-  X(176, 56, 7, 8, 11, 16, 256) X(176, 56, 7, 8, 11, 16, 512)
+  X(96, 48, 3, 16, 8, 12, 192) X(96, 48, 3, 16, 8, 12, 384) X(96, 48, 3, 16, 8, 12, 512)                      \
+  X(96, 48, 3, 16, 12, 8, 384) X(96, 48, 12, 4, 8, 12, 384)                                                    \
+  X(128, 32, 4, 8, 16, 8, 256) X(128, 32, 4, 8, 16, 8, 512) X(64, 16, 4, 4, 8, 8, 128) X(80, 16, 4, 4, 8, 10, 128) \
+  X(176, 56, 7, 8, 11, 16, 256) X(176, 56, 7, 8, 11, 16, 512) X(96, 32, 4, 8, 8, 12, 256)                      \
+  X(96, 32, 4, 8, 8, 12, 512)
 // Interaction with the theta contractions on MFMA (phi length, wavefronts, phi radices R1 x R2 of the fused phi
 // step); the staged-FFT variant is instantiated with 8 wavefronts.
 #define SPO_MFMA_SHAPES(X)                                                                                   \
@@ -169,6 +222,10 @@ void set_lds(const void* kernel, std::size_t bytes) {
   X(8, 8, 4, 2) X(56, 8, 8, 7) X(56, 4, 8, 7)
 // T_xy tiles with whole-line DFTs in registers (NX = NY a single radix; C izr per tile, C * NX threads).
 #define SPO_XY_REG_SHAPES(X) X(16, 16, 32) X(16, 16, 16) X(16, 16, 8) X(8, 8, 32) X(8, 8, 16) X(4, 4, 64) X(4, 4, 32)
+// The buffer-addressed interaction kernel (phi length, wavefronts, phi radices): the MFMA shapes, the phi steps of
+// the reduced-JMax batch members (28 and 40 points) and two more factorisations of the 48-point step (16 x 3,
+// 8 x 6) for the launch tuner.
+#define SPO_BUF_SHAPES(X) SPO_MFMA_SHAPES(X) X(28, 8, 7, 4) X(40, 8, 10, 4) X(48, 8, 16, 3) X(48, 8, 8, 6)
 
 // A launch variant of a kernel family: the kernel, its block size, its tile parameter (T_xy: izr per tile;
 // interaction: wavefronts) and its LDS bytes.
@@ -236,6 +293,28 @@ std::vector<Variant> interaction_mfma_variants(int nphi, int n_theta, bool fused
   return out;
 }
 
+// The buffer-addressed interaction kernel: every MFMA shape with at most 8 mJ blocks per wavefront; bf16x3: the
+// complex64 split-BF16x3 contraction variant (instantiated for the 48-point phi step).
+template <typename T>
+std::vector<Variant> interaction_buf_variants(int nphi, int n_theta, int n_blocks, bool bf16x3) {
+  std::vector<Variant> out;
+  const std::size_t lds = spo_dev::interaction_buf_lds<T>(n_theta, nphi);
+  if (bf16x3) {
+    if constexpr (sizeof(T) == 4) {
+      if (nphi == 48 && n_blocks <= 64)
+        out.push_back({reinterpret_cast<const void*>(spo_dev::interaction_buf_kernel<T, 48, 8, 12, 4, true>), 512,
+                       8, lds});
+    }
+    return out;
+  }
+#define X(A, W, R1, R2)                                                                                         \
+  if (nphi == A && n_blocks <= 8 * W)                                                                           \
+    out.push_back({reinterpret_cast<const void*>(spo_dev::interaction_buf_kernel<T, A, W, R1, R2>), 64 * W, W, lds});
+  SPO_BUF_SHAPES(X)
+#undef X
+  return out;
+}
+
 template <typename T>
 const void* interaction_ct_symbol(int nphi) {
 #define X(A) \
@@ -266,6 +345,7 @@ struct spo_hip_backend {
   // events) at its first use, then replayed. The step's launches and arguments do not change between steps.
   bool use_graph = false;
   hipGraphExec_t step_graph = nullptr;
+  int batches = 0;  // spo_hip_batch objects this backend is a member of (it may not be destroyed before them)
   int max_lds = 65536;
   spo_hip_analysis analysis{};
 
@@ -424,10 +504,14 @@ namespace {
 template <typename T>
 struct State final : spo_hip_backend {
   Buffer psi, zr_full, zr_half, xy_full, xy_half, rot_half, rot_half_t, before, gas_step, z_opt;
+  // complex64: the modulus corrections of the factor tables (factor_eps); empty in FP64
+  Buffer zr_full_eps, zr_half_eps, xy_full_eps, xy_half_eps, before_eps, gas_step_eps, shift_f_eps, shift_b_eps;
   Buffer shift_f, shift_b, s2sc, legendre, block_first, block_size, block_offset, rot_block, rep_of, op_of, phi_index;
   Buffer exp_table, legendre_t;
-  Buffer frag_fwd, frag_bwd, fwd_offset, bwd_offset, site_order;
+  Buffer frag_fwd, frag_bwd, fwd_offset, bwd_offset, site_order, rot_pm, lo_fwd, lo_bwd, hi_fwd, hi_bwd;
   int max_jt = 1;
+  int frag_fwd_count = 0, frag_bwd_count = 0;
+  bool v_buf = false;  // v_variants are interaction_buf_kernel instantiations
   int legendre_size = 0;
   std::size_t zr_lds = 0, xy_lds = 0, v_lds = 0;
   std::unique_ptr<DevicePlan<T>> plan_z, plan_r, plan_x, plan_y, plan_phi;
@@ -473,7 +557,11 @@ void State<T>::run_zr(const Variant& v, bool half) {
   const cx<T>* tz = plan_z->tw();
   const cx<T>* tr = plan_r->tw();
   const cx<T>* zr = (half ? zr_half : zr_full).as<cx<T>>();
-  void* args[] = {&ps, const_cast<cx<T>**>(&tz), const_cast<cx<T>**>(&tr), const_cast<cx<T>**>(&zr)};
+  const T* ez = plan_z->tw_eps();
+  const T* er = plan_r->tw_eps();
+  const T* fe = (half ? zr_half_eps : zr_full_eps).as<T>();
+  void* args[] = {&ps, const_cast<cx<T>**>(&tz), const_cast<cx<T>**>(&tr), const_cast<cx<T>**>(&zr),
+                  const_cast<T**>(&ez), const_cast<T**>(&er), const_cast<T**>(&fe)};
   launch(v.fn, s.n_rot * s.n_xy, v.threads, v.lds, stream, args);
 }
 
@@ -489,9 +577,11 @@ void State<T>::run_xy(const Variant& v, bool half, bool kinetic_only) {
   const T* zo = z_opt.as<T>();
   const cx<T>* bf = before.as<cx<T>>();
   const cx<T>* gs = gas_step.as<cx<T>>();
+  spo_dev::XyEps<T> eps{(half ? xy_half_eps : xy_full_eps).as<T>(), shift_f_eps.as<T>(), shift_b_eps.as<T>(),
+                         before_eps.as<T>(), gas_step_eps.as<T>()};
   void* args[] = {&ps, &s, const_cast<cx<T>**>(&factor), const_cast<cx<T>**>(&sf), const_cast<cx<T>**>(&sb),
                   &needs_shift, &apply_potential, const_cast<T**>(&zo), const_cast<cx<T>**>(&bf),
-                  const_cast<cx<T>**>(&gs)};
+                  const_cast<cx<T>**>(&gs), &eps};
   launch(v.fn, tiles * s.n_rot, v.threads, v.lds, stream, args);
 }
 
@@ -508,14 +598,29 @@ void State<T>::run_interaction_mfma(const Variant& v) {
   const cx<T>* rh = rot_half_t.as<cx<T>>();
   const cx<T>* ssc = s2sc.as<cx<T>>();
   const cx<T>* tp = plan_phi->tw();
+  const T* ep = plan_phi->tw_eps();
   spo_dev::MfmaTables mt{fwd_offset.as<int>(), bwd_offset.as<int>(), site_order.as<int>(), pair};
   const T* ff = frag_fwd.as<T>();
   const T* fb = frag_bwd.as<T>();
   int mj = max_jt;
+  const int grid = pair ? 2 * ((tiles + 1) / 2) * s.n_xy : tiles * s.n_xy;
+  if (v_buf) {
+    const cx<T>* rp = rot_pm.as<cx<T>>();
+    const unsigned* lf = lo_fwd.as<unsigned>();
+    const unsigned* lb = lo_bwd.as<unsigned>();
+    const unsigned* hf = hi_fwd.as<unsigned>();
+    const unsigned* hb = hi_bwd.as<unsigned>();
+    int nf = frag_fwd_count, nbw = frag_bwd_count;
+    void* args[] = {&ps, &s, const_cast<cx<T>**>(&tp), &at, &mt, const_cast<T**>(&ff), const_cast<T**>(&fb),
+                    &nf, &nbw, const_cast<cx<T>**>(&et), const_cast<cx<T>**>(&rp), const_cast<unsigned**>(&lf),
+                    const_cast<unsigned**>(&lb), const_cast<T**>(&ep), const_cast<unsigned**>(&hf),
+                    const_cast<unsigned**>(&hb)};
+    launch(v.fn, grid, v.threads, v.lds, stream, args);
+    return;
+  }
   void* args[] = {&ps, const_cast<cx<double>**>(&sp), &s, const_cast<cx<T>**>(&tp), &at, &mt,
                   const_cast<T**>(&ff), const_cast<T**>(&fb), &mj, const_cast<cx<T>**>(&et),
-                  const_cast<cx<T>**>(&rh), const_cast<cx<T>**>(&ssc)};
-  const int grid = pair ? 2 * ((tiles + 1) / 2) * s.n_xy : tiles * s.n_xy;
+                  const_cast<cx<T>**>(&rh), const_cast<cx<T>**>(&ssc), const_cast<T**>(&ep)};
   launch(v.fn, grid, v.threads, v.lds, stream, args);
 }
 
@@ -815,10 +920,10 @@ std::unique_ptr<spo_hip_backend> create_state(const spo_hip_shape* shape, const 
   HIP_CHECK(hipEventCreateWithFlags(&b->spec_ready, hipEventDisableTiming));
 
   const std::size_t cT = sizeof(cx<T>), c16 = sizeof(cx<double>);
-  // Zero slack behind the last site: the interaction kernel's phase-1 loads of a partial k step or of a column past
-  // the interaction region read a few values past a site column without a branch (their products vanish: the
-  // matching Legendre fragments are zero, or the column is never stored), at most 3 n_zr + 6 values past the end.
-  const std::size_t psi_slack = 4 * std::size_t(s.n_zr) + 64;
+  // Zero slack behind the last site: the interaction kernels' phase-1 loads of a partial k step or of a column past
+  // the interaction region read values past a site column without a branch (their products vanish: the matching
+  // Legendre fragments are zero, or the column is never stored), at most 15 n_zr + 7 values past the end.
+  const std::size_t psi_slack = 16 * std::size_t(s.n_zr) + 64;
   b->psi = Buffer((psi_values + psi_slack) * cT);
   HIP_CHECK(hipMemsetAsync(b->psi.template as<void>(), 0, (psi_values + psi_slack) * cT, b->stream));
   b->spec = upload(initial_specular, std::size_t(s.nr) * s.nzsp * c16);
@@ -842,14 +947,18 @@ std::unique_ptr<spo_hip_backend> create_state(const spo_hip_shape* shape, const 
   {
     const auto zf = fold(t->zr_full, s.n_zr, zr_f, zr_b), zh = fold(t->zr_half, s.n_zr, zr_f, zr_b);
     b->zr_full = upload_complex<T>(zf.data(), s.n_zr);
+    b->zr_full_eps = factor_eps<T>(zf.data(), s.n_zr);
     b->zr_half = upload_complex<T>(zh.data(), s.n_zr);
+    b->zr_half_eps = factor_eps<T>(zh.data(), s.n_zr);
     const auto sf = fold(t->specular_full, std::size_t(s.nr) * s.nzsp, zr_f, sp_b);
     const auto sh = fold(t->specular_half, std::size_t(s.nr) * s.nzsp, zr_f, sp_b);
     b->spec_full = upload(sf.data(), sf.size() * sizeof(double));
     b->spec_half = upload(sh.data(), sh.size() * sizeof(double));
     const auto xf = fold(t->xy_full, s.n_xy, xy_f, xy_b), xh = fold(t->xy_half, s.n_xy, xy_f, xy_b);
     b->xy_full = upload_complex<T>(xf.data(), s.n_xy);
+    b->xy_full_eps = factor_eps<T>(xf.data(), s.n_xy);
     b->xy_half = upload_complex<T>(xh.data(), s.n_xy);
+    b->xy_half_eps = factor_eps<T>(xh.data(), s.n_xy);
   }
   b->rot_half = upload_complex<T>(t->rotation_half, std::size_t(s.n_rot) * s.nr);
   {
@@ -863,12 +972,16 @@ std::unique_ptr<spo_hip_backend> create_state(const spo_hip_shape* shape, const 
   }
   b->before = upload_complex<T>(t->before_interaction, std::size_t(s.n_rot) * s.nr);
   b->gas_step = upload_complex<T>(t->gas_phase_step, std::size_t(s.n_rot) * s.nr);
+  b->before_eps = factor_eps<T>(t->before_interaction, std::size_t(s.n_rot) * s.nr);
+  b->gas_step_eps = factor_eps<T>(t->gas_phase_step, std::size_t(s.n_rot) * s.nr);
   b->spec_gas = upload(t->specular_gas_phase, std::size_t(s.nr) * c16);
   b->z_opt = upload_real<T>(t->z_optical, s.nz);
   b->r_opt_d = upload(t->r_optical, std::size_t(s.nr) * sizeof(double));
   b->z_spec_opt = upload(t->z_specular_optical, std::size_t(s.nzsp) * sizeof(double));
   b->shift_f = upload_complex<T>(t->shift_forward, s.n_xy);
   b->shift_b = upload_complex<T>(t->shift_backward, s.n_xy);
+  b->shift_f_eps = factor_eps<T>(t->shift_forward, s.n_xy);
+  b->shift_b_eps = factor_eps<T>(t->shift_backward, s.n_xy);
   b->shift_f_d = upload(t->shift_forward, std::size_t(s.n_xy) * c16);
   b->s2sc = upload_complex<T>(t->specular_to_scattering, s.n_xy);
   b->s2sc_d = upload(t->specular_to_scattering, std::size_t(s.n_xy) * c16);
@@ -973,14 +1086,21 @@ std::unique_ptr<spo_hip_backend> create_state(const spo_hip_shape* shape, const 
   if (b->v_lds > lds_limit) throw Status(SPO_HIP_E_NOT_SUPPORTED, "an angular column does not fit in LDS");
   set_lds(b->v_fn ? b->v_fn : reinterpret_cast<const void*>(spo_dev::interaction_kernel<T>), b->v_lds);
   // Interaction with the theta contractions on MFMA: Legendre fragments in lane order, XCD-grouped site order.
-  // SPO_HIP_INTERACTION selects a variant for comparisons: "mfma" (default: MFMA contractions, fused register phi
-  // step), "mfma_staged" (MFMA contractions, staged LDS phi FFTs), "lds" (contractions on the vector units).
+  // SPO_HIP_INTERACTION selects a variant for comparisons: "buf" (default: the buffer-addressed MFMA kernel where
+  // its shape conditions hold, else "mfma"), "mfma" (MFMA contractions, fused register phi step), "mfma_staged"
+  // (MFMA contractions, staged LDS phi FFTs), "lds" (contractions on the vector units).
   {
     const char* env = std::getenv("SPO_HIP_INTERACTION");
-    const std::string choice = env ? env : "mfma";
-    if (choice != "mfma" && choice != "mfma_staged" && choice != "lds")
-      throw Status(SPO_HIP_E_ARGUMENT, "SPO_HIP_INTERACTION must be mfma, mfma_staged or lds");
-    const bool fused = choice == "mfma" && s.n_theta <= 16;
+    const std::string choice = env ? env : "buf";
+    if (choice != "buf" && choice != "mfma" && choice != "mfma_staged" && choice != "lds")
+      throw Status(SPO_HIP_E_ARGUMENT, "SPO_HIP_INTERACTION must be buf, mfma, mfma_staged or lds");
+    const bool fused = (choice == "mfma" || choice == "buf") && s.n_theta <= 16;
+    // SPO_HIP_CONTRACTION=bf16x3: the complex64 theta contractions as split-BF16x3 products (buffer kernel, 48-point
+    // phi step); the default "fp32" is FP32 MFMA with the bf16 low part of L. FP64 ignores it.
+    const char* contraction = std::getenv("SPO_HIP_CONTRACTION");
+    if (contraction && std::string(contraction) != "fp32" && std::string(contraction) != "bf16x3")
+      throw Status(SPO_HIP_E_ARGUMENT, "SPO_HIP_CONTRACTION must be fp32 or bf16x3");
+    const bool bf16x3 = sizeof(T) == 4 && contraction && std::string(contraction) == "bf16x3";
     std::vector<Variant> variants;
     for (const Variant& v : interaction_mfma_variants<T>(s.n_phi, s.n_theta, fused))
       if (v.lds <= lds_limit) variants.push_back(v);
@@ -1031,6 +1151,83 @@ std::unique_ptr<spo_hip_backend> create_state(const spo_hip_shape* shape, const 
       b->bwd_offset = upload(bo.data(), bo.size() * sizeof(int));
       b->site_order = upload(order.data(), order.size() * sizeof(int));
       b->max_jt = (max_nb + 15) / 16;
+      b->frag_fwd_count = nf;
+      b->frag_bwd_count = nbw;
+      // The buffer-addressed kernel where its shape conditions hold: one theta tile, blocks of at most 16 J, a
+      // site column addressable with 32-bit offsets.
+      if (choice == "buf" && s.n_theta <= 16 && max_nb <= 16 &&
+          std::size_t(s.n_zr) * s.n_rot * cT < (std::size_t(1) << 31)) {
+        std::vector<Variant> buf;
+        for (const Variant& v : interaction_buf_variants<T>(s.n_phi, s.n_theta, s.n_blocks, bf16x3))
+          if (v.lds <= lds_limit) buf.push_back(v);
+        if (bf16x3 && buf.empty())
+          throw Status(SPO_HIP_E_NOT_SUPPORTED, "SPO_HIP_CONTRACTION=bf16x3: instantiated for the 48-point phi step only");
+        if (!buf.empty()) {
+          // Sign-split rotation factors, 16 zero rows past the last irot (the loads of a block's rows past the
+          // basis stay inside the table).
+          std::vector<spo_dev::RotEntry<T>> pm(2 * (std::size_t(s.n_rot) + 16) * s.nr, spo_dev::RotEntry<T>{});
+          for (int irot = 0; irot < s.n_rot; ++irot)
+            for (int ir = 0; ir < s.nr; ++ir) {
+              const double x = t->rotation_half[2 * (irot + std::size_t(s.n_rot) * ir)];
+              const double y = t->rotation_half[2 * (irot + std::size_t(s.n_rot) * ir) + 1];
+              const cx<T> w = to_storage<T>(x, y);
+              const std::size_t e0 = spo_dev::rot_pm_index(irot, ir, s.nr, 0);
+              pm[e0].x = w.x;
+              pm[e0].y = -w.y;
+              pm[e0 + 1].x = w.x;
+              pm[e0 + 1].y = w.y;
+              if constexpr (sizeof(T) == 4) {
+                const double m2 = static_cast<double>(w.x) * w.x + static_cast<double>(w.y) * w.y;
+                const float e = static_cast<float>(std::sqrt((x * x + y * y) / m2) - 1.0);
+                pm[e0].eps = e;
+                pm[e0 + 1].eps = e;
+              }
+            }
+          b->rot_pm = upload(pm.data(), pm.size() * sizeof(spo_dev::RotEntry<T>));
+          if (sizeof(T) == 4) {
+            // bf16 fragments of a function of L, lane (i, q) element t = contraction row 4 t + q
+            // (legendre_low_part): forward A = L (theta = i, J = row), backward A = L^T (J = i, theta = row).
+            auto bf16 = [](double x) -> std::uint16_t {
+              const float f = static_cast<float>(x);
+              std::uint32_t u;
+              std::memcpy(&u, &f, 4);
+              u += 0x7fffu + ((u >> 16) & 1u);  // round to nearest even (the values are finite and small)
+              return static_cast<std::uint16_t>(u >> 16);
+            };
+            auto bf16_value = [&](double x) {
+              const std::uint32_t u = std::uint32_t(bf16(x)) << 16;
+              float f;
+              std::memcpy(&f, &u, 4);
+              return static_cast<double>(f);
+            };
+            auto fragments = [&](auto&& piece, Buffer& fwd, Buffer& bwd) {
+              std::vector<std::uint16_t> lf(256 * std::size_t(s.n_blocks), 0), lb(256 * std::size_t(s.n_blocks), 0);
+              for (int blk = 0; blk < s.n_blocks; ++blk) {
+                const int nb = t->block_size[blk];
+                const double* L = t->legendre + offsets[blk];  // (theta, J)
+                for (int lane = 0; lane < 64; ++lane)
+                  for (int e = 0; e < 4; ++e) {
+                    const int i = lane % 16, q = lane / 16, row = 4 * e + q;
+                    if (i < s.n_theta && row < nb) lf[256 * std::size_t(blk) + 4 * lane + e] = bf16(piece(L[i * nb + row]));
+                    if (row < s.n_theta && i < nb) lb[256 * std::size_t(blk) + 4 * lane + e] = bf16(piece(L[row * nb + i]));
+                  }
+              }
+              fwd = upload(lf.data(), lf.size() * sizeof(std::uint16_t));
+              bwd = upload(lb.data(), lb.size() * sizeof(std::uint16_t));
+            };
+            if (bf16x3) {
+              // split-BF16x3: A_hi = bf16(L), A_lo = bf16(L - A_hi)
+              fragments([&](double l) { return l; }, b->hi_fwd, b->hi_bwd);
+              fragments([&](double l) { return l - bf16_value(l); }, b->lo_fwd, b->lo_bwd);
+            } else {
+              // the low part L - float(L) of the FP32 contraction
+              fragments([](double l) { return l - static_cast<double>(static_cast<float>(l)); }, b->lo_fwd, b->lo_bwd);
+            }
+          }
+          variants = buf;
+          b->v_buf = true;
+        }
+      }
       b->v_variants = variants;
       b->v_c = 8;
       for (const Variant& v : variants) set_lds(v.fn, v.lds);
@@ -1075,8 +1272,11 @@ int spo_hip_create(const spo_hip_shape* shape, const spo_hip_tables* t, const sp
 }
 
 int spo_hip_destroy(spo_hip_backend* b) {
-  delete b;
-  return SPO_HIP_OK;
+  return guarded([&] {
+    if (b && b->batches > 0)
+      throw Status(SPO_HIP_E_ARGUMENT, "spo_hip_destroy: the backend is still a member of a batch (destroy the batch first)");
+    delete b;
+  });
 }
 
 int spo_hip_upload_wave(spo_hip_backend* b, const double* scattering, const double* specular) {
@@ -1184,6 +1384,107 @@ int spo_hip_propagate_batch(spo_hip_backend* const* backends, int n) {
         b->kinetic_and_potential(r == 0 || r == blocks - 1, r == blocks - 1);
       }
     for (int i = 0; i < n; ++i) HIP_CHECK(hipStreamSynchronize(backends[i]->stream));
+  });
+}
+
+}  // extern "C"
+
+// A batch of backends stepped together through one hipGraph: at its first step, block r of every member's schedule
+// is enqueued round robin on the members' streams (as spo_hip_propagate_batch does) inside one capture that starts
+// on the first member's stream, forks to the others through an event and joins them back; later steps replay the
+// graph with one launch. Each member's launches keep their order on its own streams, so a member's results are
+// bitwise those of the member stepped alone.
+struct spo_hip_batch {
+  std::vector<spo_hip_backend*> members;
+  hipEvent_t fork = nullptr;
+  std::vector<hipEvent_t> joins;
+  hipGraphExec_t graph = nullptr;
+  ~spo_hip_batch() {
+    if (graph) (void)hipGraphExecDestroy(graph);
+    if (fork) (void)hipEventDestroy(fork);
+    for (hipEvent_t e : joins) (void)hipEventDestroy(e);
+    for (spo_hip_backend* b : members) --b->batches;
+  }
+};
+
+namespace {
+
+void enqueue_round_robin(spo_hip_backend* const* backends, int n) {
+  int longest = 0;
+  for (int i = 0; i < n; ++i) longest = std::max(longest, backends[i]->shape.steps_per_analysis + 1);
+  for (int r = 0; r < longest; ++r)
+    for (int i = 0; i < n; ++i) {
+      spo_hip_backend* b = backends[i];
+      const int blocks = b->shape.steps_per_analysis + 1;
+      if (r >= blocks) continue;
+      b->kinetic_and_potential(r == 0 || r == blocks - 1, r == blocks - 1);
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int spo_hip_batch_create(spo_hip_backend* const* backends, int n, spo_hip_batch** out) {
+  return guarded([&] {
+    if (!out || n < 1 || !backends) throw Status(SPO_HIP_E_ARGUMENT, "spo_hip_batch_create: bad arguments");
+    *out = nullptr;
+    for (int i = 0; i < n; ++i) {
+      if (!backends[i]) throw Status(SPO_HIP_E_ARGUMENT, "spo_hip_batch_create: NULL backend");
+      if (backends[i]->profiling) throw Status(SPO_HIP_E_ARGUMENT, "spo_hip_batch_create: kernel timing is on");
+      for (int j = 0; j < i; ++j)
+        if (backends[j] == backends[i]) throw Status(SPO_HIP_E_ARGUMENT, "spo_hip_batch_create: a backend twice");
+    }
+    auto batch = std::make_unique<spo_hip_batch>();
+    HIP_CHECK(hipEventCreateWithFlags(&batch->fork, hipEventDisableTiming));
+    for (int i = 0; i < n; ++i) {
+      hipEvent_t e = nullptr;
+      HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+      batch->joins.push_back(e);
+    }
+    for (int i = 0; i < n; ++i) {
+      batch->members.push_back(backends[i]);
+      ++backends[i]->batches;
+    }
+    *out = batch.release();
+  });
+}
+
+int spo_hip_batch_step(spo_hip_batch* batch) {
+  return guarded([&] {
+    if (!batch) throw Status(SPO_HIP_E_ARGUMENT, "spo_hip_batch_step: NULL batch");
+    std::vector<spo_hip_backend*>& m = batch->members;
+    hipStream_t origin = m[0]->stream;
+    for (spo_hip_backend* b : m)
+      if (b->profiling) throw Status(SPO_HIP_E_ARGUMENT, "spo_hip_batch_step: kernel timing is on");
+    if (!batch->graph) {
+      // Every member's earlier work is complete before the capture (the members' streams join the capture only
+      // through the fork event).
+      for (spo_hip_backend* b : m) HIP_CHECK(hipStreamSynchronize(b->stream));
+      hipGraph_t graph = nullptr;
+      HIP_CHECK(hipStreamBeginCapture(origin, hipStreamCaptureModeThreadLocal));
+      HIP_CHECK(hipEventRecord(batch->fork, origin));
+      for (std::size_t i = 1; i < m.size(); ++i) HIP_CHECK(hipStreamWaitEvent(m[i]->stream, batch->fork, 0));
+      enqueue_round_robin(m.data(), static_cast<int>(m.size()));
+      for (std::size_t i = 1; i < m.size(); ++i) {
+        HIP_CHECK(hipEventRecord(batch->joins[i], m[i]->stream));
+        HIP_CHECK(hipStreamWaitEvent(origin, batch->joins[i], 0));
+      }
+      HIP_CHECK(hipStreamEndCapture(origin, &graph));
+      const hipError_t e = hipGraphInstantiate(&batch->graph, graph, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(graph);
+      HIP_CHECK(e);
+    }
+    HIP_CHECK(hipGraphLaunch(batch->graph, origin));
+    HIP_CHECK(hipStreamSynchronize(origin));
+  });
+}
+
+int spo_hip_batch_destroy(spo_hip_batch* batch) {
+  return guarded([&] {
+    if (!batch) return;
+    HIP_CHECK(hipStreamSynchronize(batch->members[0]->stream));
+    delete batch;
   });
 }
 
@@ -1563,6 +1864,7 @@ int spo_hip_pes_values(int pes, long n, const double* points, double* values) {
 }
 
 }  // extern "C"
+
 
 namespace {
 

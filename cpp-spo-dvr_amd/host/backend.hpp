@@ -93,6 +93,11 @@ class PropagationBackend {
   };
   virtual void set_kernel_timing(bool) {}
   virtual std::vector<KernelTime> kernel_times() { return {}; }
+
+  // The backend that does the work (a decorator returns its inner backend's), for propagate_batch; and the
+  // notice that propagate_batch propagated this backend's analysis step without calling propagate_analysis_step.
+  virtual PropagationBackend* innermost() { return this; }
+  virtual void note_batched_step() {}
 };
 
 using BackendFactory = std::function<std::unique_ptr<PropagationBackend>(const BackendInput&)>;
@@ -107,5 +112,14 @@ std::unique_ptr<PropagationBackend> make_hip_backend(const BackendInput& input);
 // One analysis step of each backend: the HIP backends' steps enqueued together on their own streams
 // (spo_hip_propagate_batch), the others one after another.
 void propagate_batch(const std::vector<PropagationBackend*>& backends);
+
+// Analysis steps of a fixed set of backends: the HIP backends' steps captured together into one hipGraph at the
+// first step and replayed with one launch per step (spo_hip_batch_*), the others one after another.
+class BatchStepper {
+ public:
+  virtual ~BatchStepper() = default;
+  virtual void step() = 0;
+};
+std::unique_ptr<BatchStepper> make_batch_stepper(const std::vector<PropagationBackend*>& backends);
 
 }  // namespace spo

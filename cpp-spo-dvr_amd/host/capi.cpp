@@ -7,6 +7,7 @@
 
 #include "../../include/spo_host.h"
 #include "checkpoint.hpp"
+#include "counting_backend.hpp"
 #include "driver.hpp"
 #include "json.hpp"
 #include "potential.hpp"
@@ -22,6 +23,7 @@ struct spo_scm {
 
 struct spo_run {
   std::shared_ptr<spo::RunSetup> setup;
+  spo::SeamCallCounts counts;  // before propagation: the backend's decorator refers to it
   std::unique_ptr<spo::Propagation> propagation;
 };
 
@@ -98,7 +100,12 @@ int spo_run_create(const char* input_path, const char* overrides_json, int devic
     } else {
       throw spo::NotSupportedError("unknown device " + std::to_string(device));
     }
-    run->propagation = std::make_unique<spo::Propagation>(run->setup, make);
+    // Every backend is made behind the counting decorator (spo_run_call_counts).
+    spo::SeamCallCounts& counts = run->counts;
+    spo::BackendFactory counted = [make, &counts](const spo::BackendInput& in) {
+      return std::unique_ptr<spo::PropagationBackend>(std::make_unique<spo::CountingBackend>(make(in), counts));
+    };
+    run->propagation = std::make_unique<spo::Propagation>(run->setup, counted);
     *out = run.release();
   });
 }
@@ -136,6 +143,14 @@ int spo_run_get_info(const spo_run* run, spo_run_info* info) {
 }
 
 void* spo_run_backend(spo_run* run) { return &run->propagation->backend(); }
+
+int spo_run_call_counts(const spo_run* run, long* counts, int capacity) {
+  return guarded([&] {
+    if (!run || !counts || capacity < spo::SeamCallCounts::kCount)
+      throw std::invalid_argument("spo_run_call_counts: NULL run or fewer than 12 slots");
+    run->counts.to_array(counts);
+  });
+}
 
 int spo_run_steps(spo_run* run, int n) {
   return guarded([&] { run->propagation->run(n); });
@@ -324,6 +339,19 @@ int spo_run_array(const spo_run* run, const char* name, double* out, size_t capa
       v = s.energies.total;
     } else if (n == "tables.z_optical") v = s.tables.z_optical;
     else if (n == "tables.r_optical") v = s.tables.r_optical;
+    else if (n == "tables.zr_full" || n == "tables.zr_half" || n == "tables.xy_full" || n == "tables.xy_half" ||
+             n == "tables.rotation_half") {
+      // complex factors as interleaved (re, im)
+      const std::vector<spo::cplx>& c = n == "tables.zr_full"   ? s.tables.zr_kinetic.full
+                                        : n == "tables.zr_half" ? s.tables.zr_kinetic.half
+                                        : n == "tables.xy_full" ? s.tables.xy_kinetic.full
+                                        : n == "tables.xy_half" ? s.tables.xy_kinetic.half
+                                                                : s.tables.rotation.half;
+      for (const spo::cplx& z : c) {
+        v.push_back(z.real());
+        v.push_back(z.imag());
+      }
+    }
     else if (n == "angular.legendre") v = spo::legendre_matrix(s.angular, s.basis);
     else if (n == "grids.z") v = s.grids.z.points;
     else if (n == "grids.r") v = s.grids.r.points;

@@ -1,7 +1,9 @@
 #include "driver.hpp"
 
 #include <cmath>
+#include <cstdlib>
 #include <filesystem>
+#include <string>
 
 #include "checkpoint.hpp"
 
@@ -156,6 +158,12 @@ void Propagation::step() {
 }
 
 void Propagation::run_batch(const std::vector<Propagation*>& runs, int max_steps) {
+  // Graph mode (SPO_HIP_GRAPH=1): the runs' steps through one captured hipGraph per set of live runs (recaptured
+  // when a run finishes); otherwise their blocks enqueued round robin at every step.
+  const char* g = std::getenv("SPO_HIP_GRAPH");
+  const bool graph = g && std::string(g) == "1";
+  std::unique_ptr<BatchStepper> stepper;
+  std::vector<Propagation*> stepper_runs;
   for (int i = 0; i < max_steps; ++i) {
     std::vector<Propagation*> live;
     std::vector<PropagationBackend*> backends;
@@ -165,7 +173,16 @@ void Propagation::run_batch(const std::vector<Propagation*>& runs, int max_steps
         backends.push_back(p->backend_.get());
       }
     if (live.empty()) return;
-    propagate_batch(backends);
+    if (graph) {
+      if (!stepper || live != stepper_runs) {
+        stepper.reset();
+        stepper = make_batch_stepper(backends);
+        stepper_runs = live;
+      }
+      stepper->step();
+    } else {
+      propagate_batch(backends);
+    }
     for (Propagation* p : live) p->finish_step();
   }
 }

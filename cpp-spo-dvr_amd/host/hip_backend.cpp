@@ -240,9 +240,10 @@ void propagate_batch(const std::vector<PropagationBackend*>& backends) {
   std::vector<HipBackend*> hip;
   std::vector<spo_hip_backend*> handles;
   for (PropagationBackend* b : backends) {
-    if (auto* h = dynamic_cast<HipBackend*>(b)) {
+    if (auto* h = dynamic_cast<HipBackend*>(b->innermost())) {
       hip.push_back(h);
       handles.push_back(h->handle());
+      b->note_batched_step();
     } else {
       b->propagate_analysis_step();
     }
@@ -250,6 +251,47 @@ void propagate_batch(const std::vector<PropagationBackend*>& backends) {
   if (handles.empty()) return;
   check(spo_hip_propagate_batch(handles.data(), static_cast<int>(handles.size())));
   for (HipBackend* h : hip) h->refresh_specular();
+}
+
+namespace {
+
+class GraphBatch final : public BatchStepper {
+ public:
+  explicit GraphBatch(const std::vector<PropagationBackend*>& backends) {
+    std::vector<spo_hip_backend*> handles;
+    for (PropagationBackend* b : backends) {
+      if (auto* h = dynamic_cast<HipBackend*>(b->innermost())) {
+        hip_.push_back(h);
+        hip_outer_.push_back(b);
+        handles.push_back(h->handle());
+      } else {
+        others_.push_back(b);
+      }
+    }
+    if (!handles.empty()) check(spo_hip_batch_create(handles.data(), static_cast<int>(handles.size()), &batch_));
+  }
+  ~GraphBatch() override {
+    if (batch_) (void)spo_hip_batch_destroy(batch_);
+  }
+  void step() override {
+    if (batch_) {
+      check(spo_hip_batch_step(batch_));
+      for (HipBackend* h : hip_) h->refresh_specular();
+      for (PropagationBackend* b : hip_outer_) b->note_batched_step();
+    }
+    for (PropagationBackend* b : others_) b->propagate_analysis_step();
+  }
+
+ private:
+  spo_hip_batch* batch_ = nullptr;
+  std::vector<HipBackend*> hip_;
+  std::vector<PropagationBackend*> hip_outer_, others_;
+};
+
+}  // namespace
+
+std::unique_ptr<BatchStepper> make_batch_stepper(const std::vector<PropagationBackend*>& backends) {
+  return std::make_unique<GraphBatch>(backends);
 }
 
 std::unique_ptr<PropagationBackend> make_hip_backend(const BackendInput& input) {

@@ -66,6 +66,7 @@ def host_library() -> ctypes.CDLL:
         lib.spo_run_get_info.argtypes = [ctypes.c_void_p, ctypes.POINTER(RunInfo)]
         lib.spo_run_backend.argtypes = [ctypes.c_void_p]
         lib.spo_run_backend.restype = ctypes.c_void_p
+        lib.spo_run_call_counts.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int]
         for fn in ("spo_run_steps",):
             getattr(lib, fn).argtypes = [ctypes.c_void_p, ctypes.c_int]
         lib.spo_run_propagate.argtypes = [ctypes.c_void_p]
@@ -419,6 +420,16 @@ class Run:
     @property
     def backend_pointer(self) -> int:
         return host_library().spo_run_backend(self._h)
+
+    SEAM_CALLS = ("propagate_analysis_step", "kinetic_and_potential", "z_plane", "block_densities", "specular",
+                  "wave_packet_on_host", "read_wave_packet", "site_column", "analyse", "s_matrix_sums",
+                  "add_earlier_analysis_step", "energy_resolved_add")
+
+    def call_counts(self) -> dict:
+        """The calls the run has made across the backend seam so far (spo_run_call_counts)."""
+        out = np.zeros(len(self.SEAM_CALLS), dtype=np.int64)
+        _check(host_library().spo_run_call_counts(self._h, _ptr(out), len(out)))
+        return dict(zip(self.SEAM_CALLS, (int(v) for v in out)))
 
     # time loop
     def steps(self, n: int) -> None:

@@ -122,6 +122,17 @@ int spo_hip_propagate_batch(spo_hip_backend* const* backends, int n);
  * backend's two streams; the step's launches and arguments are the same every step) and replayed from then on,
  * with the results of the uncaptured step. Not with kernel timing. No counterpart in the reference. */
 int spo_hip_graph_enable(spo_hip_backend* backend, int on);
+/* A batch of n distinct backends stepped together through one hipGraph: at the first spo_hip_batch_step the blocks
+ * of every member's analysis step are enqueued round robin on the members' streams (as spo_hip_propagate_batch)
+ * inside one capture that forks from the first member's stream to the others and joins them back; every step
+ * launches that graph once and returns when all members are done. Each member's results are those of
+ * spo_hip_propagate_analysis_step. Not with kernel timing. A member may not be destroyed before the batch
+ * (spo_hip_destroy refuses it). No counterpart in the reference: one launch per step for the batch of initial
+ * states or distorted surfaces of one device. */
+typedef struct spo_hip_batch spo_hip_batch;
+int spo_hip_batch_create(spo_hip_backend* const* backends, int n, spo_hip_batch** out);
+int spo_hip_batch_step(spo_hip_batch* batch);
+int spo_hip_batch_destroy(spo_hip_batch* batch);
 /* One T block: specular grid, T_Zr, T_xy, then V unless kinetic_only (src/propagation/split_operator.cpp:17-21);
  * the unit the reference's CUDA tests check stage by stage (tests/cuda/test_cuda_backend.cpp:94-106). */
 int spo_hip_kinetic_and_potential(spo_hip_backend* backend, int half_step, int kinetic_only);

@@ -53,6 +53,11 @@ int spo_run_destroy(spo_run* run);
 int spo_run_get_info(const spo_run* run, spo_run_info* info);
 /* The backend object (a spo::PropagationBackend*), for stage-level checks. */
 void* spo_run_backend(spo_run* run);
+/* The calls the run has made across the backend seam so far, after the reference's CountingBackend
+ * (tests/counting_backend.hpp:35-113): counts[0..11] = propagate_analysis_step (batched steps included),
+ * kinetic_and_potential, z_plane, block_densities, specular, wave_packet_on_host, read_wave_packet, site_column,
+ * analyse, s_matrix_sums, add_earlier_analysis_step, energy_resolved_add. capacity >= 12. */
+int spo_run_call_counts(const spo_run* run, long* counts, int capacity);
 
 /* Analysis steps of the time loop: propagate, move the analysis surface, project and accumulate. */
 int spo_run_steps(spo_run* run, int n_analysis_steps);

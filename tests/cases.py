@@ -104,6 +104,28 @@ DZ_6D = dict(_COMMON, **{
 })
 
 
+def _dz_member(vin: int, jin: int, mjin: int, jmax: int | None = None, nr: int = 48) -> dict:
+    """A DZ_6D run in complex64 with initial state (v, j, mJ): JMax 24 (MJMax 23) for even j, 23 for odd j (an odd
+    j needs an odd JMax for the homonuclear molecule); a given jmax and nr make the reduced members."""
+    if jmax is None:
+        jmax = 24 if jin % 2 == 0 else 23
+    p = dict(DZ_6D)
+    p.update({"Precision": "mixed", "VIn": vin, "JIn": jin, "MJIn": mjin, "JMax": jmax, "MJMax": min(jmax, 23),
+              "nR": nr, "dR": 5.2 / (nr - 1)})
+    return p
+
+
+# The 12 runs of the batched workload: eight initial states (v, j, mJ) from v in {0, 1} x j in {0 .. 3} x mJ in
+# {0 .. j} on the full DZ grid, and four at reduced JMax 12 / 18 with NR 32 (uneven sizes).
+BATCH_12 = [
+    ("v0j0m0", _dz_member(0, 0, 0)), ("v0j1m0", _dz_member(0, 1, 0)), ("v0j1m1", _dz_member(0, 1, 1)),
+    ("v0j2m0", _dz_member(0, 2, 0)), ("v0j2m2", _dz_member(0, 2, 2)), ("v0j3m0", _dz_member(0, 3, 0)),
+    ("v1j0m0", _dz_member(1, 0, 0)), ("v1j1m1", _dz_member(1, 1, 1)),
+    ("v0j0m0_jmax12_nr32", _dz_member(0, 0, 0, 12, 32)), ("v0j0m0_jmax18_nr32", _dz_member(0, 0, 0, 18, 32)),
+    ("v1j0m0_jmax12_nr32", _dz_member(1, 0, 0, 12, 32)), ("v0j2m1_jmax18_nr32", _dz_member(0, 2, 1, 18, 32)),
+]
+
+
 def write_case(name: str, params: dict, directory: str | None = None) -> str:
     """Writes params as <directory>/<name>.json (a temporary directory by default) and returns the path."""
     directory = directory or os.path.join(tempfile.gettempdir(), "spodvr_cases")

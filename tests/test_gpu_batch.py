@@ -69,3 +69,46 @@ def test_batch_of_small_runs_is_faster():
     finally:
         for r in runs:
             r.close()
+
+
+def _graph_mode(on):
+    import os
+    if on:
+        os.environ["SPO_HIP_GRAPH"] = "1"
+    else:
+        os.environ.pop("SPO_HIP_GRAPH", None)
+
+
+def test_batch_graph_equals_single_runs():
+    """The batch in graph mode (SPO_HIP_GRAPH=1: one hipGraph captured over every member's streams, one launch per
+    analysis step, spo_hip_batch_*) gives each member the BK records and S sums of the member stepped alone, bit for
+    bit, over several replays; the members differ in size, precision and steps per analysis."""
+    steps = 4
+    single = []
+    for case, precision in MEMBERS:
+        _note("single (eager)", case, precision)
+        run = _run(case, precision)
+        try:
+            run.steps(steps)
+            single.append((run.bk_records(), run.s_sums()))
+        finally:
+            run.close()
+    _graph_mode(True)
+    try:
+        _note("graph batch of", len(MEMBERS))
+        runs = [_run(c, p) for c, p in MEMBERS]
+    finally:
+        _graph_mode(False)
+    try:
+        _graph_mode(True)
+        try:
+            spodvr_amd.run_batch(runs, steps)
+        finally:
+            _graph_mode(False)
+        for (bk, s), run in zip(single, runs):
+            assert np.array_equal(run.bk_records(), bk)
+            assert np.array_equal(run.s_sums(), s)
+            assert run.call_counts()["propagate_analysis_step"] == steps
+    finally:
+        for r in runs:
+            r.close()
