@@ -250,8 +250,9 @@ constexpr int ct_max_values() {
   return sizeof(T) == 8 ? 16 : 32;
 }
 
+// eps (complex64): the twiddle table's modulus corrections, 1 / |table[m]| - 1 (applied through mul_hilo).
 template <typename T, int N, int STAGE, int THREADS, class Layout>
-__device__ __forceinline__ void ct_stage(cx<T>* buf, int count, const cx<T>* table, int sign) {
+__device__ __forceinline__ void ct_stage(cx<T>* buf, int count, const cx<T>* table, const T* eps, int sign) {
   using P = CtPlan<N>;
   constexpr int R = P::radix(STAGE);
   constexpr int SPAN = P::span(STAGE);
@@ -284,7 +285,12 @@ __device__ __forceinline__ void ct_stage(cx<T>* buf, int count, const cx<T>* tab
             for (int r = 1; r < R; ++r) {
               cx<T> w = table[r * k * TW];
               if (sign > 0) w = conj(w);
-              v[q][r] = v[q][r] * w;
+              if constexpr (sizeof(T) == 4) {
+                const T e = eps[r * k * TW];
+                v[q][r] = mul_hilo(w, {e * w.x, e * w.y}, v[q][r]);
+              } else {
+                v[q][r] = v[q][r] * w;
+              }
             }
           }
         }
@@ -304,10 +310,10 @@ __device__ __forceinline__ void ct_stage(cx<T>* buf, int count, const cx<T>* tab
 }
 
 template <typename T, int N, int THREADS, class Layout, int STAGE = 0>
-__device__ __forceinline__ void ct_fft(cx<T>* buf, int count, const cx<T>* table, int sign) {
+__device__ __forceinline__ void ct_fft(cx<T>* buf, int count, const cx<T>* table, const T* eps, int sign) {
   if constexpr (STAGE < CtPlan<N>::stages) {
-    ct_stage<T, N, STAGE, THREADS, Layout>(buf, count, table, sign);
-    ct_fft<T, N, THREADS, Layout, STAGE + 1>(buf, count, table, sign);
+    ct_stage<T, N, STAGE, THREADS, Layout>(buf, count, table, eps, sign);
+    ct_fft<T, N, THREADS, Layout, STAGE + 1>(buf, count, table, eps, sign);
   }
 }
 

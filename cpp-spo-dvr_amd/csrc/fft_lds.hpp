@@ -36,12 +36,29 @@ __device__ __forceinline__ cx<T> conj(cx<T> a) { return {a.x, -a.y}; }
 template <typename T>
 __device__ __forceinline__ cx<T> times_i(cx<T> a, int sign) { return sign < 0 ? cx<T>{a.y, -a.x} : cx<T>{-a.y, a.x}; }
 
+// w v for a complex64 factor held as hi + lo: hi the nearest complex64 value of the FP64 factor, lo the remainder
+// in FP32. The lo products are formed first and the hi products fused on top of them, so the sum is rounded once
+// more at most and the factor takes part with about twice the float precision, modulus and phase. A correction
+// added after the product is rounded (p + eps p) does not do this: a relative change below half a unit in the last
+// place of p rounds back to p. FP64: hi v.
+template <typename T>
+__device__ __forceinline__ cx<T> mul_hilo(cx<T> hi, cx<T> lo, cx<T> v) {
+  if constexpr (sizeof(T) == 8) {
+    return hi * v;
+  } else {
+    const float re = fmaf(hi.x, v.x, fmaf(-hi.y, v.y, fmaf(lo.x, v.x, -lo.y * v.y)));
+    const float im = fmaf(hi.x, v.y, fmaf(hi.y, v.x, fmaf(lo.x, v.y, lo.y * v.x)));
+    return {re, im};
+  }
+}
+
 // Plan of one length: radices in the order of the stages and the twiddle table.
 struct FftPlan {
   int n;
   int n_stages;
   int radix[12];
   const void* table;  // cx<T>[n], exp(-2 pi i m / n)
+  const void* eps;    // complex64: T[n], 1 / |table[m]| - 1 (applied through mul_hilo); FP64: unused
 };
 
 struct SeqLayout {
@@ -152,7 +169,8 @@ __device__ __forceinline__ void dft(cx<T>* v, int sign) {
 constexpr int kMaxValues = 16;
 
 template <int R, typename T>
-__device__ void lds_stage(cx<T>* buf, const SeqLayout& lay, int n, int span, const cx<T>* table, int sign) {
+__device__ void lds_stage(cx<T>* buf, const SeqLayout& lay, int n, int span, const cx<T>* table, const void* eps,
+                          int sign) {
   constexpr int NB = kMaxValues / R;
   const int tid = threadIdx.x;
   const int nthreads = blockDim.x;
@@ -180,7 +198,12 @@ __device__ void lds_stage(cx<T>* buf, const SeqLayout& lay, int n, int span, con
           if (r > 0 && k > 0) {
             cx<T> w = table[r * k * tw_step];
             if (sign > 0) w = conj(w);
-            x = x * w;
+            if constexpr (sizeof(T) == 4) {
+              const T e = static_cast<const T*>(eps)[r * k * tw_step];
+              x = mul_hilo(w, {e * w.x, e * w.y}, x);
+            } else {
+              x = x * w;
+            }
           }
           v[q][r] = x;
         }
@@ -207,13 +230,13 @@ __device__ void lds_fft(cx<T>* buf, const SeqLayout& lay, const FftPlan& plan, i
   for (int st = 0; st < plan.n_stages; ++st) {
     const int R = plan.radix[st];
     switch (R) {
-      case 2: lds_stage<2, T>(buf, lay, plan.n, span, table, sign); break;
-      case 3: lds_stage<3, T>(buf, lay, plan.n, span, table, sign); break;
-      case 4: lds_stage<4, T>(buf, lay, plan.n, span, table, sign); break;
-      case 5: lds_stage<5, T>(buf, lay, plan.n, span, table, sign); break;
-      case 7: lds_stage<7, T>(buf, lay, plan.n, span, table, sign); break;
-      case 8: lds_stage<8, T>(buf, lay, plan.n, span, table, sign); break;
-      case 11: lds_stage<11, T>(buf, lay, plan.n, span, table, sign); break;
+      case 2: lds_stage<2, T>(buf, lay, plan.n, span, table, plan.eps, sign); break;
+      case 3: lds_stage<3, T>(buf, lay, plan.n, span, table, plan.eps, sign); break;
+      case 4: lds_stage<4, T>(buf, lay, plan.n, span, table, plan.eps, sign); break;
+      case 5: lds_stage<5, T>(buf, lay, plan.n, span, table, plan.eps, sign); break;
+      case 7: lds_stage<7, T>(buf, lay, plan.n, span, table, plan.eps, sign); break;
+      case 8: lds_stage<8, T>(buf, lay, plan.n, span, table, plan.eps, sign); break;
+      case 11: lds_stage<11, T>(buf, lay, plan.n, span, table, plan.eps, sign); break;
       default: break;
     }
     span *= R;

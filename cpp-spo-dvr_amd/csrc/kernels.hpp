@@ -17,12 +17,33 @@ struct Shape {
   int r_abs_first, r_abs_last;
 };
 
+// w v for a factor w of a complex64 table whose entry idx has the FP32 remainder lo[idx] of the FP64 factor
+// (mul_hilo); FP64 passes no lo.
+template <typename T>
+__device__ __forceinline__ cx<T> mul_factor(cx<T> w, cx<T> v, const cx<T>* lo, int idx) {
+  if constexpr (sizeof(T) == 4) {
+    return mul_hilo(w, lo[idx], v);
+  } else {
+    return w * v;
+  }
+}
+
+// The complex64 remainders (mul_hilo) of the tables xy_reg_kernel multiplies with (NULL in FP64).
+template <typename T>
+struct XyLo {
+  const cx<T>* factor;
+  const cx<T>* shift_f;
+  const cx<T>* shift_b;
+  const cx<T>* before;
+  const cx<T>* gas_step;
+};
+
 // ---------------------------------------------------------------------------------------------------------------
 // Batched 1D FFTs of lines in global memory (the specular grid, and the T_Zr pass when a whole (Z, r) plane does
 // not fit in LDS). Line l of the batch: offset (l % inner) * inner_stride + (l / inner) * outer_stride, element i
 // at offset + i * elem_stride. mode 0: transform with sign_a; mode 1: transform (sign_a), multiply by
-// factor[(l % inner) * f_line + i * f_elem], transform back (-sign_a). Each workgroup takes `lines_per_block`
-// consecutive lines.
+// factor[(l % inner) * f_line + i * f_elem] (complex64: with its remainder factor_lo, mul_factor), transform back
+// (-sign_a). Each workgroup takes `lines_per_block` consecutive lines.
 struct Lines {
   int n;
   int count, inner;
@@ -32,7 +53,8 @@ struct Lines {
 
 template <typename T>
 __global__ __launch_bounds__(256) void lines_fft_kernel(cx<T>* data, Lines L, FftPlan plan, int sign_a, int mode,
-                                                        const cx<T>* factor, int f_line, int f_elem) {
+                                                        const cx<T>* factor, const cx<T>* factor_lo, int f_line,
+                                                        int f_elem) {
   extern __shared__ unsigned char smem_raw[];
   cx<T>* tile = reinterpret_cast<cx<T>*>(smem_raw);
   const int l0 = blockIdx.x * L.lines_per_block;
@@ -52,7 +74,8 @@ __global__ __launch_bounds__(256) void lines_fft_kernel(cx<T>* data, Lines L, Ff
     for (int e = threadIdx.x; e < total; e += blockDim.x) {
       const int li = e / L.n, i = e % L.n;
       const int l = l0 + li;
-      tile[e] = factor[(l % L.inner) * f_line + i * f_elem] * tile[e];
+      const int f = (l % L.inner) * f_line + i * f_elem;
+      tile[e] = mul_factor(factor[f], tile[e], factor_lo, f);
     }
     __syncthreads();
     lds_fft<T>(tile, lay, plan, -sign_a);
@@ -82,10 +105,10 @@ __global__ void specular_potential_kernel(cx<T>* spec, Shape s, const cx<T>* gas
 
 // ---------------------------------------------------------------------------------------------------------------
 // T_Zr: one (irot, ixy) plane per workgroup, the whole (Z, r) plane in LDS: 2D FFT, x exp(-i (T_Z + T_r) dt) with
-// the FFT scales folded in, inverse 2D FFT. One read and one write of psi.
+// the FFT scales folded in (complex64: with its remainder factor_lo), inverse 2D FFT. One read and one write of psi.
 template <typename T>
 __global__ __launch_bounds__(256) void zr_plane_kernel(cx<T>* psi, Shape s, FftPlan plan_z, FftPlan plan_r,
-                                                       const cx<T>* factor) {
+                                                       const cx<T>* factor, const cx<T>* factor_lo) {
   extern __shared__ unsigned char smem_raw[];
   cx<T>* tile = reinterpret_cast<cx<T>*>(smem_raw);
   cx<T>* plane = psi + (long)blockIdx.x * s.n_zr;
@@ -95,7 +118,7 @@ __global__ __launch_bounds__(256) void zr_plane_kernel(cx<T>* psi, Shape s, FftP
   const SeqLayout cols{s.nr, s.nr, 1, 0, s.nr};   // Z sequences: stride nr, one per ir
   lds_fft<T>(tile, rows, plan_r, -1);
   lds_fft<T>(tile, cols, plan_z, -1);
-  for (int e = threadIdx.x; e < s.n_zr; e += blockDim.x) tile[e] = factor[e] * tile[e];
+  for (int e = threadIdx.x; e < s.n_zr; e += blockDim.x) tile[e] = mul_factor(factor[e], tile[e], factor_lo, e);
   __syncthreads();
   lds_fft<T>(tile, cols, plan_z, +1);
   lds_fft<T>(tile, rows, plan_r, +1);
@@ -111,7 +134,7 @@ template <typename T>
 __global__ __launch_bounds__(256) void xy_kernel(cx<T>* psi, Shape s, int C, FftPlan plan_x, FftPlan plan_y,
                                                  const cx<T>* factor,
                           const cx<T>* shift_f, const cx<T>* shift_b, int needs_shift, int apply_potential,
-                          const T* z_opt, const cx<T>* before, const cx<T>* gas_step) {
+                          const T* z_opt, const cx<T>* before, const cx<T>* gas_step, XyLo<T> lo) {
   extern __shared__ unsigned char smem_raw[];
   cx<T>* tile = reinterpret_cast<cx<T>*>(smem_raw);
   const int tiles_per_rot = (s.n_zr + C - 1) / C;
@@ -123,7 +146,7 @@ __global__ __launch_bounds__(256) void xy_kernel(cx<T>* psi, Shape s, int C, Fft
     const int c = e % C, ixy = e / C;
     cx<T> v = {T(0), T(0)};
     if (c < cw) v = psi[izr0 + c + (long)s.n_zr * (irot + (long)s.n_rot * ixy)];
-    if (needs_shift) v = v * shift_f[ixy];
+    if (needs_shift) v = mul_factor(shift_f[ixy], v, lo.shift_f, ixy);
     tile[e] = v;
   }
   __syncthreads();
@@ -131,7 +154,7 @@ __global__ __launch_bounds__(256) void xy_kernel(cx<T>* psi, Shape s, int C, Fft
   const SeqLayout xs{C * s.ny, C * s.ny, 1, 0, C * s.ny};  // for each (iy, c): ix at stride C ny
   lds_fft<T>(tile, ys, plan_y, -1);
   lds_fft<T>(tile, xs, plan_x, -1);
-  for (int e = threadIdx.x; e < total; e += blockDim.x) tile[e] = factor[e / C] * tile[e];
+  for (int e = threadIdx.x; e < total; e += blockDim.x) tile[e] = mul_factor(factor[e / C], tile[e], lo.factor, e / C);
   __syncthreads();
   lds_fft<T>(tile, xs, plan_x, +1);
   lds_fft<T>(tile, ys, plan_y, +1);
@@ -139,12 +162,13 @@ __global__ __launch_bounds__(256) void xy_kernel(cx<T>* psi, Shape s, int C, Fft
     const int c = e % C, ixy = e / C;
     if (c >= cw) continue;
     cx<T> v = tile[e];
-    if (needs_shift) v = v * shift_b[ixy];
+    if (needs_shift) v = mul_factor(shift_b[ixy], v, lo.shift_b, ixy);
     const int izr = izr0 + c;
     if (apply_potential) {
       const int iz = izr / s.nr, ir = izr % s.nr;
-      const cx<T> f = iz >= s.nzpot ? gas_step[irot + s.n_rot * ir] : before[irot + s.n_rot * ir];
-      v = scale(f * v, z_opt[iz]);
+      const int k = irot + s.n_rot * ir;
+      const cx<T> u = iz >= s.nzpot ? mul_factor(gas_step[k], v, lo.gas_step, k) : mul_factor(before[k], v, lo.before, k);
+      v = scale(u, z_opt[iz]);
     }
     psi[izr + (long)s.n_zr * (irot + (long)s.n_rot * ixy)] = v;
   }
@@ -170,7 +194,8 @@ struct AngularTables {
 template <typename T>
 __global__ __launch_bounds__(256) void interaction_kernel(cx<T>* psi, const cx<double>* spec, Shape s, int C,
                                                           FftPlan plan_phi, AngularTables at, const cx<T>* exp_table,
-                                                          const cx<T>* rot_half, const cx<T>* s2sc) {
+                                                          const cx<T>* rot_half, const cx<T>* s2sc,
+                                                          const cx<T>* rot_lo) {
   extern __shared__ unsigned char smem_raw[];
   cx<T>* fbr = reinterpret_cast<cx<T>*>(smem_raw);
   cx<T>* dvr = fbr + C * s.n_rot;
@@ -244,7 +269,7 @@ __global__ __launch_bounds__(256) void interaction_kernel(cx<T>* psi, const cx<d
     for (int th = 0; th < s.n_theta; ++th) acc = acc + scale(dvr[(c * s.n_theta + th) * s.n_phi + slot], T(L[th * nj]));
     const int izr = izr0 + c;
     const int ir = izr % s.nr;
-    psi[izr + col + (long)s.n_zr * irot] = rot_half[irot + s.n_rot * ir] * acc;
+    psi[izr + col + (long)s.n_zr * irot] = mul_factor(rot_half[irot + s.n_rot * ir], acc, rot_lo, irot + s.n_rot * ir);
   }
 }
 

@@ -30,9 +30,10 @@ constexpr int per_thread() {
 
 // SKIP_ZERO: skip the (exact) multiplications by 1 of j == 0, worth a branch when j is uniform over a wavefront.
 // Table entry k sits at table[k * TS]. In FP32 the table entries cannot all be of modulus 1 (no pair of floats near
-// an eighth turn (h, h) is), so each comes with eps[k * ES] = 1 / |w_k| - 1 (a few 1e-8, from FP64) and the product
-// is corrected as p + eps p: the twiddles then keep the norm to ~1e-15 instead of shrinking the modes they multiply
-// by up to ~3e-8 at every transform. FP64 passes no eps.
+// an eighth turn (h, h) is), so each comes with eps[k * ES] = 1 / |w_k| - 1 (a few 1e-8, from FP64) and the
+// twiddle is applied as w + eps w in hi + lo form (mul_hilo): the twiddles then keep the norm to ~1e-15 instead of
+// shrinking the modes they multiply by up to ~3e-8 at every transform. Their phase errors need no correction: the
+// inverse transform applies the conjugates, which undo them. FP64 passes no eps.
 template <typename T, int R, bool SKIP_ZERO = true, int TS = 1, int ES = 1>
 __device__ __forceinline__ void twiddle(cx<T>* v, const cx<T>* table, int j, int sign, const T* eps = nullptr) {
   if (SKIP_ZERO && j == 0) return;
@@ -40,35 +41,14 @@ __device__ __forceinline__ void twiddle(cx<T>* v, const cx<T>* table, int j, int
   for (int r = 1; r < R; ++r) {
     cx<T> w = table[r * j * TS];
     if (sign > 0) w = conj(w);
-    v[r] = v[r] * w;
     if constexpr (sizeof(T) == 4) {
       const T e = eps[r * j * ES];
-      v[r] = {fmaf(v[r].x, e, v[r].x), fmaf(v[r].y, e, v[r].y)};
+      v[r] = mul_hilo(w, {e * w.x, e * w.y}, v[r]);
+    } else {
+      v[r] = v[r] * w;
     }
   }
 }
-
-// w v for a factor w of a complex64 table whose entry idx has the modulus correction eps[idx] = |w_exact| / |w| - 1
-// (the product's modulus then matches the FP64 factor's to ~1e-15; see twiddle()); FP64 passes no eps.
-template <typename T>
-__device__ __forceinline__ cx<T> mul_factor(cx<T> w, cx<T> v, const T* eps, int idx) {
-  cx<T> p = w * v;
-  if constexpr (sizeof(T) == 4) {
-    const T e = eps[idx];
-    p = {fmaf(p.x, e, p.x), fmaf(p.y, e, p.y)};
-  }
-  return p;
-}
-
-// The complex64 modulus corrections of the tables xy_reg_kernel multiplies with (NULL in FP64).
-template <typename T>
-struct XyEps {
-  const T* factor;
-  const T* shift_f;
-  const T* shift_b;
-  const T* before;
-  const T* gas_step;
-};
 
 template <typename T>
 constexpr bool zr_twiddles_in_lds() {
@@ -82,11 +62,12 @@ inline std::size_t zr_reg_lds(int nz, int nr) {
          (zr_twiddles_in_lds<T>() ? std::size_t(nz + nr) * (sizeof(cx<T>) + sizeof(T)) : 0);
 }
 
-// eps_z, eps_r: the complex64 twiddle tables' modulus corrections (twiddle()); unused in FP64.
+// eps_z, eps_r: the complex64 twiddle tables' modulus corrections (twiddle()); factor_lo: the factor's FP32
+// remainders (mul_hilo). Unused in FP64.
 template <typename T, int NZ, int NR, int A1, int A2, int B1, int B2, int THREADS>
 __global__ __launch_bounds__(THREADS) void zr_reg_kernel(cx<T>* psi, const cx<T>* table_z, const cx<T>* table_r,
                                                          const cx<T>* factor, const T* eps_z, const T* eps_r,
-                                                         const T* factor_eps) {
+                                                         const cx<T>* factor_lo) {
   static_assert(A1 * A2 == NR && B1 * B2 == NZ, "zr_reg_kernel: radices must multiply to the lengths");
   extern __shared__ unsigned char smem_raw[];
   cx<T>* t = reinterpret_cast<cx<T>*>(smem_raw);
@@ -191,7 +172,7 @@ __global__ __launch_bounds__(THREADS) void zr_reg_kernel(cx<T>* psi, const cx<T>
 #pragma unroll
       for (int r = 0; r < B2; ++r) {
         const int f = (j + r * B1) * NR + kr;
-        v[q][r] = mul_factor(factor[f], v[q][r], factor_eps, f);
+        v[q][r] = mul_factor(factor[f], v[q][r], factor_lo, f);
       }
       rdft<B2, T>(v[q], +1);
     }
@@ -291,7 +272,7 @@ template <typename T, int NX, int NY, int C, bool SPLIT = false>
 __global__ __launch_bounds__(C * NX) __attribute__((amdgpu_waves_per_eu(3))) void xy_reg_kernel(cx<T>* psi, Shape s, const cx<T>* factor,
                                                          const cx<T>* shift_f, const cx<T>* shift_b, int needs_shift,
                                                          int apply_potential, const T* z_opt, const cx<T>* before,
-                                                         const cx<T>* gas_step, XyEps<T> eps) {
+                                                         const cx<T>* gas_step, XyLo<T> lo) {
   static_assert(single_radix<NX>() && single_radix<NY>() && NX == NY, "xy_reg_kernel: one radix per axis, NX = NY");
   static_assert(!SPLIT || NY % 2 == 0, "xy_reg_kernel: halves of the y lines");
   extern __shared__ unsigned char smem_raw[];
@@ -315,7 +296,7 @@ __global__ __launch_bounds__(C * NX) __attribute__((amdgpu_waves_per_eu(3))) voi
   for (int iy = 0; iy < NY; ++iy) {
     const int ixy = a * NY + iy;
     v[iy] = live ? g[xy_stride * ixy] : cx<T>{T(0), T(0)};
-    if (needs_shift) v[iy] = mul_factor(shift_f[ixy], v[iy], eps.shift_f, ixy);
+    if (needs_shift) v[iy] = mul_factor(shift_f[ixy], v[iy], lo.shift_f, ixy);
   }
   rdft<NY, T>(v, -1);
   cx<T> w[NX];
@@ -341,7 +322,7 @@ __global__ __launch_bounds__(C * NX) __attribute__((amdgpu_waves_per_eu(3))) voi
   }
   rdft<NX, T>(w, -1);
 #pragma unroll
-  for (int ix = 0; ix < NX; ++ix) w[ix] = mul_factor(factor[ix * NY + a], w[ix], eps.factor, ix * NY + a);
+  for (int ix = 0; ix < NX; ++ix) w[ix] = mul_factor(factor[ix * NY + a], w[ix], lo.factor, ix * NY + a);
   rdft<NX, T>(w, +1);
   if constexpr (!SPLIT) {
 #pragma unroll
@@ -367,22 +348,21 @@ __global__ __launch_bounds__(C * NX) __attribute__((amdgpu_waves_per_eu(3))) voi
   const int izr = izr0 + c;
   const int iz = izr / s.nr, ir = izr % s.nr;
   cx<T> f = {T(1), T(0)};
+  cx<T> f_lo = {T(0), T(0)};  // the FP32 remainder of f (complex64)
   T zo = T(1);
-  T fe = T(0);  // the modulus correction of f (complex64)
   if (apply_potential) {
     const int k = irot + s.n_rot * ir;
     f = iz >= s.nzpot ? gas_step[k] : before[k];
-    if constexpr (sizeof(T) == 4) fe = iz >= s.nzpot ? eps.gas_step[k] : eps.before[k];
+    if constexpr (sizeof(T) == 4) f_lo = iz >= s.nzpot ? lo.gas_step[k] : lo.before[k];
     zo = z_opt[iz];
   }
 #pragma unroll
   for (int iy = 0; iy < NY; ++iy) {
     const int ixy = a * NY + iy;
     cx<T> u = v[iy];
-    if (needs_shift) u = mul_factor(shift_b[ixy], u, eps.shift_b, ixy);
+    if (needs_shift) u = mul_factor(shift_b[ixy], u, lo.shift_b, ixy);
     if (apply_potential) {
-      u = f * u;
-      if constexpr (sizeof(T) == 4) u = {fmaf(u.x, fe, u.x), fmaf(u.y, fe, u.y)};
+      u = mul_hilo(f, f_lo, u);
       u = scale(u, zo);
     }
     g[xy_stride * ixy] = u;
@@ -409,7 +389,8 @@ __global__ __launch_bounds__(THREADS) void interaction_ct_kernel(cx<T>* psi, con
                                                                  const cx<T>* table_phi, AngularTables at,
                                                                  const cx<T>* exp_table, const cx<T>* rot_half,
                                                                  const cx<T>* s2sc, const T* legendre,
-                                                                 int legendre_size) {
+                                                                 int legendre_size, const cx<T>* rot_lo,
+                                                                 const T* eps_phi) {
   extern __shared__ unsigned char smem_raw[];
   constexpr int LD = NPHI + 1;
   const int n_theta = s.n_theta;
@@ -465,7 +446,7 @@ __global__ __launch_bounds__(THREADS) void interaction_ct_kernel(cx<T>* psi, con
   }
   __syncthreads();
   using Rows = CtLayout<0, LD, 0, 1>;
-  ct_fft<T, NPHI, THREADS, Rows>(dvr, C * n_theta, table_phi, +1);
+  ct_fft<T, NPHI, THREADS, Rows>(dvr, C * n_theta, table_phi, eps_phi, +1);
 
   const int rep = at.rep_of[ixy];
   const int n_tp = n_theta * NPHI;
@@ -481,7 +462,7 @@ __global__ __launch_bounds__(THREADS) void interaction_ct_kernel(cx<T>* psi, con
     v = exp_table[itp + static_cast<long>(n_tp) * (rep + static_cast<long>(s.n_rep) * (izr0 + cc))] * v;
   }
   __syncthreads();
-  ct_fft<T, NPHI, THREADS, Rows>(dvr, C * n_theta, table_phi, -1);
+  ct_fft<T, NPHI, THREADS, Rows>(dvr, C * n_theta, table_phi, eps_phi, -1);
 
   // DVR -> FBR, then rot(dt/2), stored straight to psi.
   for (int tt = threadIdx.x; tt < n_tasks; tt += THREADS) {
@@ -503,7 +484,8 @@ __global__ __launch_bounds__(THREADS) void interaction_ct_kernel(cx<T>* psi, con
       for (int th = 0; th < kMaxTheta; ++th)
         if (th < n_theta) acc = acc + scale(d[th], L[th * nb + j]);
       const int irot = f0 + j;
-      psi[izr + col + static_cast<long>(s.n_zr) * irot] = rot_half[irot + s.n_rot * ir] * acc;
+      psi[izr + col + static_cast<long>(s.n_zr) * irot] = mul_factor(rot_half[irot + s.n_rot * ir], acc, rot_lo,
+                                                                      irot + s.n_rot * ir);
     }
   }
 }

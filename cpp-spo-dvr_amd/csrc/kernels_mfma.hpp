@@ -232,7 +232,7 @@ template <typename T, int NPHI, int NT, int WAVES, int R1 = 0, int R2 = 0>
 __global__ __launch_bounds__(64 * WAVES) void interaction_mfma_kernel(
     cx<T>* psi, const cx<double>* spec, Shape s, const cx<T>* table_phi, AngularTables at, MfmaTables mt,
     const T* frag_fwd, const T* frag_bwd, int max_jt, const cx<T>* exp_table, const cx<T>* rot_t,
-    const cx<T>* s2sc, const T* eps_phi) {
+    const cx<T>* s2sc, const T* eps_phi, const cx<T>* rot_lo_t) {
   using M = Mfma<T>;
   using acc_t = typename M::acc_t;
   using Tile = DvrTile<NPHI, dvr_swizzled<T, NPHI, R1>()>;
@@ -412,7 +412,7 @@ __global__ __launch_bounds__(64 * WAVES) void interaction_mfma_kernel(
     });
   } else {
     using Rows = CtLayout<0, Tile::ld, 0, 1>;
-    ct_fft<T, NPHI, THREADS, Rows>(dvr, rows, tw_phi, +1);
+    ct_fft<T, NPHI, THREADS, Rows>(dvr, rows, tw_phi, eps_phi, +1);
     for (int e = tid; e < rows * NPHI; e += THREADS) {
       const int ip = e % NPHI;
       const int row = e / NPHI;  // theta * C + col
@@ -423,7 +423,7 @@ __global__ __launch_bounds__(64 * WAVES) void interaction_mfma_kernel(
       v = exp_table[phi_map[ip] + NPHI * th + static_cast<long>(n_tp) * (rep + static_cast<long>(s.n_rep) * (izr0 + cc))] * v;
     }
     __syncthreads();
-    ct_fft<T, NPHI, THREADS, Rows>(dvr, rows, tw_phi, -1);
+    ct_fft<T, NPHI, THREADS, Rows>(dvr, rows, tw_phi, eps_phi, -1);
   }
 
   // 3. DVR -> FBR, x rot(dt/2), to HBM. Task = (mJ block, J tile, N tile); max_jt = ceil(largest block / 16).
@@ -482,7 +482,7 @@ __global__ __launch_bounds__(64 * WAVES) void interaction_mfma_kernel(
           const int j = M::row(lane, v);
           const cx<T> c = ri ? cx<T>{other, mine} : cx<T>{mine, other};
           if (live_of[nt] && j < nb) {
-            const cx<T> out = rot[nt][v] * c;
+            const cx<T> out = mul_factor(rot[nt][v], c, rot_lo_t, ir_of[nt] + s.nr * (f0 + j));
             dst[2 * (izr_of[nt] + s.n_zr * j) + ri] = ri ? out.y : out.x;
           }
         }
@@ -518,7 +518,7 @@ __global__ __launch_bounds__(64 * WAVES) void interaction_mfma_kernel(
           const cx<T> c = ri ? cx<T>{other, mine} : cx<T>{mine, other};
           if (cc < cw && j < nb) {
             const int irot = f0 + j;
-            const cx<T> out = rot_t[ir + s.nr * irot] * c;
+            const cx<T> out = mul_factor(rot_t[ir + s.nr * irot], c, rot_lo_t, ir + s.nr * irot);
             site_t[2 * (izr + s.n_zr * irot) + ri] = ri ? out.y : out.x;
           }
         }
@@ -615,15 +615,15 @@ __device__ __forceinline__ Mfma<float>::acc_t legendre_bf16x3(rsrc_t hi, rsrc_t 
 }
 
 // An entry of the sign-split rotation table rot_pm (16 bytes): (Re w, -Im w) for the lane holding Re c and
-// (Re w, Im w) for the lane holding Im c, and in complex64 the modulus correction eps = |w_exact| / |w| - 1
-// (mul_factor, kernels_ct.hpp).
+// (Re w, Im w) for the lane holding Im c; in complex64 followed by the FP32 remainders of the same two values
+// (the FP64 factor minus its nearest complex64 value, mul_hilo in kernels_ct.hpp).
 template <typename T>
 struct alignas(16) RotEntry {
   T x, y;
 };
 template <>
 struct alignas(16) RotEntry<float> {
-  float x, y, eps, pad;
+  float x, y, lo_x, lo_y;
 };
 
 // The interaction step of one (8 izr, site) tile with buffer addressing, for shapes where every mJ block has at
@@ -682,7 +682,8 @@ __global__ __launch_bounds__(64 * WAVES, sizeof(T) == 8 ? 1 : 2) void interactio
   const rsrc_t site = buffer_rsrc(psi + static_cast<long>(s.n_zr) * s.n_rot * ixy, static_cast<long>(s.n_zr) * s.n_rot * CB);
   const rsrc_t ffwd = buffer_rsrc(frag_fwd, 64L * frag_fwd_count * sizeof(T));
   const rsrc_t fbwd = buffer_rsrc(frag_bwd, 64L * frag_bwd_count * sizeof(T));
-  // rot_pm entries are 16 bytes: (Re w, +-Im w) in FP64, (Re w, +-Im w, eps, 0) in complex64 (RotEntry)
+  // rot_pm entries are 16 bytes: (Re w, +-Im w) in FP64, (Re w, +-Im w) and their remainders in complex64
+  // (RotEntry)
   const rsrc_t rots = buffer_rsrc(rot_pm, 2L * s.n_rot * s.nr * 16);
   const rsrc_t lo_f = buffer_rsrc(lo_fwd, sizeof(T) == 4 ? 512L * s.n_blocks : 0);
   const rsrc_t lo_b = buffer_rsrc(lo_bwd, sizeof(T) == 4 ? 512L * s.n_blocks : 0);
@@ -818,8 +819,11 @@ __global__ __launch_bounds__(64 * WAVES, sizeof(T) == 8 ? 1 : 2) void interactio
     for (int v = 0; v < 4; ++v) {
       const T mine = acc[v];
       const T other = swap_adjacent_lanes(mine);
-      T out = w[v].x * mine + w[v].y * other;
-      if constexpr (sizeof(T) == 4) out = fmaf(out, w[v].eps, out);  // the factor's modulus correction
+      T out;
+      if constexpr (sizeof(T) == 4)  // remainder products first, the nearest values fused on top (mul_hilo)
+        out = fmaf(w[v].x, mine, fmaf(w[v].y, other, fmaf(w[v].lo_x, mine, w[v].lo_y * other)));
+      else
+        out = w[v].x * mine + w[v].y * other;
       if (live && M::row(lane, v) < nb) buffer_store<T>(out, site, st_lane[v], sbase);
     }
   }

@@ -1,12 +1,18 @@
-// Rounding of complex factors to complex64 that keeps their modulus.
+// Complex64 forms of the constants and factors of the split-operator step.
 //
-// A factor of the split-operator step that is applied every time step (a kinetic or rotational phase, exp(i A),
-// an FFT twiddle) multiplies the wave by the same complex64 value thousands of times. Rounding its two components
-// to the nearest floats independently changes its modulus by up to about 6e-8 relative, always in the same
-// direction for that factor, so the error grows linearly with the number of steps instead of as a random walk
-// (2e4 steps: about 1e-3). Here the two floats are chosen among the neighbours of the nearest ones so that
-// x^2 + y^2 matches the FP64 modulus as closely as possible, at a phase error of a few units in the last place
-// (a phase error is unitary: it does not change the norm, and it moves an energy by about 1e-7 / dt).
+// A factor that multiplies the wave at every time step (a kinetic or rotational phase, exp(i A), an FFT twiddle)
+// is applied thousands of times with the same complex64 value, so its rounding errors add up linearly with the
+// number of steps instead of as a random walk. Two kinds of error matter:
+//   modulus: rounding the two components to the nearest floats changes |z| by up to about 6e-8 relative, always in
+//     the same direction for that entry; a table of such factors carries a per-entry correction
+//     eps = |z| / |fl(z)| - 1 in FP32 that the kernels apply to the product (mul_factor, twiddle);
+//   phase: the nearest floats are also the smallest phase error (half a unit in the last place per component);
+//     a phase error is an energy shift of about 1e-7 / dt that accumulates coherently. Choosing other nearby floats
+//     whose squared sum matches |z|^2 better trades a few units of phase for the modulus: on a 3000-step
+//     CRP + SCM run that moved the reaction probabilities by 1.4e-4, against 1.4e-5 for nearest rounding with
+//     the eps correction. So the factor tables are rounded to nearest, and only the fixed twiddles inside the
+//     register DFTs (unit_f32) trade phase for modulus: a DFT whose twiddles have unit modulus is exactly unitary
+//     and its inverse applies the conjugates, so their phase errors cancel over the forward and backward pair.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -23,56 +29,13 @@ __host__ __device__ constexpr float float_step(float f, int k) {
   return std::bit_cast<float>(f > 0.0f ? b + k : b - k);
 }
 
-// (fx, fy) close to (x, y) with fx^2 + fy^2 as close as the search finds to x^2 + y^2 (the sum of two exactly
-// representable squares, evaluated in FP64). One coordinate is stepped up to 8 values around its nearest float; the
-// other is then solved for and rounded, and its two neighbours are tried; both roles are tried. Candidates farther
-// than 1e-6 |z| from (x, y) are not taken. Measured over uniformly random phases: | |z'|^2 / |z|^2 - 1 | has median
-// 6e-10 and 90th percentile 4e-9 (plain rounding: 3e-8 and 5e-8); phases within a few tenths of a degree of a
-// multiple of 45 degrees (about 2 %) keep errors of plain size, since near them no nearby pair does better.
-__host__ __device__ inline void round_keep_modulus(double x, double y, float& fx, float& fy) {
-  const float x0 = static_cast<float>(x), y0 = static_cast<float>(y);
-  fx = x0;
-  fy = y0;
-  const double m2 = x * x + y * y;
-  if (!(m2 > 0.0) || !(m2 < 1e300)) return;
-  const double tol = 1e-6 * sqrt(m2);
-  double best = fabs(static_cast<double>(x0) * x0 + static_cast<double>(y0) * y0 - m2);
-  auto consider = [&](float cx, float cy) {
-    if (fabs(cx - x) + fabs(cy - y) > tol) return;
-    const double e = fabs(static_cast<double>(cx) * cx + static_cast<double>(cy) * cy - m2);
-    if (e < best) {
-      best = e;
-      fx = cx;
-      fy = cy;
-    }
-  };
-  for (int role = 0; role < 2; ++role) {
-    const float a0 = role == 0 ? x0 : y0;          // stepped coordinate
-    const double other = role == 0 ? y : x;        // solved coordinate (its sign is kept)
-    if (a0 == 0.0f) continue;
-    for (int i = -8; i <= 8; ++i) {
-      const float a = float_step(a0, i);
-      const double r = m2 - static_cast<double>(a) * a;
-      if (r < 0.0) continue;
-      const float b0 = static_cast<float>(copysign(sqrt(r), other));
-      if (b0 == 0.0f) continue;
-      for (int j = -1; j <= 1; ++j) {
-        const float b = float_step(b0, j);
-        if (role == 0)
-          consider(a, b);
-        else
-          consider(b, a);
-      }
-    }
-  }
-}
-
-// round_keep_modulus over a table of n complex values (interleaved re, im; times a real factor, in FP64 first),
-// one value per thread.
-__global__ inline void round_keep_modulus_kernel(const double* in, float* out, long n, double factor) {
+// A table of n complex values (interleaved re, im) times a real factor (in FP64 first), rounded to the nearest
+// complex64 values; one value per thread.
+__global__ inline void round_to_complex64_kernel(const double* in, float* out, long n, double factor) {
   const long i = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  round_keep_modulus(in[2 * i] * factor, in[2 * i + 1] * factor, out[2 * i], out[2 * i + 1]);
+  out[2 * i] = static_cast<float>(in[2 * i] * factor);
+  out[2 * i + 1] = static_cast<float>(in[2 * i + 1] * factor);
 }
 
 // A unit complex constant (c, s) as two floats whose squares sum to 1 as closely as a search over a few units in

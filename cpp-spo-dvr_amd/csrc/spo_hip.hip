@@ -98,16 +98,14 @@ Buffer upload(const void* host, std::size_t bytes) {
   return b;
 }
 
-// A complex factor in the storage type: FP64 as is; complex64 rounded so that its modulus is kept (the factors
-// multiply the wave at every step, and plain rounding would change their modulus by the same ~6e-8 every time).
+// A complex factor in the storage type: FP64 as is; complex64 rounded to nearest (tables applied at every step
+// also carry the remainder, factor_lo, or for twiddles a modulus correction, DevicePlan::eps).
 template <typename T>
 cx<T> to_storage(double x, double y) {
   if constexpr (sizeof(T) == 8) {
     return {x, y};
   } else {
-    cx<float> r;
-    spo_dev::round_keep_modulus(x, y, r.x, r.y);
-    return r;
+    return {static_cast<float>(x), static_cast<float>(y)};
   }
 }
 
@@ -118,8 +116,8 @@ Buffer upload_complex(const double* p, std::size_t n, double factor = 1.0) {
   if (sizeof(T) == 4 && n > (std::size_t(1) << 18)) {
     Buffer full = upload(p, n * sizeof(cx<double>));
     Buffer out(n * sizeof(cx<T>));
-    hipLaunchKernelGGL(spo_dev::round_keep_modulus_kernel, dim3(static_cast<unsigned>((n + 255) / 256)), dim3(256), 0,
-                       nullptr, full.as<const double>(), out.as<float>(), static_cast<long>(n), factor);
+    hipLaunchKernelGGL(spo_dev::round_to_complex64_kernel, dim3(static_cast<unsigned>((n + 255) / 256)), dim3(256),
+                       0, nullptr, full.as<const double>(), out.as<float>(), static_cast<long>(n), factor);
     HIP_CHECK(hipGetLastError());
     HIP_CHECK(hipStreamSynchronize(nullptr));
     return out;
@@ -128,21 +126,20 @@ Buffer upload_complex(const double* p, std::size_t n, double factor = 1.0) {
   for (std::size_t i = 0; i < n; ++i) v[i] = to_storage<T>(p[2 * i] * factor, p[2 * i + 1] * factor);
   return upload(v.data(), n * sizeof(cx<T>));
 }
-// complex64: per entry of a factor table uploaded by upload_complex, |z| / |stored z| - 1 in FP32 (z the FP64 value
-// times factor); the kernels multiply by the stored value and add eps times the product (mul_factor). Empty in FP64.
+// complex64: per entry of a factor table uploaded by upload_complex, the FP32 remainder z - stored z (z the FP64
+// value times factor); the kernels multiply by stored z + remainder (mul_hilo). Empty in FP64.
 template <typename T>
-Buffer factor_eps(const double* p, std::size_t n, double factor = 1.0) {
+Buffer factor_lo(const double* p, std::size_t n, double factor = 1.0) {
   if constexpr (sizeof(T) == 8) {
     return Buffer();
   } else {
-    std::vector<float> e(n);
+    std::vector<cx<float>> lo(n);
     for (std::size_t i = 0; i < n; ++i) {
       const double x = p[2 * i] * factor, y = p[2 * i + 1] * factor;
       const cx<float> w = to_storage<float>(x, y);
-      const double m2 = static_cast<double>(w.x) * w.x + static_cast<double>(w.y) * w.y;
-      e[i] = m2 > 0.0 ? static_cast<float>(std::sqrt((x * x + y * y) / m2) - 1.0) : 0.0f;
+      lo[i] = {static_cast<float>(x - static_cast<double>(w.x)), static_cast<float>(y - static_cast<double>(w.y))};
     }
-    return upload(e.data(), n * sizeof(float));
+    return upload(lo.data(), n * sizeof(cx<float>));
   }
 }
 
@@ -188,6 +185,7 @@ struct DevicePlan {
     plan.n_stages = static_cast<int>(r.size());
     for (std::size_t i = 0; i < r.size(); ++i) plan.radix[i] = r[i];
     plan.table = table.as<void>();
+    plan.eps = eps.as<void>();
   }
   const cx<T>* tw() const { return table.as<cx<T>>(); }
   // 1 / |w_m| - 1 of the stored twiddles (complex64 kernels only)
@@ -444,7 +442,7 @@ void spo_hip_backend::lines_d(cx<double>* data, spo_dev::Lines L, const DevicePl
   const unsigned grid = blocks_for(L.count, L.lines_per_block);
   timed("specular_lines_fft", [&] {
     hipLaunchKernelGGL(spo_dev::lines_fft_kernel<double>, dim3(grid), dim3(kThreads), lds, side, data, L, p.plan,
-                       sign, mode, factor, f_line, f_elem);
+                       sign, mode, factor, static_cast<const cx<double>*>(nullptr), f_line, f_elem);
   }, side);
 }
 
@@ -504,8 +502,9 @@ namespace {
 template <typename T>
 struct State final : spo_hip_backend {
   Buffer psi, zr_full, zr_half, xy_full, xy_half, rot_half, rot_half_t, before, gas_step, z_opt;
-  // complex64: the modulus corrections of the factor tables (factor_eps); empty in FP64
-  Buffer zr_full_eps, zr_half_eps, xy_full_eps, xy_half_eps, before_eps, gas_step_eps, shift_f_eps, shift_b_eps;
+  // complex64: the FP32 remainders of the factor tables (factor_lo); empty in FP64
+  Buffer zr_full_lo, zr_half_lo, xy_full_lo, xy_half_lo, before_lo, gas_step_lo, shift_f_lo, shift_b_lo, rot_lo,
+      rot_lo_t;
   Buffer shift_f, shift_b, s2sc, legendre, block_first, block_size, block_offset, rot_block, rep_of, op_of, phi_index;
   Buffer exp_table, legendre_t;
   Buffer frag_fwd, frag_bwd, fwd_offset, bwd_offset, site_order, rot_pm, lo_fwd, lo_bwd, hi_fwd, hi_bwd;
@@ -536,18 +535,18 @@ struct State final : spo_hip_backend {
   void bk_scattering() override;
   void energy_resolved_scattering() override;
   void lines(cx<T>* data, spo_dev::Lines L, const DevicePlan<T>& p, int sign, int mode, const cx<T>* factor,
-             int f_line, int f_elem);
+             const cx<T>* factor_lo, int f_line, int f_elem);
 };
 
 template <typename T>
 void State<T>::lines(cx<T>* data, spo_dev::Lines L, const DevicePlan<T>& p, int sign, int mode, const cx<T>* factor,
-                     int f_line, int f_elem) {
+                     const cx<T>* factor_lo, int f_line, int f_elem) {
   L.lines_per_block = std::max(1, std::min(64, 65536 / (L.n * static_cast<int>(sizeof(cx<T>)))));
   const std::size_t lds = std::size_t(L.lines_per_block) * L.n * sizeof(cx<T>);
   const unsigned grid = blocks_for(L.count, L.lines_per_block);
   timed("zr_lines_fft", [&] {
     hipLaunchKernelGGL(spo_dev::lines_fft_kernel<T>, dim3(grid), dim3(kThreads), lds, stream, data, L, p.plan, sign,
-                       mode, factor, f_line, f_elem);
+                       mode, factor, factor_lo, f_line, f_elem);
   });
 }
 
@@ -559,9 +558,9 @@ void State<T>::run_zr(const Variant& v, bool half) {
   const cx<T>* zr = (half ? zr_half : zr_full).as<cx<T>>();
   const T* ez = plan_z->tw_eps();
   const T* er = plan_r->tw_eps();
-  const T* fe = (half ? zr_half_eps : zr_full_eps).as<T>();
+  const cx<T>* fl = (half ? zr_half_lo : zr_full_lo).as<cx<T>>();
   void* args[] = {&ps, const_cast<cx<T>**>(&tz), const_cast<cx<T>**>(&tr), const_cast<cx<T>**>(&zr),
-                  const_cast<T**>(&ez), const_cast<T**>(&er), const_cast<T**>(&fe)};
+                  const_cast<T**>(&ez), const_cast<T**>(&er), const_cast<cx<T>**>(&fl)};
   launch(v.fn, s.n_rot * s.n_xy, v.threads, v.lds, stream, args);
 }
 
@@ -577,11 +576,11 @@ void State<T>::run_xy(const Variant& v, bool half, bool kinetic_only) {
   const T* zo = z_opt.as<T>();
   const cx<T>* bf = before.as<cx<T>>();
   const cx<T>* gs = gas_step.as<cx<T>>();
-  spo_dev::XyEps<T> eps{(half ? xy_half_eps : xy_full_eps).as<T>(), shift_f_eps.as<T>(), shift_b_eps.as<T>(),
-                         before_eps.as<T>(), gas_step_eps.as<T>()};
+  spo_dev::XyLo<T> lo{(half ? xy_half_lo : xy_full_lo).as<cx<T>>(), shift_f_lo.as<cx<T>>(),
+                      shift_b_lo.as<cx<T>>(), before_lo.as<cx<T>>(), gas_step_lo.as<cx<T>>()};
   void* args[] = {&ps, &s, const_cast<cx<T>**>(&factor), const_cast<cx<T>**>(&sf), const_cast<cx<T>**>(&sb),
                   &needs_shift, &apply_potential, const_cast<T**>(&zo), const_cast<cx<T>**>(&bf),
-                  const_cast<cx<T>**>(&gs), &eps};
+                  const_cast<cx<T>**>(&gs), &lo};
   launch(v.fn, tiles * s.n_rot, v.threads, v.lds, stream, args);
 }
 
@@ -596,6 +595,7 @@ void State<T>::run_interaction_mfma(const Variant& v) {
   const cx<double>* sp = spec.as<cx<double>>();
   const cx<T>* et = exp_table.as<cx<T>>();
   const cx<T>* rh = rot_half_t.as<cx<T>>();
+  const cx<T>* rl = rot_lo_t.as<cx<T>>();
   const cx<T>* ssc = s2sc.as<cx<T>>();
   const cx<T>* tp = plan_phi->tw();
   const T* ep = plan_phi->tw_eps();
@@ -620,7 +620,8 @@ void State<T>::run_interaction_mfma(const Variant& v) {
   }
   void* args[] = {&ps, const_cast<cx<double>**>(&sp), &s, const_cast<cx<T>**>(&tp), &at, &mt,
                   const_cast<T**>(&ff), const_cast<T**>(&fb), &mj, const_cast<cx<T>**>(&et),
-                  const_cast<cx<T>**>(&rh), const_cast<cx<T>**>(&ssc), const_cast<T**>(&ep)};
+                  const_cast<cx<T>**>(&rh), const_cast<cx<T>**>(&ssc), const_cast<T**>(&ep),
+                  const_cast<cx<T>**>(&rl)};
   launch(v.fn, grid, v.threads, v.lds, stream, args);
 }
 
@@ -630,7 +631,8 @@ void State<T>::run_interaction_mfma(const Variant& v) {
 // repeated runs bitwise identical. SPO_HIP_TUNE=auto makes the first backend of a process with a given shape and
 // precision time every variant on the (still zero) scattering wave, one warm-up and two timed launches each, and
 // keep the fastest for the process; SPO_HIP_TUNE="zr=384,xy=8,interaction=16" forces variants by thread count
-// (zr), izr per tile (xy) or wavefronts (interaction).
+// (zr), izr per tile (xy) or wavefronts (interaction), and "interaction=#2" the variant at that place in the
+// family's list (variants of one size that differ in their FFT radices).
 std::map<std::string, int>& tune_cache() {
   static std::map<std::string, int> cache;
   return cache;
@@ -645,7 +647,15 @@ int forced_variant(const std::vector<Variant>& v, const char* family, bool by_th
   const std::string key = std::string(family) + "=";
   const std::size_t at = spec.find(key);
   if (at == std::string::npos) return 0;
-  const int want = std::atoi(spec.c_str() + at + key.size());
+  const char* value = spec.c_str() + at + key.size();
+  if (*value == '#') {  // "family=#k": the k-th variant of the family's list (0 = the default)
+    const int k = std::atoi(value + 1);
+    if (k < 0 || k >= static_cast<int>(v.size()))
+      throw Status(SPO_HIP_E_ARGUMENT, std::string("SPO_HIP_TUNE: ") + family + " has " + std::to_string(v.size()) +
+                                           " variants, no #" + std::to_string(k));
+    return k;
+  }
+  const int want = std::atoi(value);
   for (std::size_t i = 0; i < v.size(); ++i)
     if ((by_threads ? v[i].threads : v[i].param) == want) return static_cast<int>(i);
   throw Status(SPO_HIP_E_ARGUMENT, std::string("SPO_HIP_TUNE: no ") + family + " variant " + std::to_string(want));
@@ -705,20 +715,21 @@ void State<T>::kinetic_and_potential(bool half, bool kinetic_only) {
   cx<T>* ps = psi.as<cx<T>>();
   // T_Zr.
   const cx<T>* zr = (half ? zr_half : zr_full).as<cx<T>>();
+  const cx<T>* zr_lo = (half ? zr_half_lo : zr_full_lo).as<cx<T>>();
   const int n_planes = s.n_rot * s.n_xy;
   if (!zr_variants.empty()) {
     timed("zr_plane", [&] { run_zr(zr_variants[zr_pick], half); });
   } else if (zr_fused) {
     timed("zr_plane_generic", [&] {
       hipLaunchKernelGGL(spo_dev::zr_plane_kernel<T>, dim3(n_planes), dim3(kThreads), std::size_t(s.n_zr) * sizeof(cx<T>),
-                         stream, ps, s, plan_z->plan, plan_r->plan, zr);
+                         stream, ps, s, plan_z->plan, plan_r->plan, zr, zr_lo);
     });
   } else {
     spo_dev::Lines r_lines{s.nr, s.nz * n_planes, s.nz, s.nr, s.n_zr, 1, 1};
     spo_dev::Lines z_lines{s.nz, s.nr * n_planes, s.nr, 1, s.n_zr, s.nr, 1};
-    lines(ps, r_lines, *plan_r, -1, 0, nullptr, 0, 0);
-    lines(ps, z_lines, *plan_z, -1, 1, zr, 1, s.nr);
-    lines(ps, r_lines, *plan_r, +1, 0, nullptr, 0, 0);
+    lines(ps, r_lines, *plan_r, -1, 0, nullptr, nullptr, 0, 0);
+    lines(ps, z_lines, *plan_z, -1, 1, zr, zr_lo, 1, s.nr);
+    lines(ps, r_lines, *plan_r, +1, 0, nullptr, nullptr, 0, 0);
   }
   // T_xy and the diagonal part of V.
   if (!xy_variants.empty()) {
@@ -727,10 +738,12 @@ void State<T>::kinetic_and_potential(bool half, bool kinetic_only) {
     const int tiles = (s.n_zr + xy_c - 1) / xy_c;
     const cx<T>* factor = (half ? xy_half : xy_full).as<cx<T>>();
     const T* zo = z_opt.as<T>();
+    const spo_dev::XyLo<T> lo{(half ? xy_half_lo : xy_full_lo).as<cx<T>>(), shift_f_lo.as<cx<T>>(),
+                              shift_b_lo.as<cx<T>>(), before_lo.as<cx<T>>(), gas_step_lo.as<cx<T>>()};
     timed("xy_tile_generic", [&] {
       hipLaunchKernelGGL(spo_dev::xy_kernel<T>, dim3(tiles * s.n_rot), dim3(kThreads), xy_lds, stream, ps, s, xy_c,
                          plan_x->plan, plan_y->plan, factor, shift_f.as<cx<T>>(), shift_b.as<cx<T>>(),
-                         shape.needs_shift, kinetic_only ? 0 : 1, zo, before.as<cx<T>>(), gas_step.as<cx<T>>());
+                         shape.needs_shift, kinetic_only ? 0 : 1, zo, before.as<cx<T>>(), gas_step.as<cx<T>>(), lo);
     });
   }
   // The specular step (on `side`) is complete before the specular wave is read or written on `stream` again.
@@ -757,14 +770,16 @@ void State<T>::kinetic_and_potential(bool half, bool kinetic_only) {
     const T* lt = legendre_t.as<T>();
     int c = v_c;
     int ls = legendre_size;
+    const cx<T>* rl = rot_lo.as<cx<T>>();
+    const T* ep = plan_phi->tw_eps();
     void* args[] = {&ps, const_cast<cx<double>**>(&sp), &s, &c, const_cast<cx<T>**>(&tp), &at,
                     const_cast<cx<T>**>(&et), const_cast<cx<T>**>(&rh), const_cast<cx<T>**>(&ssc),
-                    const_cast<T**>(&lt), &ls};
+                    const_cast<T**>(&lt), &ls, const_cast<cx<T>**>(&rl), const_cast<T**>(&ep)};
     timed("interaction", [&] { launch(v_fn, tiles * s.n_xy, kThreads, lds, stream, args); });
   } else {
     timed("interaction_generic", [&] {
       hipLaunchKernelGGL(spo_dev::interaction_kernel<T>, dim3(tiles * s.n_xy), dim3(kThreads), lds, stream, ps, sp, s,
-                         v_c, plan_phi->plan, at, et, rh, ssc);
+                         v_c, plan_phi->plan, at, et, rh, ssc, rot_lo.as<cx<T>>());
     });
   }
   timed("projection", [&] {
@@ -947,20 +962,21 @@ std::unique_ptr<spo_hip_backend> create_state(const spo_hip_shape* shape, const 
   {
     const auto zf = fold(t->zr_full, s.n_zr, zr_f, zr_b), zh = fold(t->zr_half, s.n_zr, zr_f, zr_b);
     b->zr_full = upload_complex<T>(zf.data(), s.n_zr);
-    b->zr_full_eps = factor_eps<T>(zf.data(), s.n_zr);
+    b->zr_full_lo = factor_lo<T>(zf.data(), s.n_zr);
     b->zr_half = upload_complex<T>(zh.data(), s.n_zr);
-    b->zr_half_eps = factor_eps<T>(zh.data(), s.n_zr);
+    b->zr_half_lo = factor_lo<T>(zh.data(), s.n_zr);
     const auto sf = fold(t->specular_full, std::size_t(s.nr) * s.nzsp, zr_f, sp_b);
     const auto sh = fold(t->specular_half, std::size_t(s.nr) * s.nzsp, zr_f, sp_b);
     b->spec_full = upload(sf.data(), sf.size() * sizeof(double));
     b->spec_half = upload(sh.data(), sh.size() * sizeof(double));
     const auto xf = fold(t->xy_full, s.n_xy, xy_f, xy_b), xh = fold(t->xy_half, s.n_xy, xy_f, xy_b);
     b->xy_full = upload_complex<T>(xf.data(), s.n_xy);
-    b->xy_full_eps = factor_eps<T>(xf.data(), s.n_xy);
+    b->xy_full_lo = factor_lo<T>(xf.data(), s.n_xy);
     b->xy_half = upload_complex<T>(xh.data(), s.n_xy);
-    b->xy_half_eps = factor_eps<T>(xh.data(), s.n_xy);
+    b->xy_half_lo = factor_lo<T>(xh.data(), s.n_xy);
   }
   b->rot_half = upload_complex<T>(t->rotation_half, std::size_t(s.n_rot) * s.nr);
+  b->rot_lo = factor_lo<T>(t->rotation_half, std::size_t(s.n_rot) * s.nr);
   {
     // (irot, ir) -> (ir, irot), ir fastest, for the MFMA interaction kernel
     std::vector<double> rt(2 * std::size_t(s.n_rot) * s.nr);
@@ -969,19 +985,20 @@ std::unique_ptr<spo_hip_backend> create_state(const spo_hip_shape* shape, const 
         for (int c = 0; c < 2; ++c)
           rt[2 * (ir + std::size_t(s.nr) * irot) + c] = t->rotation_half[2 * (irot + std::size_t(s.n_rot) * ir) + c];
     b->rot_half_t = upload_complex<T>(rt.data(), rt.size() / 2);
+    b->rot_lo_t = factor_lo<T>(rt.data(), rt.size() / 2);
   }
   b->before = upload_complex<T>(t->before_interaction, std::size_t(s.n_rot) * s.nr);
   b->gas_step = upload_complex<T>(t->gas_phase_step, std::size_t(s.n_rot) * s.nr);
-  b->before_eps = factor_eps<T>(t->before_interaction, std::size_t(s.n_rot) * s.nr);
-  b->gas_step_eps = factor_eps<T>(t->gas_phase_step, std::size_t(s.n_rot) * s.nr);
+  b->before_lo = factor_lo<T>(t->before_interaction, std::size_t(s.n_rot) * s.nr);
+  b->gas_step_lo = factor_lo<T>(t->gas_phase_step, std::size_t(s.n_rot) * s.nr);
   b->spec_gas = upload(t->specular_gas_phase, std::size_t(s.nr) * c16);
   b->z_opt = upload_real<T>(t->z_optical, s.nz);
   b->r_opt_d = upload(t->r_optical, std::size_t(s.nr) * sizeof(double));
   b->z_spec_opt = upload(t->z_specular_optical, std::size_t(s.nzsp) * sizeof(double));
   b->shift_f = upload_complex<T>(t->shift_forward, s.n_xy);
   b->shift_b = upload_complex<T>(t->shift_backward, s.n_xy);
-  b->shift_f_eps = factor_eps<T>(t->shift_forward, s.n_xy);
-  b->shift_b_eps = factor_eps<T>(t->shift_backward, s.n_xy);
+  b->shift_f_lo = factor_lo<T>(t->shift_forward, s.n_xy);
+  b->shift_b_lo = factor_lo<T>(t->shift_backward, s.n_xy);
   b->shift_f_d = upload(t->shift_forward, std::size_t(s.n_xy) * c16);
   b->s2sc = upload_complex<T>(t->specular_to_scattering, s.n_xy);
   b->s2sc_d = upload(t->specular_to_scattering, std::size_t(s.n_xy) * c16);
@@ -1177,10 +1194,12 @@ std::unique_ptr<spo_hip_backend> create_state(const spo_hip_shape* shape, const 
               pm[e0 + 1].x = w.x;
               pm[e0 + 1].y = w.y;
               if constexpr (sizeof(T) == 4) {
-                const double m2 = static_cast<double>(w.x) * w.x + static_cast<double>(w.y) * w.y;
-                const float e = static_cast<float>(std::sqrt((x * x + y * y) / m2) - 1.0);
-                pm[e0].eps = e;
-                pm[e0 + 1].eps = e;
+                const float lo_x = static_cast<float>(x - static_cast<double>(w.x));
+                const float lo_y = static_cast<float>(y - static_cast<double>(w.y));
+                pm[e0].lo_x = lo_x;
+                pm[e0].lo_y = -lo_y;
+                pm[e0 + 1].lo_x = lo_x;
+                pm[e0 + 1].lo_y = lo_y;
               }
             }
           b->rot_pm = upload(pm.data(), pm.size() * sizeof(spo_dev::RotEntry<T>));
@@ -1864,7 +1883,6 @@ int spo_hip_pes_values(int pes, long n, const double* points, double* values) {
 }
 
 }  // extern "C"
-
 
 namespace {
 

@@ -152,7 +152,8 @@ def test_two_runs_are_bitwise_identical(precision):
 
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
-@pytest.mark.parametrize("case,settings", [("dz_angular", ["interaction=4", "interaction=16", "xy=32", "xy=64"]),
+@pytest.mark.parametrize("case,settings", [("dz_angular", ["interaction=16", "interaction=#2", "interaction=#3",
+                                                          "xy=32", "xy=64"]),
                                            ("hardwall_short", ["zr=256", "zr=512", "xy=32", "xy=64"]),
                                            ("dz", ["xy=8", "xy=16", "xy=32", "zr=192", "zr=512"])])
 def test_launch_variants_match_the_default(case, settings, precision):
