@@ -16,6 +16,7 @@
 #include <cstring>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <optional>
 #include <stdexcept>
 #include <string>
@@ -1879,6 +1880,59 @@ int spo_hip_pes_values(int pes, long n, const double* points, double* values) {
                        p.as<double>(), out.as<double>());
     HIP_CHECK(hipGetLastError());
     HIP_CHECK(hipMemcpy(values, out.as<void>(), std::size_t(n) * sizeof(double), hipMemcpyDeviceToHost));
+  });
+}
+
+}  // extern "C"
+
+// Scratch of spo_hip_pes_values_static, one copy per device for the life of the process: the points (6 doubles
+// each) of up to kPesStaticMax points, then their values. 3.5 MiB.
+constexpr long kPesStaticMax = 1L << 16;
+static __device__ __align__(256) double scratch[7 * kPesStaticMax];
+
+namespace {
+
+// The device address of `scratch` on each device, looked up once per device with that device current. In host
+// code the name `scratch` is only a stand-in for the symbol; kernels and copies get this address instead.
+constexpr int kMaxDevices = 64;
+std::mutex scratch_mutex;
+double* scratch_address[kMaxDevices] = {};
+
+double* scratch_on_current_device() {
+  int dev = 0;
+  HIP_CHECK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) throw Status(SPO_HIP_E_NOT_SUPPORTED, "device index beyond the scratch table");
+  std::lock_guard<std::mutex> lock(scratch_mutex);
+  if (!scratch_address[dev]) {
+    void* p = nullptr;
+    HIP_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    scratch_address[dev] = static_cast<double*>(p);
+  }
+  return scratch_address[dev];
+}
+
+}  // namespace
+
+extern "C" {
+
+int spo_hip_pes_values_static(int pes, long n, const double* points, double* values) {
+  return guarded([&] {
+    if (pes != SPO_HIP_PES_HARDWALL && pes != SPO_HIP_PES_DAILIGHT)
+      throw Status(SPO_HIP_E_ARGUMENT, "spo_hip_pes_values_static: unknown surface " + std::to_string(pes));
+    if (n < 0 || (n > 0 && (!points || !values)))
+      throw Status(SPO_HIP_E_ARGUMENT, "spo_hip_pes_values_static: bad arrays");
+    if (n > kPesStaticMax)
+      throw Status(SPO_HIP_E_ARGUMENT, "spo_hip_pes_values_static: " + std::to_string(n) + " points, at most " +
+                                           std::to_string(kPesStaticMax) + " per call");
+    if (n == 0) return;
+    require_device();
+    double* p = scratch_on_current_device();
+    double* out = p + 6 * kPesStaticMax;
+    // Every value of the call is written by the kernel, so nothing of an earlier call's contents is read.
+    HIP_CHECK(hipMemcpy(p, points, std::size_t(n) * 6 * sizeof(double), hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(spo_dev::pes_points_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, nullptr, pes, n, p, out);
+    HIP_CHECK(hipGetLastError());
+    HIP_CHECK(hipMemcpy(values, out, std::size_t(n) * sizeof(double), hipMemcpyDeviceToHost));
   });
 }
 

@@ -193,6 +193,13 @@ enum { SPO_HIP_PES_HARDWALL = 1, SPO_HIP_PES_DAILIGHT = 2, SPO_HIP_PES_CRP = 3 }
  * Pes::value (src/potential/pes.hpp) for these surfaces. Needs no backend. */
 int spo_hip_pes_values(int pes, long n, const double* points, double* values);
 
+/* spo_hip_pes_values without any device allocation, for callers that cannot allocate device memory (tight
+ * serving loops, code that forbids hipMalloc). No counterpart in the reference. The points and values pass
+ * through a fixed scratch array in device memory (3.5 MiB per device, held until the process ends), so n is at
+ * most 65536 per call (larger n: SPO_HIP_E_ARGUMENT). All calls on one device share that array: calls that
+ * overlap on one device (from several host threads) race. Same results as spo_hip_pes_values. */
+int spo_hip_pes_values_static(int pes, long n, const double* points, double* values);
+
 /* The potential grid the propagation stores (src/potential/potential_grid.cpp:65-127): V in hartree at the
  * symmetry representatives, values[iphi + n_phi (itheta + n_theta (irep + n_representatives izr))] for
  * izr = ir + nr iz < nz_potential nr, with the cosine switch into gas_phase between z_min_switch and
