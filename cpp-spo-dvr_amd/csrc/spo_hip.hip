@@ -1104,12 +1104,13 @@ std::unique_ptr<spo_hip_backend> create_state(const spo_hip_shape* shape, const 
   if (b->v_lds > lds_limit) throw Status(SPO_HIP_E_NOT_SUPPORTED, "an angular column does not fit in LDS");
   set_lds(b->v_fn ? b->v_fn : reinterpret_cast<const void*>(spo_dev::interaction_kernel<T>), b->v_lds);
   // Interaction with the theta contractions on MFMA: Legendre fragments in lane order, XCD-grouped site order.
-  // SPO_HIP_INTERACTION selects a variant for comparisons: "buf" (default: the buffer-addressed MFMA kernel where
-  // its shape conditions hold, else "mfma"), "mfma" (MFMA contractions, fused register phi step), "mfma_staged"
-  // (MFMA contractions, staged LDS phi FFTs), "lds" (contractions on the vector units).
+  // SPO_HIP_INTERACTION selects a variant for comparisons: "buf" (the buffer-addressed MFMA kernel where its shape
+  // conditions hold, else "mfma"; the complex64 default), "mfma" (MFMA contractions, fused register phi step; the
+  // FP64 default: 4.33 against 4.89 ms per launch at the DZ workload on MI355X), "mfma_staged" (MFMA contractions,
+  // staged LDS phi FFTs), "lds" (contractions on the vector units).
   {
     const char* env = std::getenv("SPO_HIP_INTERACTION");
-    const std::string choice = env ? env : "buf";
+    const std::string choice = env ? env : (sizeof(T) == 8 ? "mfma" : "buf");
     if (choice != "buf" && choice != "mfma" && choice != "mfma_staged" && choice != "lds")
       throw Status(SPO_HIP_E_ARGUMENT, "SPO_HIP_INTERACTION must be buf, mfma, mfma_staged or lds");
     const bool fused = (choice == "mfma" || choice == "buf") && s.n_theta <= 16;
