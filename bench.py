@@ -203,7 +203,8 @@ def measure(path: str, workload: str, precision: str, steps: int, warmup: int) -
 def measure_batch(steps: int, warmup: int) -> dict:
     """The 12-run batch on the GPU in graph mode (SPO_HIP_GRAPH=1: every analysis step of the batch is one launch of
     a hipGraph captured over all runs' streams); throughput summed over the runs. Each run starts from its own
-    synthetic wave packet."""
+    synthetic wave packet. The same number of steps is then timed for each run alone, one run after another (also
+    in graph mode), for the batch's gain over running the members one by one."""
     import spodvr_amd
     from tests import cases
     os.environ["SPO_HIP_GRAPH"] = "1"
@@ -221,15 +222,24 @@ def measure_batch(steps: int, warmup: int) -> dict:
         t0 = time.perf_counter()
         spodvr_amd.run_batch(runs, timed)
         elapsed = time.perf_counter() - t0
+        alone = []
+        for run in runs:
+            spodvr_amd.run_batch([run], 1)  # warm-up of the single-run graph
+            t0 = time.perf_counter()
+            spodvr_amd.run_batch([run], timed)
+            alone.append(time.perf_counter() - t0)
     finally:
         members = [{"name": name, "n_rot": r.info.n_rot, "nr": r.info.nr, "n_theta": r.info.n_theta,
                     "n_phi": r.info.n_phi, "dvr_points": dvr_points(r.info)} for (name, _), r in zip(cases.BATCH_12, runs)]
+        for m, a in zip(members, alone):
+            m["alone_ms_per_step"] = round(a / (timed * spa) * 1e3, 4)
         for r in runs:
             r.close()
         os.environ.pop("SPO_HIP_GRAPH", None)
     n_steps = timed * spa
     return {"value": points * n_steps / elapsed, "ms_per_step": elapsed / n_steps * 1e3, "steps": n_steps,
-            "members": members, "dvr_points": points}
+            "members": members, "dvr_points": points,
+            "one_by_one": {"value": points * n_steps / sum(alone), "ms_per_step": sum(alone) / n_steps * 1e3}}
 
 
 def main() -> int:
@@ -267,6 +277,7 @@ def main() -> int:
             "vs_baseline": None, "dtype": "f32 (complex64 waves, FP64 phases and analysis)", "data": "synthetic",
             "config": {"workload": WORKLOADS["batch12"][1], "precision": "mixed", "runs": len(b["members"]),
                        "dvr_points_summed": b["dvr_points"], "members": b["members"]},
+            "one_by_one": b["one_by_one"], "batch_over_one_by_one": b["value"] / b["one_by_one"]["value"],
             "roofline": None, "cpu_baseline": None}), flush=True)
         return 0
 
