@@ -1123,7 +1123,10 @@ std::unique_ptr<spo_hip_backend> create_state(const spo_hip_shape* shape, const 
     std::vector<Variant> variants;
     for (const Variant& v : interaction_mfma_variants<T>(s.n_phi, s.n_theta, fused))
       if (v.lds <= lds_limit) variants.push_back(v);
-    if (!variants.empty() && choice != "lds") {
+    // phi lengths with a buffer-kernel instantiation only (the reduced-JMax batch members' 28 and 40 points)
+    const bool buf_candidate =
+        choice == "buf" && !interaction_buf_variants<T>(s.n_phi, s.n_theta, s.n_blocks, bf16x3).empty();
+    if ((!variants.empty() || buf_candidate) && choice != "lds") {
       const int n_tt = (s.n_theta + 15) / 16, k_bwd = (s.n_theta + 3) / 4;
       std::vector<int> fo(s.n_blocks), bo(s.n_blocks);
       int nf = 0, nbw = 0, max_nb = 0;
@@ -1249,9 +1252,11 @@ std::unique_ptr<spo_hip_backend> create_state(const spo_hip_shape* shape, const 
           b->v_buf = true;
         }
       }
-      b->v_variants = variants;
-      b->v_c = 8;
-      for (const Variant& v : variants) set_lds(v.fn, v.lds);
+      if (!variants.empty()) {  // else the kernels chosen above (v_fn or the run-time-shaped one) stay
+        b->v_variants = variants;
+        b->v_c = 8;
+        for (const Variant& v : variants) set_lds(v.fn, v.lds);
+      }
     }
   }
   set_lds(reinterpret_cast<const void*>(spo_dev::lines_fft_kernel<T>), 65536);

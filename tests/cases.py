@@ -51,6 +51,14 @@ SMOKE_GENERIC = dict(SMOKE_DAILIGHT, **{
     "dZ": 0.14, "nZ": 72, "nX": 6, "nY": 6, "JMax": 6, "MJMax": 6, "ZOptEnd": 8.94,
 })
 
+# The angular bases of the reduced batch members (JMax 12: 7 theta x 28 phi; JMax 18: 10 theta x 40 phi) on the
+# small grid.
+JMAX12_ANGULAR = dict(SMOKE_DAILIGHT, **{
+    "Title": "small Dai-Light grid with JMax 12",
+    "JMax": 12, "MJMax": 12, "UseSymmetry": True,
+})
+JMAX18_ANGULAR = dict(JMAX12_ANGULAR, **{"Title": "small Dai-Light grid with JMax 18", "JMax": 18, "MJMax": 18})
+
 # The angular basis of the DZ workload (JMax 24, MJMax 23: 323 rotational states, 13 theta x 48 phi, C6v symmetry)
 # on a small spatial grid: the instantiated MFMA interaction shapes at a size the CPU oracle runs in seconds.
 DZ_ANGULAR = dict(SMOKE_DAILIGHT, **{

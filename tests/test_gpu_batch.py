@@ -112,3 +112,41 @@ def test_batch_graph_equals_single_runs():
     finally:
         for r in runs:
             r.close()
+
+
+# Three members of the 12-run DZ batch: a full one and the two reduced-JMax, NR 32 ones (other rotational bases,
+# phi steps of 28 and 40 points, another r grid), as the batch workload runs them (complex64, graph mode).
+DZ_MEMBERS = [cases.BATCH_12[0], cases.BATCH_12[8], cases.BATCH_12[9]]
+
+
+def test_dz_batch_members_graph_equals_single_runs():
+    """Uneven members of the DZ batch workload in one graph-mode batch: each member's BK records and S sums equal
+    those of the member stepped alone (eagerly), bit for bit."""
+    steps = 2
+    single = []
+    for name, params in DZ_MEMBERS:
+        _note("single (eager)", name)
+        run = spodvr_amd.Run(cases.write_case(f"batch_{name}", params), device="hip")
+        try:
+            run.steps(steps)
+            single.append((run.bk_records(), run.s_sums()))
+        finally:
+            run.close()
+    _graph_mode(True)
+    try:
+        _note("graph batch of", [name for name, _ in DZ_MEMBERS])
+        runs = [spodvr_amd.Run(cases.write_case(f"batch_{name}", params), device="hip") for name, params in DZ_MEMBERS]
+    finally:
+        _graph_mode(False)
+    try:
+        _graph_mode(True)
+        try:
+            spodvr_amd.run_batch(runs, steps)
+        finally:
+            _graph_mode(False)
+        for (bk, s), run in zip(single, runs):
+            assert np.array_equal(run.bk_records(), bk)
+            assert np.array_equal(run.s_sums(), s)
+    finally:
+        for r in runs:
+            r.close()

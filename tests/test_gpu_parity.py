@@ -19,6 +19,8 @@ CASE_PARAMS = {
     "smoke": lambda: cases.write_case("smoke_dailight", cases.SMOKE_DAILIGHT),
     "generic": lambda: cases.write_case("smoke_generic", cases.SMOKE_GENERIC),
     "dz_angular": lambda: cases.write_case("dz_angular", cases.DZ_ANGULAR),
+    "jmax12_angular": lambda: cases.write_case("jmax12_angular", cases.JMAX12_ANGULAR),
+    "jmax18_angular": lambda: cases.write_case("jmax18_angular", cases.JMAX18_ANGULAR),
     "hardwall_short": lambda: cases.golden_input("hardwall_short"),
     "cu111_dailight_short": lambda: cases.golden_input("cu111_dailight_short"),
     "cu111_dailight_short_sym": lambda: cases.golden_input("cu111_dailight_short_sym"),
@@ -95,8 +97,8 @@ def test_analysis_step_matches_oracle(case):
     assert relative_difference(cpu.site_column(3), gpu.site_column(3)) < TOL
 
 
-@pytest.mark.parametrize("case", ["smoke", "generic", "dz_angular", "cu111_dailight_short_sym",
-                                  "cu111_dailight_oblique"])
+@pytest.mark.parametrize("case", ["smoke", "generic", "dz_angular", "jmax12_angular", "jmax18_angular",
+                                  "cu111_dailight_short_sym", "cu111_dailight_oblique"])
 def test_mixed_precision_t_block_against_fp64_oracle(case):
     """The complex64 storage mode against the FP64 oracle on one full T block from the same wave packet:
     single-precision rounding only (max |a - b| / max |a| < 1e-5)."""
@@ -113,7 +115,8 @@ def test_mixed_precision_t_block_against_fp64_oracle(case):
     assert ds > 1e-9  # the mode really stores complex64
 
 
-@pytest.mark.parametrize("case", ["smoke", "dz_angular", "cu111_dailight_short_sym", "cu111_dailight_oblique"])
+@pytest.mark.parametrize("case", ["smoke", "dz_angular", "jmax12_angular", "jmax18_angular",
+                                  "cu111_dailight_short_sym", "cu111_dailight_oblique"])
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 @pytest.mark.parametrize("interaction", ["lds", "mfma_staged", "mfma", "buf"])
 def test_interaction_variants_match_oracle(case, precision, interaction):
@@ -131,6 +134,31 @@ def test_interaction_variants_match_oracle(case, precision, interaction):
     print(f"  scattering {ds:.3e} specular {dp:.3e}", flush=True)
     tol = TOL if precision == "fp64" else 1e-5
     assert ds < tol and dp < tol
+
+
+def test_bf16x3_contraction_t_block_against_fp64_oracle():
+    """The split-BF16x3 theta contraction (SPO_HIP_CONTRACTION=bf16x3, complex64, the 48-point phi step of the DZ
+    angular basis) against the FP64 oracle on one full T block: L enters with about 16 significant bits, so the bound
+    is ten times the FP32-MFMA path's."""
+    import os
+    print("case dz_angular seed 61 mixed bf16x3", flush=True)
+    old = os.environ.get("SPO_HIP_CONTRACTION")
+    os.environ["SPO_HIP_CONTRACTION"] = "bf16x3"
+    try:
+        gpu, cpu = _pair("dz_angular", {"Precision": "mixed"})
+    finally:
+        os.environ.pop("SPO_HIP_CONTRACTION", None)
+        if old is not None:
+            os.environ["SPO_HIP_CONTRACTION"] = old
+    _randomise(gpu, cpu, 61)
+    gpu.kinetic_and_potential(False, False)
+    cpu.kinetic_and_potential(False, False)
+    sg, pg = gpu.get_wave()
+    sc, pc = cpu.get_wave()
+    ds, dp = relative_difference(sc, sg), relative_difference(pc, pg)
+    print(f"  scattering {ds:.3e} specular {dp:.3e}", flush=True)
+    assert ds < 1e-4 and dp < 1e-4
+    assert ds > 1e-9
 
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
