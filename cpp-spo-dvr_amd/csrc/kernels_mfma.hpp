@@ -574,22 +574,29 @@ __device__ __forceinline__ void buffer_store(V v, rsrc_t r, unsigned voff, unsig
 // for t = 0 .. 3 (bf16 element t in bits 16 t .. 16 t + 15). The contraction index is relabelled as
 // kappa = 4 (l / 16) + t <-> row 4 t + l / 16 of the FP32 k steps, so B element t is the lane's value of k step t,
 // and the host arranges the fragments of L_lo (512 bytes per mJ block, lane order) the same way.
-__device__ __forceinline__ Mfma<float>::acc_t legendre_low_part(rsrc_t lo, int bb, const float (&b)[4],
-                                                               Mfma<float>::acc_t acc) {
+typedef unsigned lo_frag_t __attribute__((ext_vector_type(2)));
+
+// The lane's L_lo fragment of mJ block bb (four bf16 values).
+__device__ __forceinline__ lo_frag_t legendre_low_fragment(rsrc_t lo, int bb) {
+  return buffer_load<lo_frag_t>(lo, (threadIdx.x % 64) * 8u, static_cast<unsigned>(bb) * 512u);
+}
+
+// acc + L_lo B with the lane's fragment a of L_lo (legendre_low_fragment).
+__device__ __forceinline__ Mfma<float>::acc_t legendre_low_product(lo_frag_t a, const float (&b)[4],
+                                                                  Mfma<float>::acc_t acc) {
   typedef short s4 __attribute__((ext_vector_type(4)));
-  typedef unsigned u2 __attribute__((ext_vector_type(2)));
-  const u2 a = buffer_load<u2>(lo, (threadIdx.x % 64) * 8u, static_cast<unsigned>(bb) * 512u);
   // the top halves of (b0, b1) and (b2, b3): element t = b[t] truncated to bf16
-  const u2 bb16 = {__builtin_amdgcn_perm(__float_as_uint(b[1]), __float_as_uint(b[0]), 0x07060302u),
-                   __builtin_amdgcn_perm(__float_as_uint(b[3]), __float_as_uint(b[2]), 0x07060302u)};
+  const lo_frag_t bb16 = {__builtin_amdgcn_perm(__float_as_uint(b[1]), __float_as_uint(b[0]), 0x07060302u),
+                          __builtin_amdgcn_perm(__float_as_uint(b[3]), __float_as_uint(b[2]), 0x07060302u)};
   return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(__builtin_bit_cast(s4, a), __builtin_bit_cast(s4, bb16), acc, 0,
                                                    0, 0);
 }
 
+
 // The split-BF16x3 variant of the complex64 Legendre contraction (SPO_HIP_CONTRACTION=bf16x3): L and the four B
 // values of the lane are each split into bf16 pieces, L = A_hi + A_lo (host) and b = b_hi + b_lo (here, both
 // rounded to nearest), and A_hi b_hi + A_hi b_lo + A_lo b_hi is accumulated in FP32 by three 16 x 16 x 16 bf16
-// products (smallest first; A_lo b_lo is dropped). The contraction index is relabelled as in legendre_low_part; the
+// products (smallest first; A_lo b_lo is dropped). The contraction index is relabelled as in legendre_low_product; the
 // host arranges the A_hi and A_lo fragments (512 bytes each per mJ block) the same way. Each piece carries 8
 // significant bits, so L takes part with about 16 bits: its relative error, ~2^-17, is ~100 times that of an FP32
 // L, and the same at every step.
@@ -640,8 +647,11 @@ struct alignas(16) RotEntry<float> {
 // complex64 tile of interaction_mfma_kernel; the tile has at least 2 n_phi rows.
 // BF16X3 (complex64 only): the theta contractions as split-BF16x3 products (legendre_bf16x3) instead of FP32 MFMA
 // with the bf16 low part; lo_fwd / lo_bwd then hold the A_lo fragments and hi_fwd / hi_bwd the A_hi ones.
+// Complex64 asks for eight wavefronts per SIMD (at most 64 VGPRs): with the phase-3 fragment prefetch the compiler
+// would otherwise take 65 and fit one workgroup fewer per CU (measured on MI355X at the DZ shape: 3.02 ms per
+// launch against 3.11 without the prefetch and 3.24 with it at six wavefronts per SIMD).
 template <typename T, int NPHI, int WAVES, int R1, int R2, bool BF16X3 = false>
-__global__ __launch_bounds__(64 * WAVES, sizeof(T) == 8 ? 1 : 2) void interaction_buf_kernel(
+__global__ __launch_bounds__(64 * WAVES, sizeof(T) == 8 ? 1 : 8) void interaction_buf_kernel(
     cx<T>* psi, Shape s, const cx<T>* table_phi, AngularTables at, MfmaTables mt, const T* frag_fwd,
     const T* frag_bwd, int frag_fwd_count, int frag_bwd_count, const cx<T>* exp_table, const cx<T>* rot_pm,
     const unsigned* lo_fwd, const unsigned* lo_bwd, const T* eps_phi, const unsigned* hi_fwd,
@@ -742,7 +752,7 @@ __global__ __launch_bounds__(64 * WAVES, sizeof(T) == 8 ? 1 : 2) void interactio
       for (int k = 0; k < 4; ++k)
         if (k < k_steps)
           acc = M::fma(buffer_load<T>(ffwd, lane * sizeof(T), fbase + k * 64u * sizeof(T)), bv[i][k], acc);
-      if constexpr (sizeof(T) == 4) acc = legendre_low_part(lo_f, bb, bv[i], acc);
+      if constexpr (sizeof(T) == 4) acc = legendre_low_product(legendre_low_fragment(lo_f, bb), bv[i], acc);
     }
     unsigned char* d = dvr_b + d_lane + slot * CB;
 #pragma unroll
@@ -790,18 +800,30 @@ __global__ __launch_bounds__(64 * WAVES, sizeof(T) == 8 ? 1 : 2) void interactio
     st_lane[v] = (static_cast<unsigned>(j) * s.n_zr + col) * CB + ri * sizeof(T);
     rot_lane[v] = ((static_cast<unsigned>(j) * s.nr + ir) * 2u + ri) * 16u;
   }
+  // The A fragments, rotation factors (and L_lo fragment) of the wavefront's next block are loaded before this
+  // block's products, so that their latency overlaps them. Past the basis the last block's are loaded (never used).
+  auto load_bwd = [&](int bb, T (&a)[4], lo_frag_t& lo) {
+    const int bc = min(bb, s.n_blocks - 1);
+    const unsigned fbase = static_cast<unsigned>(mt.bwd_offset[bc]) * 64u * sizeof(T);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) a[k] = k < k_bwd ? buffer_load<T>(fbwd, lane * sizeof(T), fbase + k * 64u * sizeof(T)) : T(0);
+    if constexpr (sizeof(T) == 4 && !BF16X3) lo = legendre_low_fragment(lo_b, bc);
+  };
+  T a[4];
+  lo_frag_t lo = {};
+  if (wave < s.n_blocks) load_bwd(wave, a, lo);
   for (int bb = wave; bb < s.n_blocks; bb += WAVES) {
     const int f0 = at.block_first[bb], nb = at.block_size[bb];
     const int m = bb - s.mj_max;
     const int slot = m >= 0 ? m : NPHI + m;
-    const unsigned fbase = static_cast<unsigned>(mt.bwd_offset[bb]) * 64u * sizeof(T);
-    T a[4];
-#pragma unroll
-    for (int k = 0; k < 4; ++k) a[k] = k < k_bwd ? buffer_load<T>(fbwd, lane * sizeof(T), fbase + k * 64u * sizeof(T)) : T(0);
     RotEntry<T> w[4];
     const unsigned rbase = static_cast<unsigned>(f0) * s.nr * 2u * 16u;
 #pragma unroll
     for (int v = 0; v < 4; ++v) w[v] = buffer_load<RotEntry<T>>(rots, rot_lane[v], rbase);
+    T na[4];
+    lo_frag_t nlo = {};
+    const bool more = bb + WAVES < s.n_blocks;  // wavefront-uniform
+    if (more) load_bwd(bb + WAVES, na, nlo);
     T b[4];
 #pragma unroll
     for (int k = 0; k < 4; ++k) b[k] = k < k_bwd ? *reinterpret_cast<const T*>(dvr_b + t_lane[k] + slot * CB) : T(0);
@@ -812,7 +834,7 @@ __global__ __launch_bounds__(64 * WAVES, sizeof(T) == 8 ? 1 : 2) void interactio
 #pragma unroll
       for (int k = 0; k < 4; ++k)
         if (k < k_bwd) acc = M::fma(a[k], b[k], acc);
-      if constexpr (sizeof(T) == 4) acc = legendre_low_part(lo_b, bb, b, acc);
+      if constexpr (sizeof(T) == 4) acc = legendre_low_product(lo, b, acc);
     }
     const unsigned sbase = (static_cast<unsigned>(f0) * s.n_zr + izr0) * CB;
 #pragma unroll
@@ -825,6 +847,11 @@ __global__ __launch_bounds__(64 * WAVES, sizeof(T) == 8 ? 1 : 2) void interactio
       else
         out = w[v].x * mine + w[v].y * other;
       if (live && M::row(lane, v) < nb) buffer_store<T>(out, site, st_lane[v], sbase);
+    }
+    if (more) {
+#pragma unroll
+      for (int k = 0; k < 4; ++k) a[k] = na[k];
+      lo = nlo;
     }
   }
 }

@@ -1210,7 +1210,7 @@ std::unique_ptr<spo_hip_backend> create_state(const spo_hip_shape* shape, const 
           b->rot_pm = upload(pm.data(), pm.size() * sizeof(spo_dev::RotEntry<T>));
           if (sizeof(T) == 4) {
             // bf16 fragments of a function of L, lane (i, q) element t = contraction row 4 t + q
-            // (legendre_low_part): forward A = L (theta = i, J = row), backward A = L^T (J = i, theta = row).
+            // (legendre_low_product): forward A = L (theta = i, J = row), backward A = L^T (J = i, theta = row).
             auto bf16 = [](double x) -> std::uint16_t {
               const float f = static_cast<float>(x);
               std::uint32_t u;
