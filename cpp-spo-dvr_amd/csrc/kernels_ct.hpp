@@ -55,11 +55,33 @@ constexpr bool zr_twiddles_in_lds() {
   return sizeof(T) == 4;
 }
 
-// LDS bytes of zr_reg_kernel (complex64: the twiddles and their modulus corrections behind the plane).
+// LDS bytes of zr_reg_kernel (complex64: the twiddles and their remainders behind the plane).
 template <typename T>
 inline std::size_t zr_reg_lds(int nz, int nr) {
   return std::size_t(nz) * (nr + 1) * sizeof(cx<T>) +
-         (zr_twiddles_in_lds<T>() ? std::size_t(nz + nr) * (sizeof(cx<T>) + sizeof(T)) : 0);
+         (zr_twiddles_in_lds<T>() ? std::size_t(nz + nr) * 2 * sizeof(cx<T>) : 0);
+}
+
+// twiddle() with the complex64 remainders already formed: lo[k * LS] = eps_k w_k (the part of the twiddle that
+// restores its modulus), applied through mul_hilo. FP64 passes no lo.
+template <typename T, int R, bool SKIP_ZERO = true, int TS = 1, int LS = 1>
+__device__ __forceinline__ void twiddle_lo(cx<T>* v, const cx<T>* table, const cx<T>* lo, int j, int sign) {
+  if (SKIP_ZERO && j == 0) return;
+#pragma unroll
+  for (int r = 1; r < R; ++r) {
+    cx<T> w = table[r * j * TS];
+    if constexpr (sizeof(T) == 4) {
+      cx<T> l = lo[r * j * LS];
+      if (sign > 0) {
+        w = conj(w);
+        l = conj(l);
+      }
+      v[r] = mul_hilo(w, l, v[r]);
+    } else {
+      if (sign > 0) w = conj(w);
+      v[r] = v[r] * w;
+    }
+  }
 }
 
 // eps_z, eps_r: the complex64 twiddle tables' modulus corrections (twiddle()); factor_lo: the factor's FP32
@@ -78,23 +100,30 @@ __global__ __launch_bounds__(THREADS) void zr_reg_kernel(cx<T>* psi, const cx<T>
   // complex64: the twiddle tables live in LDS behind the plane (lower latency than the cache for the in-pass
   // twiddles); FP64 keeps them in the cache, where they do not compete with the plane for LDS bandwidth (measured
   // on MI355X: 1.89 -> 1.73 ms and 3.85 -> 4.15 ms per time step respectively).
+  // complex64: each twiddle's remainder eps w is formed once here, so that the passes spend no products on it.
+  const cx<T>* lo_r = nullptr;
+  const cx<T>* lo_z = nullptr;
   if constexpr (zr_twiddles_in_lds<T>()) {
     cx<T>* tw_r = t + NZ * LD;
     cx<T>* tw_z = tw_r + NR;
-    T* e_r = reinterpret_cast<T*>(tw_z + NZ);
-    T* e_z = e_r + NR;
+    cx<T>* l_r = tw_z + NZ;
+    cx<T>* l_z = l_r + NR;
     for (int e = tid; e < NR; e += THREADS) {
-      tw_r[e] = table_r[e];
-      e_r[e] = eps_r[e];
+      const cx<T> w = table_r[e];
+      const T k = eps_r[e];
+      tw_r[e] = w;
+      l_r[e] = {k * w.x, k * w.y};
     }
     for (int e = tid; e < NZ; e += THREADS) {
-      tw_z[e] = table_z[e];
-      e_z[e] = eps_z[e];
+      const cx<T> w = table_z[e];
+      const T k = eps_z[e];
+      tw_z[e] = w;
+      l_z[e] = {k * w.x, k * w.y};
     }
     table_r = tw_r;
     table_z = tw_z;
-    eps_r = e_r;
-    eps_z = e_z;
+    lo_r = l_r;
+    lo_z = l_z;
   }
 
   // Pass 1, r forward, radix A1: butterfly (iz, j < A2) from HBM.
@@ -125,7 +154,7 @@ __global__ __launch_bounds__(THREADS) void zr_reg_kernel(cx<T>* psi, const cx<T>
       cx<T> v[A2];
 #pragma unroll
       for (int r = 0; r < A2; ++r) v[r] = t[iz * LD + j + r * A1];
-      twiddle<T, A2>(v, table_r, j, -1, eps_r);
+      twiddle_lo<T, A2>(v, table_r, lo_r, j, -1);
       rdft<A2, T>(v, -1);
 #pragma unroll
       for (int r = 0; r < A2; ++r) t[iz * LD + j + r * A1] = v[r];
@@ -167,7 +196,7 @@ __global__ __launch_bounds__(THREADS) void zr_reg_kernel(cx<T>* psi, const cx<T>
       const int kr = b % NR, j = b / NR;
 #pragma unroll
       for (int r = 0; r < B2; ++r) v[q][r] = t[(j + r * B1) * LD + kr];
-      twiddle<T, B2>(v[q], table_z, j, -1, eps_z);
+      twiddle_lo<T, B2>(v[q], table_z, lo_z, j, -1);
       rdft<B2, T>(v[q], -1);
 #pragma unroll
       for (int r = 0; r < B2; ++r) {
@@ -198,7 +227,7 @@ __global__ __launch_bounds__(THREADS) void zr_reg_kernel(cx<T>* psi, const cx<T>
       cx<T> v[B1];
 #pragma unroll
       for (int r = 0; r < B1; ++r) v[r] = t[(j + r * B2) * LD + kr];
-      twiddle<T, B1>(v, table_z, j, +1, eps_z);
+      twiddle_lo<T, B1>(v, table_z, lo_z, j, +1);
       rdft<B1, T>(v, +1);
 #pragma unroll
       for (int r = 0; r < B1; ++r) t[(j + r * B2) * LD + kr] = v[r];
@@ -240,7 +269,7 @@ __global__ __launch_bounds__(THREADS) void zr_reg_kernel(cx<T>* psi, const cx<T>
       cx<T> v[A1];
 #pragma unroll
       for (int r = 0; r < A1; ++r) v[r] = t[iz * LD + j + r * A2];
-      twiddle<T, A1>(v, table_r, j, +1, eps_r);
+      twiddle_lo<T, A1>(v, table_r, lo_r, j, +1);
       rdft<A1, T>(v, +1);
 #pragma unroll
       for (int r = 0; r < A1; ++r) plane[iz * NR + j + r * A2] = v[r];

@@ -264,15 +264,10 @@ std::vector<Variant> xy_reg_variants(int nx, int ny) {
   }
   SPO_XY_REG_SHAPES(X)
 #undef X
-  // Default (first) variant, as measured on MI355X at 16 x 16: FP64 tiles of 16 izr through LDS in halves (3.13 vs
-  // 3.40 ms per time step for tiles of 8), complex64 tiles of 32 (1.64 vs 1.74 for 16). FP64: closest to 16 izr.
-  std::stable_sort(out.begin(), out.end(), [](const Variant& a, const Variant& b) {
-    if (sizeof(T) == 8) {
-      const int da = std::abs(a.param - 16), db = std::abs(b.param - 16);
-      return da != db ? da < db : a.param < b.param;
-    }
-    return a.param > b.param;
-  });
+  // Default (first) variant, as measured on MI355X: the widest tile in both precisions. FP64 at 16 x 16: tiles of
+  // 32 izr 3.04 ms per time step, of 16 (through LDS in halves) 3.12, of 8 3.43; at 8 x 8: 32 izr 0.112, 16 izr
+  // 0.115. Complex64 at 16 x 16: 32 izr 1.76, 16 izr 1.90, 8 izr 2.58.
+  std::stable_sort(out.begin(), out.end(), [](const Variant& a, const Variant& b) { return a.param > b.param; });
   return out;
 }
 
@@ -699,8 +694,8 @@ void State<T>::tune() {
     HIP_CHECK(hipEventDestroy(e1));
     tune_cache()[key] = best;
     if (std::getenv("SPO_HIP_TUNE_LOG"))
-      std::fprintf(stderr, "spo_hip tune %s: %s=%d (%.3f ms for two launches)\n", key.c_str(), family,
-                   by_threads ? v[best].threads : v[best].param, best_ms);
+      std::fprintf(stderr, "spo_hip tune %s: %s=%d (#%d of %zu, %.3f ms for two launches)\n", key.c_str(), family,
+                   by_threads ? v[best].threads : v[best].param, best, v.size(), best_ms);
     return best;
   };
   // The scattering wave is zero here, and every kernel maps zero to zero.
