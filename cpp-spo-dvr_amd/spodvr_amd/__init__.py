@@ -57,7 +57,7 @@ def host_library() -> ctypes.CDLL:
     if _host is None:
         path = library_path("libspo_host.so")
         if not os.path.exists(path):
-            raise RuntimeError(f"{path} is missing: build the extension first (__graft_entry__.build())")
+            raise RuntimeError(f"{path} is missing: build the libraries first (python cpp-spo-dvr_amd/build.py)")
         lib = ctypes.CDLL(path, mode=ctypes.RTLD_GLOBAL)
         lib.spo_last_error.restype = ctypes.c_char_p
         lib.spo_run_create.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int, ctypes.c_void_p,

@@ -18,7 +18,7 @@ def library() -> ctypes.CDLL:
     global _lib
     if _lib is None:
         if not os.path.exists(LIB_PATH):
-            raise RuntimeError(f"{LIB_PATH} is missing: build it first (__graft_entry__.build())")
+            raise RuntimeError(f"{LIB_PATH} is missing: build it first (python oracle/build.py)")
         import spodvr_amd  # noqa: F401  (loads libspo_host.so globally first)
         spodvr_amd.host_library()
         lib = ctypes.CDLL(LIB_PATH)
