@@ -209,8 +209,9 @@ void set_lds(const void* kernel, std::size_t bytes) {
 // Other factorisations of a shape (the radices of each axis) are further entries of that shape: they compute the same
 // transforms with their roundings in another order, and the launch tuner picks among them as among block sizes.
 #define SPO_ZR_REG_SHAPES(X)                                                                                 \
-  X(96, 48, 3, 16, 8, 12, 192) X(96, 48, 3, 16, 8, 12, 384) X(96, 48, 3, 16, 8, 12, 512)                      \
-  X(96, 48, 3, 16, 12, 8, 384) X(96, 48, 12, 4, 8, 12, 384)                                                    \
+  X(96, 48, 3, 16, 12, 8, 192) X(96, 48, 3, 16, 8, 12, 192) X(96, 48, 4, 12, 8, 12, 192)                      \
+  X(96, 48, 3, 16, 8, 12, 384) X(96, 48, 3, 16, 8, 12, 512) X(96, 48, 3, 16, 12, 8, 384)                       \
+  X(96, 48, 12, 4, 8, 12, 384)                                                                                 \
   X(128, 32, 4, 8, 16, 8, 256) X(128, 32, 4, 8, 16, 8, 512) X(64, 16, 4, 4, 8, 8, 128) X(80, 16, 4, 4, 8, 10, 128) \
   X(176, 56, 7, 8, 11, 16, 256) X(176, 56, 7, 8, 11, 16, 512) X(96, 32, 4, 8, 8, 12, 256)                      \
   X(96, 32, 4, 8, 8, 12, 512)
@@ -244,8 +245,10 @@ std::vector<Variant> zr_reg_variants(int nz, int nr) {
                    TH, spo_dev::zr_reg_lds<T>(A, B)});
   SPO_ZR_REG_SHAPES(X)
 #undef X
-  // Default (first) variant, as measured on MI355X at NZ 96 x NR 48: FP64 planes run best with the larger thread
-  // count (3.9 vs 4.7 ms per time step), complex64 planes with the smaller one.
+  // Default (first) variant, as measured on MI355X at NZ 96 x NR 48: FP64 planes run best with the largest thread
+  // count (512 threads: 3.42 ms per time step; 384: 3.44; 192: 4.2-4.5), complex64 planes with the smallest one, and
+  // among those with the Z radices 12 x 8 (1.88 ms; 8 x 12: 1.92; r radices 4 x 12: 1.91; 6 x 8, 12 x 4 and the Z
+  // radices 16 x 6 or 6 x 16: 1.98-2.15). Ties keep the order of the shape list.
   std::stable_sort(out.begin(), out.end(), [](const Variant& a, const Variant& b) {
     return sizeof(T) == 8 ? a.threads > b.threads : a.threads < b.threads;
   });
