@@ -151,13 +151,19 @@ def cpu_baseline(path: str, sample_steps: int, wave) -> dict:
                                       "step (the reference's one-NUMA-node choice, src/core/loop_schedule.hpp:6-18)")}
 
 
-def measure(path: str, workload: str, precision: str, steps: int, warmup: int) -> dict:
+def measure(path: str, workload: str, precision: str, steps: int, warmup: int, merge_half_steps: bool = False) -> dict:
     """One precision on the GPU: warm-up analysis steps, then the timed analysis steps with per-kernel HIP events.
     Every seam call returns with the device idle (the analysis step ends in a synchronous copy of the BK
-    coefficients), so the wall clock brackets exactly the timed work."""
+    coefficients), so the wall clock brackets exactly the timed work. merge_half_steps: the schedule with the
+    trailing and leading T(dt/2) of consecutive analysis steps merged into one T(dt) (SPO_HIP_MERGE_HALF_STEPS=1);
+    otherwise the plain schedule of the reference."""
     import spodvr_amd
     os.environ.setdefault("SPO_HIP_TUNE_LOG", "1")  # the launch tuner's choices go to stderr
-    run = spodvr_amd.Run(path, {"Precision": precision}, device="hip")
+    os.environ["SPO_HIP_MERGE_HALF_STEPS"] = "1" if merge_half_steps else "0"
+    try:
+        run = spodvr_amd.Run(path, {"Precision": precision}, device="hip")
+    finally:
+        os.environ.pop("SPO_HIP_MERGE_HALF_STEPS", None)
     info = run.info
     spa = info.steps_per_analysis
     timed_analysis = max(1, -(-steps // spa))
@@ -288,6 +294,11 @@ def main() -> int:
     mixed = measure(path, args.workload, "mixed", args.steps, args.warmup) if args.precision in ("both", "mixed") else None
     main_run = mixed if primary == "mixed" else fp64
     info = main_run["info"]
+    # The same workload with merged half steps (opt-in; results equal the plain schedule's up to rounding): reported
+    # beside the headline, which stays on the plain schedule.
+    merged = measure(path, args.workload, primary, args.steps, args.warmup, merge_half_steps=True)
+    merged = {"precision": primary, "value": merged["value"], "ms_per_step": merged["ms_per_step"],
+              "over_plain": merged["value"] / main_run["value"], "kernels": merged["kernels"]}
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -328,6 +339,7 @@ def main() -> int:
         "mixed_over_fp64": (mixed["value"] / fp64["value"]) if (mixed and fp64) else None,
         "fp64": brief(fp64),
         "mixed": brief(mixed),
+        "merged_half_steps": merged,
         "cpu_baseline": cpu,
         "hardwall": hardwall,
     }

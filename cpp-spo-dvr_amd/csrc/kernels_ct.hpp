@@ -86,10 +86,12 @@ __device__ __forceinline__ void twiddle_lo(cx<T>* v, const cx<T>* table, const c
 
 // eps_z, eps_r: the complex64 twiddle tables' modulus corrections (twiddle()); factor_lo: the factor's FP32
 // remainders (mul_hilo). Unused in FP64.
+// row >= 0: psi is only read, and Z row `row` of each transformed plane (NR values) goes to row_out[plane * NR + ir]
+// (the inverse r passes run for that row alone); row < 0: psi is transformed in place and row_out is unused.
 template <typename T, int NZ, int NR, int A1, int A2, int B1, int B2, int THREADS>
 __global__ __launch_bounds__(THREADS) void zr_reg_kernel(cx<T>* psi, const cx<T>* table_z, const cx<T>* table_r,
                                                          const cx<T>* factor, const T* eps_z, const T* eps_r,
-                                                         const cx<T>* factor_lo) {
+                                                         const cx<T>* factor_lo, cx<T>* row_out, int row) {
   static_assert(A1 * A2 == NR && B1 * B2 == NZ, "zr_reg_kernel: radices must multiply to the lengths");
   extern __shared__ unsigned char smem_raw[];
   cx<T>* t = reinterpret_cast<cx<T>*>(smem_raw);
@@ -243,6 +245,7 @@ __global__ __launch_bounds__(THREADS) void zr_reg_kernel(cx<T>* psi, const cx<T>
       const int b = tid + q * THREADS;
       if (N % THREADS != 0 && b >= N) break;
       const int iz = b % NZ, j = b / NZ;
+      if (row >= 0 && iz != row) continue;
 #pragma unroll
       for (int r = 0; r < A2; ++r) v[q][r] = t[iz * LD + j + r * A1];
       rdft<A2, T>(v[q], +1);
@@ -253,6 +256,7 @@ __global__ __launch_bounds__(THREADS) void zr_reg_kernel(cx<T>* psi, const cx<T>
       const int b = tid + q * THREADS;
       if (N % THREADS != 0 && b >= N) break;
       const int iz = b % NZ, j = b / NZ;
+      if (row >= 0 && iz != row) continue;
 #pragma unroll
       for (int r = 0; r < A2; ++r) t[iz * LD + j * A2 + r] = v[q][r];
     }
@@ -266,13 +270,20 @@ __global__ __launch_bounds__(THREADS) void zr_reg_kernel(cx<T>* psi, const cx<T>
       const int b = tid + q * THREADS;
       if (N % THREADS != 0 && b >= N) break;
       const int j = b % A2, iz = b / A2;
+      if (row >= 0 && iz != row) continue;
       cx<T> v[A1];
 #pragma unroll
       for (int r = 0; r < A1; ++r) v[r] = t[iz * LD + j + r * A2];
       twiddle_lo<T, A1>(v, table_r, lo_r, j, +1);
       rdft<A1, T>(v, +1);
+      if (row >= 0) {
+        cx<T>* out = row_out + static_cast<long>(blockIdx.x) * NR;
 #pragma unroll
-      for (int r = 0; r < A1; ++r) plane[iz * NR + j + r * A2] = v[r];
+        for (int r = 0; r < A1; ++r) out[j + r * A2] = v[r];
+      } else {
+#pragma unroll
+        for (int r = 0; r < A1; ++r) plane[iz * NR + j + r * A2] = v[r];
+      }
     }
   }
 }

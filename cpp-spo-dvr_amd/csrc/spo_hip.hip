@@ -413,7 +413,18 @@ struct spo_hip_backend {
     }
   }
 
-  virtual void kinetic_and_potential(bool half, bool kinetic_only) = 0;
+  // One block of the schedule: the specular grid's kinetic step (half or full), the scattering grid's T_Zr and
+  // T_xy (scattering: 0 none, 1 half, 2 full) and, unless kinetic_only, V.
+  virtual void step_block(bool specular_half, int scattering, bool kinetic_only) = 0;
+  void kinetic_and_potential(bool half, bool kinetic_only) { step_block(half, half ? 1 : 2, kinetic_only); }
+  // Merged half steps (SPO_HIP_MERGE_HALF_STEPS=1, eager single-run steps): an analysis step leaves the scattering
+  // wave's trailing T(dt/2) pending, and the next step's leading T(dt/2) applies both as one T(dt); the analysis
+  // reads its plane from a T(dt/2) of the plane alone, and every other read of the wave applies the pending half
+  // step first (flush_pending). Same results up to rounding; off by default, since a run split by a restart file
+  // then no longer repeats the unsplit run bit for bit.
+  bool merge_halves = false;
+  bool pending_half = false;
+  virtual void flush_pending() = 0;
   virtual void upload_wave(const double* scattering) = 0;
   virtual void download_wave(double* scattering) = 0;
   virtual void z_plane(int iz) = 0;
@@ -520,12 +531,17 @@ struct State final : spo_hip_backend {
   const void* v_fn = nullptr;
   bool zr_fused = false;
 
-  void run_zr(const Variant& v, bool half);
-  void run_xy(const Variant& v, bool half, bool kinetic_only);
+  Buffer ana_row;  // the analysis row of the scattering wave at the analysis time (merged half steps)
+
+  void run_zr(const Variant& v, bool half, cx<T>* row_out = nullptr, int row = -1);
+  void run_xy(const Variant& v, bool half, bool kinetic_only, cx<T>* data = nullptr,
+              const spo_dev::Shape* on = nullptr);
   void run_interaction_mfma(const Variant& v);
   void tune();
+  void scattering_kinetic(bool half, bool kinetic_only);
 
-  void kinetic_and_potential(bool half, bool kinetic_only) override;
+  void step_block(bool specular_half, int scattering, bool kinetic_only) override;
+  void flush_pending() override;
   void upload_wave(const double* scattering) override;
   void download_wave(double* scattering) override;
   void z_plane(int iz) override;
@@ -550,7 +566,7 @@ void State<T>::lines(cx<T>* data, spo_dev::Lines L, const DevicePlan<T>& p, int 
 }
 
 template <typename T>
-void State<T>::run_zr(const Variant& v, bool half) {
+void State<T>::run_zr(const Variant& v, bool half, cx<T>* row_out, int row) {
   cx<T>* ps = psi.as<cx<T>>();
   const cx<T>* tz = plan_z->tw();
   const cx<T>* tr = plan_r->tw();
@@ -559,14 +575,15 @@ void State<T>::run_zr(const Variant& v, bool half) {
   const T* er = plan_r->tw_eps();
   const cx<T>* fl = (half ? zr_half_lo : zr_full_lo).as<cx<T>>();
   void* args[] = {&ps, const_cast<cx<T>**>(&tz), const_cast<cx<T>**>(&tr), const_cast<cx<T>**>(&zr),
-                  const_cast<T**>(&ez), const_cast<T**>(&er), const_cast<cx<T>**>(&fl)};
+                  const_cast<T**>(&ez), const_cast<T**>(&er), const_cast<cx<T>**>(&fl), &row_out, &row};
   launch(v.fn, s.n_rot * s.n_xy, v.threads, v.lds, stream, args);
 }
 
 template <typename T>
-void State<T>::run_xy(const Variant& v, bool half, bool kinetic_only) {
-  cx<T>* ps = psi.as<cx<T>>();
-  const int tiles = (s.n_zr + v.param - 1) / v.param;
+void State<T>::run_xy(const Variant& v, bool half, bool kinetic_only, cx<T>* data, const spo_dev::Shape* on) {
+  cx<T>* ps = data ? data : psi.as<cx<T>>();
+  spo_dev::Shape sh = on ? *on : s;
+  const int tiles = (sh.n_zr + v.param - 1) / v.param;
   const cx<T>* factor = (half ? xy_half : xy_full).as<cx<T>>();
   const cx<T>* sf = shift_f.as<cx<T>>();
   const cx<T>* sb = shift_b.as<cx<T>>();
@@ -577,10 +594,10 @@ void State<T>::run_xy(const Variant& v, bool half, bool kinetic_only) {
   const cx<T>* gs = gas_step.as<cx<T>>();
   spo_dev::XyLo<T> lo{(half ? xy_half_lo : xy_full_lo).as<cx<T>>(), shift_f_lo.as<cx<T>>(),
                       shift_b_lo.as<cx<T>>(), before_lo.as<cx<T>>(), gas_step_lo.as<cx<T>>()};
-  void* args[] = {&ps, &s, const_cast<cx<T>**>(&factor), const_cast<cx<T>**>(&sf), const_cast<cx<T>**>(&sb),
+  void* args[] = {&ps, &sh, const_cast<cx<T>**>(&factor), const_cast<cx<T>**>(&sf), const_cast<cx<T>**>(&sb),
                   &needs_shift, &apply_potential, const_cast<T**>(&zo), const_cast<cx<T>**>(&bf),
                   const_cast<cx<T>**>(&gs), &lo};
-  launch(v.fn, tiles * s.n_rot, v.threads, v.lds, stream, args);
+  launch(v.fn, tiles * sh.n_rot, v.threads, v.lds, stream, args);
 }
 
 template <typename T>
@@ -709,8 +726,7 @@ void State<T>::tune() {
 }
 
 template <typename T>
-void State<T>::kinetic_and_potential(bool half, bool kinetic_only) {
-  specular_step(half, kinetic_only);
+void State<T>::scattering_kinetic(bool half, bool kinetic_only) {
   cx<T>* ps = psi.as<cx<T>>();
   // T_Zr.
   const cx<T>* zr = (half ? zr_half : zr_full).as<cx<T>>();
@@ -745,6 +761,21 @@ void State<T>::kinetic_and_potential(bool half, bool kinetic_only) {
                          shape.needs_shift, kinetic_only ? 0 : 1, zo, before.as<cx<T>>(), gas_step.as<cx<T>>(), lo);
     });
   }
+}
+
+template <typename T>
+void State<T>::flush_pending() {
+  if (!pending_half) return;
+  pending_half = false;
+  scattering_kinetic(true, true);
+}
+
+template <typename T>
+void State<T>::step_block(bool specular_half, int scattering, bool kinetic_only) {
+  if (scattering == 0 && !kinetic_only) throw Status(SPO_HIP_E_ARGUMENT, "a potential step without its kinetic step");
+  specular_step(specular_half, kinetic_only);
+  if (scattering != 0) scattering_kinetic(scattering == 1, kinetic_only);
+  cx<T>* ps = psi.as<cx<T>>();
   // The specular step (on `side`) is complete before the specular wave is read or written on `stream` again.
   HIP_CHECK(hipStreamWaitEvent(stream, spec_ready, 0));
   if (kinetic_only || s.n_zr_pot == 0) return;
@@ -790,6 +821,7 @@ void State<T>::kinetic_and_potential(bool half, bool kinetic_only) {
 
 template <typename T>
 void State<T>::upload_wave(const double* scattering) {
+  pending_half = false;  // a new wave at the analysis time
   const std::size_t per_izr = std::size_t(s.n_rot) * s.n_xy;
   const int chunk = static_cast<int>(std::max<std::size_t>(1, staging_values / per_izr));
   for (int izr0 = 0; izr0 < s.n_zr; izr0 += chunk) {
@@ -805,6 +837,7 @@ void State<T>::upload_wave(const double* scattering) {
 
 template <typename T>
 void State<T>::download_wave(double* scattering) {
+  flush_pending();
   const std::size_t per_izr = std::size_t(s.n_rot) * s.n_xy;
   const int chunk = static_cast<int>(std::max<std::size_t>(1, staging_values / per_izr));
   for (int izr0 = 0; izr0 < s.n_zr; izr0 += chunk) {
@@ -820,6 +853,7 @@ void State<T>::download_wave(double* scattering) {
 
 template <typename T>
 void State<T>::z_plane(int iz) {
+  flush_pending();
   const long n = long(s.n_rot) * s.n_xy * s.nr;
   hipLaunchKernelGGL(spo_dev::z_plane_kernel<T>, dim3(blocks_for(n, kThreads)), dim3(kThreads), 0, stream,
                      psi.as<cx<T>>(), s, iz, plane.as<cx<double>>());
@@ -828,6 +862,7 @@ void State<T>::z_plane(int iz) {
 
 template <typename T>
 void State<T>::block_densities() {
+  flush_pending();
   // One part per site (n_zr x n_xy partial sums over irot), then the parts added in site order: enough workgroups
   // to cover the device, where a few long per-thread loops over sites left it latency-bound.
   const int chunk = 1;
@@ -844,6 +879,7 @@ void State<T>::block_densities() {
 
 template <typename T>
 void State<T>::site_column(int ixy, double* out) {
+  flush_pending();
   const std::size_t n = std::size_t(s.n_zr) * s.n_rot;
   if constexpr (sizeof(T) == sizeof(double)) {
     HIP_CHECK(hipMemcpyAsync(out, psi.as<cx<T>>() + n * ixy, n * sizeof(cx<T>), hipMemcpyDeviceToHost, stream));
@@ -862,9 +898,27 @@ void State<T>::site_column(int ixy, double* out) {
 template <typename T>
 void State<T>::bk_scattering() {
   const spo_hip_analysis& a = analysis;
+  const cx<T>* src = psi.as<cx<T>>();
+  spo_dev::Shape sh = s;
+  int iz = a.iz_analysis;
+  if (pending_half && (zr_variants.empty() || xy_variants.empty())) flush_pending();
+  if (pending_half) {
+    // The analysis plane at the analysis time without advancing the wave: T_Zr(dt/2) of every (irot, ixy) plane
+    // with only row iz_analysis written out, then T_xy(dt/2) of those rows (layout of psi with one Z point).
+    if (!ana_row.as<void>()) ana_row = Buffer(std::size_t(s.nr) * s.n_rot * s.n_xy * sizeof(cx<T>));
+    cx<T>* row = ana_row.as<cx<T>>();
+    timed("zr_row", [&] { run_zr(zr_variants[zr_pick], true, row, a.iz_analysis); });
+    sh.nz = 1;
+    sh.n_zr = s.nr;
+    sh.nzpot = 0;
+    sh.n_zr_pot = 0;
+    timed("xy_row", [&] { run_xy(xy_variants[xy_pick], true, true, row, &sh); });
+    src = row;
+    iz = 0;
+  }
   timed("bk_scattering", [&] {
     hipLaunchKernelGGL(spo_dev::bk_scattering_kernel<T>, dim3(a.n_v * a.n_rotation), dim3(kThreads),
-                       std::size_t(s.n_xy) * sizeof(cx<double>), stream, psi.as<cx<T>>(), s, a.iz_analysis,
+                       std::size_t(s.n_xy) * sizeof(cx<double>), stream, src, sh, iz,
                        a.n_rotation, a.n_diffraction, a.n_states, rot_index.as<int>(), dif_slot.as<int>(),
                        radial_weight.as<double>(), shift_f_d.as<cx<double>>(), a.xy_scale, plan_x_d->plan,
                        plan_y_d->plan, bk.as<cx<double>>());
@@ -873,6 +927,7 @@ void State<T>::bk_scattering() {
 
 template <typename T>
 void State<T>::energy_resolved_scattering() {
+  flush_pending();
   const long block = long(s.n_zr) * s.n_rot;
   timed("energy_resolved", [&] {
     hipLaunchKernelGGL(spo_dev::energy_resolved_kernel<T>, dim3(blocks_for(block, 256), er_n_sites), dim3(256), 0,
@@ -1260,6 +1315,12 @@ std::unique_ptr<spo_hip_backend> create_state(const spo_hip_shape* shape, const 
   set_lds(reinterpret_cast<const void*>(spo_dev::lines_fft_kernel<T>), 65536);
   set_lds(reinterpret_cast<const void*>(spo_dev::lines_fft_kernel<double>), 65536);
   set_lds(reinterpret_cast<const void*>(spo_dev::bk_scattering_kernel<T>), std::size_t(s.n_xy) * c16);
+  // SPO_HIP_MERGE_HALF_STEPS=1: merged half steps between analysis steps (see spo_hip_backend::merge_halves).
+  if (const char* m = std::getenv("SPO_HIP_MERGE_HALF_STEPS")) {
+    const std::string v = m;
+    if (v != "0" && v != "1") throw Status(SPO_HIP_E_ARGUMENT, "SPO_HIP_MERGE_HALF_STEPS must be 0 or 1");
+    b->merge_halves = v == "1";
+  }
   HIP_CHECK(hipStreamSynchronize(b->stream));
   b->tune();
   return b;
@@ -1325,6 +1386,7 @@ int spo_hip_download_wave(spo_hip_backend* b, double* scattering, double* specul
 
 int spo_hip_kinetic_and_potential(spo_hip_backend* b, int half_step, int kinetic_only) {
   return guarded([&] {
+    b->flush_pending();
     b->kinetic_and_potential(half_step != 0, kinetic_only != 0);
     HIP_CHECK(hipStreamSynchronize(b->stream));
   });
@@ -1336,9 +1398,17 @@ namespace {
 
 // The blocks of one analysis step, [T/2 V], (n - 1) [T V], [T/2], enqueued on the backend's streams.
 void enqueue_blocks(spo_hip_backend* b) {
-  b->kinetic_and_potential(true, false);
+  // Merged half steps only on eager steps (a captured graph replays the plain schedule).
+  const bool merge = b->merge_halves && !b->use_graph;
+  if (!merge) b->flush_pending();
+  b->step_block(true, merge && b->pending_half ? 2 : 1, false);
   for (int k = 1; k < b->shape.steps_per_analysis; ++k) b->kinetic_and_potential(false, false);
-  b->kinetic_and_potential(true, true);
+  if (merge) {
+    b->step_block(true, 0, true);  // the specular grid's trailing half step; the scattering grid's waits
+    b->pending_half = true;
+  } else {
+    b->kinetic_and_potential(true, true);
+  }
 }
 
 // One analysis step on the backend's stream: the blocks, or in graph mode the captured step (captured here the
@@ -1368,6 +1438,7 @@ int spo_hip_graph_enable(spo_hip_backend* b, int on) {
   return guarded([&] {
     if (!b) throw Status(SPO_HIP_E_ARGUMENT, "spo_hip_graph_enable: NULL backend");
     if (on && b->profiling) throw Status(SPO_HIP_E_ARGUMENT, "spo_hip_graph_enable: kernel timing is on");
+    b->flush_pending();
     HIP_CHECK(hipStreamSynchronize(b->stream));
     if (b->step_graph) {
       HIP_CHECK(hipGraphExecDestroy(b->step_graph));
@@ -1394,7 +1465,9 @@ int spo_hip_propagate_batch(spo_hip_backend* const* backends, int n) {
     }
     // Block r of every run's schedule [T/2 V], (n - 1) [T V], [T/2], round robin over the runs: each run's launches
     // stay in its own order on its own streams, and the runs' kernels overlap where one alone does not fill the GPU.
-    // Runs in graph mode launch their captured step; the others' blocks go round robin.
+    // Runs in graph mode launch their captured step; the others' blocks go round robin. A batch step runs the plain
+    // schedule: a pending half step is applied first.
+    for (int i = 0; i < n; ++i) backends[i]->flush_pending();
     for (int i = 0; i < n; ++i)
       if (backends[i]->use_graph) enqueue_analysis_step(backends[i]);
     int longest = 0;
@@ -1459,6 +1532,7 @@ int spo_hip_batch_create(spo_hip_backend* const* backends, int n, spo_hip_batch*
       for (int j = 0; j < i; ++j)
         if (backends[j] == backends[i]) throw Status(SPO_HIP_E_ARGUMENT, "spo_hip_batch_create: a backend twice");
     }
+    for (int i = 0; i < n; ++i) backends[i]->flush_pending();
     auto batch = std::make_unique<spo_hip_batch>();
     HIP_CHECK(hipEventCreateWithFlags(&batch->fork, hipEventDisableTiming));
     for (int i = 0; i < n; ++i) {
@@ -1481,6 +1555,11 @@ int spo_hip_batch_step(spo_hip_batch* batch) {
     hipStream_t origin = m[0]->stream;
     for (spo_hip_backend* b : m)
       if (b->profiling) throw Status(SPO_HIP_E_ARGUMENT, "spo_hip_batch_step: kernel timing is on");
+    for (spo_hip_backend* b : m) {  // the graph replays the plain schedule: pending half steps are applied first
+      if (!b->pending_half) continue;
+      b->flush_pending();
+      HIP_CHECK(hipStreamSynchronize(b->stream));
+    }
     if (!batch->graph) {
       // Every member's earlier work is complete before the capture (the members' streams join the capture only
       // through the fork event).
