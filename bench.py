@@ -258,6 +258,8 @@ def main() -> int:
                     help="both: the mixed-precision line with the FP64 run beside it")
     ap.add_argument("--cpu-steps", type=int, default=10, help="time steps of the CPU baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-merged", action="store_true",
+                    help="skip the merged-half-steps measure (profiling runs of the plain schedule alone)")
     ap.add_argument("--hardwall-steps", type=int, default=500,
                     help="time steps of the HardWall FP64 line beside the DZ line (0: none)")
     ap.add_argument("--hardwall-cpu-steps", type=int, default=100,
@@ -296,9 +298,11 @@ def main() -> int:
     info = main_run["info"]
     # The same workload with merged half steps (opt-in; results equal the plain schedule's up to rounding): reported
     # beside the headline, which stays on the plain schedule.
-    merged = measure(path, args.workload, primary, args.steps, args.warmup, merge_half_steps=True)
-    merged = {"precision": primary, "value": merged["value"], "ms_per_step": merged["ms_per_step"],
-              "over_plain": merged["value"] / main_run["value"], "kernels": merged["kernels"]}
+    merged = None
+    if not args.no_merged:
+        m = measure(path, args.workload, primary, args.steps, args.warmup, merge_half_steps=True)
+        merged = {"precision": primary, "value": m["value"], "ms_per_step": m["ms_per_step"],
+                  "over_plain": m["value"] / main_run["value"], "kernels": m["kernels"]}
 
     cpu = None
     if not args.no_cpu_baseline:
