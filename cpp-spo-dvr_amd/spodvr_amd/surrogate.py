@@ -1,6 +1,6 @@
 """The potential surrogate's inputs and targets: six features of a molecular geometry that are invariant under the
 C3v operations about a top site (acting on position and azimuth together), the lattice translations and the
-exchange of the two identical atoms, and the scaled energy the network learns.
+exchange of the two identical atoms, and the energy the network learns.
 
   f1 = (Z - 4) / 2, f2 = (r - 1.6) / 0.6                         height and bond length
   f3 = cos 2 theta                                                 orientation to the normal (exchange-invariant)
@@ -8,9 +8,15 @@ exchange of the two identical atoms, and the scaled energy the network learns.
   f5 = (sin g1 + sin g2 + sin g3) / 3                              its odd part, which C3v (not C6v) keeps
   f6 = sin^2 theta sum_k cos 2 (phi - alpha_k) cos g_k / 3         the azimuth coupled to the lattice
 with (u, v) the skewed coordinates, g1 = 2 pi u / a, g2 = 2 pi v / a, g3 = -(g1 + g2), and alpha_k the directions of
-the reciprocal vectors G_k (-30, 90 and 210 degrees for the 60-degree cell). The target is V in eV clipped to
-[V_LOW, V_HIGH]: above it the propagation's potential is capped, and the surrogate's error is judged only where the
-reference surface is below V_HIGH."""
+the reciprocal vectors G_k (-30, 90 and 210 degrees for the 60-degree cell).
+
+The network learns the CRP's own smooth part: the remainder R = V - w(Z) (V_atom(A) + V_atom(B)) of the
+molecule-surface energy after the two atom-surface energies (the corrugation-reducing split of the CRP method), in
+eV, where V is the QD wrapper's energy (r and Z clamped into the data, the free molecule at its last tabulated r as
+the zero) and w fades the atom terms out across the switch to the gas phase (atom_weight). The atom terms carry the
+steep, site-specific repulsion, which six features cannot resolve; R varies slowly over the cell. The surrogate's V
+is R + w (V_atom(A) + V_atom(B)), clamped to [V_MIN, V_MAX] as the propagation clamps its grid (keys ImposeVMin /
+VMin and ImposeVMax / VMax, defaults -5.5 and 10 eV)."""
 
 from __future__ import annotations
 
@@ -18,7 +24,7 @@ import math
 
 import numpy as np
 
-V_LOW, V_HIGH = -1.0, 4.0  # eV
+V_MIN, V_MAX = -5.5, 10.0  # eV: the propagation's default clamp of the potential grid
 ALPHAS = np.radians([-30.0, 90.0, 210.0])
 
 
@@ -40,48 +46,106 @@ def features(points: np.ndarray, lattice_constant: float) -> np.ndarray:
     return f.astype(np.float32)
 
 
-def target(v_ev: np.ndarray) -> np.ndarray:
-    return np.clip(np.asarray(v_ev, dtype=np.float64), V_LOW, V_HIGH).astype(np.float32)
+def clamp_points(points: np.ndarray, parameters: dict) -> np.ndarray:
+    """The geometries with r and Z clamped into the data set's range as the QD wrapper clamps them (the energy is
+    constant beyond): the network sees the geometry the CRP evaluates."""
+    p = np.array(points, dtype=np.float64)
+    delta = 1e-12
+    p[:, 3] = np.clip(p[:, 3], parameters["r_min"] + delta, parameters["r_max"] - delta)
+    p[:, 2] = np.maximum(p[:, 2], parameters["z_min"] + delta)
+    return p
 
 
-def sample_points(parameters: dict, n: int, seed: int, z_range=(0.5, 8.0), r_range=(0.6, 3.0)) -> np.ndarray:
-    """n random geometries over the cell (u, v), Z, r and the orientation sphere (theta uniform in cos theta)."""
+def atom_positions(points: np.ndarray, parameters: dict):
+    """The two atoms (bohr, Cartesian) of each geometry u v Z r theta phi, as the QD wrapper places them: r and Z
+    clamped into the data set's range, (x, y) = (u + v cos g, v sin g) with g the cell's skewing angle."""
+    p = np.asarray(points, dtype=np.float64)
+    u, v, z, r, th, ph = (p[:, k] for k in range(6))
+    delta = 1e-12
+    r = np.clip(r, parameters["r_min"] + delta, parameters["r_max"] - delta)
+    z = np.maximum(z, parameters["z_min"] + delta)
+    g = parameters["skewing_angle"]
+    centre = np.column_stack([u + v * math.cos(g), v * math.sin(g), z])
+    half = 0.5 * r[:, None] * np.column_stack([np.sin(th) * np.cos(ph), np.sin(th) * np.sin(ph), np.cos(th)])
+    return centre - half, centre + half
+
+
+def atom_weight(z: np.ndarray, parameters: dict) -> np.ndarray:
+    """1 below the molecule surface's switch to the gas phase (Z <= zso_min), 0 above it (Z >= zso_max, where the
+    molecule's energy no longer depends on the atoms), cos^2 between: the atom terms fade out with it, so that the
+    remainder stays continuous and the atoms are never evaluated beyond their data."""
+    lo, hi = parameters["zso_min"], parameters["zso_max"]
+    s = np.clip((np.asarray(z, dtype=np.float64) - lo) / (hi - lo), 0.0, 1.0)
+    return np.where(s < 1.0, np.cos(0.5 * math.pi * s) ** 2, 0.0)  # exactly 0 from zso_max on
+
+
+def atom_energies(crp, points: np.ndarray) -> np.ndarray:
+    """w(Z) (V_atom(A) + V_atom(B)) in eV at each geometry (the CRP's atom-surface PES on the host; w from
+    atom_weight at the clamped Z)."""
+    from . import CRP_EV
+    par = crp.parameters()
+    z = np.maximum(np.asarray(points, dtype=np.float64)[:, 2], par["z_min"] + 1e-12)
+    w = atom_weight(z, par)
+    out = np.zeros(len(z))
+    near = np.flatnonzero(w > 0.0)
+    if near.size:
+        a, b = atom_positions(np.asarray(points)[near], par)
+        out[near] = w[near] * (crp.atom(a, energy_unit=CRP_EV)[:, 0] + crp.atom(b, energy_unit=CRP_EV)[:, 0])
+    return out
+
+
+def target(v_ev: np.ndarray, atoms_ev: np.ndarray) -> np.ndarray:
+    """The remainder the network learns: V minus atom_energies, eV."""
+    return (np.asarray(v_ev, dtype=np.float64) - np.asarray(atoms_ev, dtype=np.float64)).astype(np.float32)
+
+
+def sample_points(parameters: dict, n: int, seed: int, z_range=None, r_range=None) -> np.ndarray:
+    """n random geometries over the cell (u, v), Z, r and the orientation sphere (theta uniform in cos theta). By
+    default Z and r cover the data set's range: Z from its lowest value to 1 bohr past the gas-phase asymptote, r
+    over the tabulated bond lengths (the QD wrapper clamps outside both)."""
     rng = np.random.default_rng(seed)
     a = parameters["lattice_constant"]
+    if z_range is None:
+        z_range = (parameters["z_min"], parameters["z_asymptotic"] + 1.0)
+    if r_range is None:
+        r_range = (parameters["r_min"], parameters["r_max"])
     return np.column_stack([rng.uniform(0, a, n), rng.uniform(0, a, n), rng.uniform(*z_range, n),
                             rng.uniform(*r_range, n), np.arccos(rng.uniform(-1, 1, n)),
                             rng.uniform(0, 2 * math.pi, n)])
 
 
 def train(net, crp, n: int = 1 << 19, seed: int = 1, schedule=((2e-3, 6000), (1e-3, 6000), (3e-4, 6000))):
-    """Trains `net` (spodvr_amd.Mlp) on n CRP energies at random geometries (the CRP evaluated on the GPU);
-    returns the batch losses of every step."""
-    pts = sample_points(crp.parameters(), n, seed)
-    x = features(pts, crp.parameters()["lattice_constant"])
-    t = target(crp.device_qd(pts))
+    """Trains `net` (spodvr_amd.Mlp) on the remainders of n CRP energies at random geometries (the molecule energy
+    from the CRP on the GPU, the atom energies on the host); returns the batch losses of every step."""
+    par = crp.parameters()
+    pts = sample_points(par, n, seed)
+    x = features(clamp_points(pts, par), par["lattice_constant"])
+    t = target(crp.device_qd(pts), atom_energies(crp, pts))
     return np.concatenate([net.train(x, t, steps=steps, lr=lr) for lr, steps in schedule])
 
 
-def grid_values(points: np.ndarray, slab: int, lattice_constant: float, net, reference, stride: int = 16,
-                tolerance: float = 0.01):
+def grid_values(points: np.ndarray, slab: int, net, crp, stride: int = 16, tolerance: float = 0.01,
+                v_min: float = V_MIN, v_max: float = V_MAX):
     """V in eV at `points` (n, 6), ordered slab by slab (`slab` points each, e.g. one (Z, r) pair of the potential
-    grid): the network where it is trusted, `reference` (the CRP on the GPU) elsewhere. A slab is trusted when the
-    network agrees with the reference within `tolerance` eV at every `stride`-th of its points and none of those
-    lies above V_HIGH; the checked points keep the reference's values. Returns (values, served, trusted): served
-    marks the points the network gave, trusted the slabs."""
+    grid), clamped to [v_min, v_max]: the network (its remainder plus the two atom energies) where it is trusted,
+    the CRP on the GPU elsewhere. A slab is trusted when the network's clamped value agrees with the CRP's within
+    `tolerance` eV at every `stride`-th of its points; the checked points keep the CRP's values. Returns (values,
+    served, trusted): served marks the points the network gave, trusted the slabs."""
     n = len(points)
     if n % slab:
         raise ValueError("the points are not a whole number of slabs")
-    y = net.predict(features(points, lattice_constant)).astype(np.float64)
+    par = crp.parameters()
+    y = net.predict(features(clamp_points(points, par), par["lattice_constant"])).astype(np.float64)
+    y = np.clip(y + atom_energies(crp, points), v_min, v_max)
     checked = (np.arange(n // slab)[:, None] * slab + np.arange(0, slab, stride)[None, :]).ravel()
-    exact = reference(points[checked])
+    exact = np.clip(crp.device_qd(points[checked]), v_min, v_max)
     err = np.abs(y[checked] - exact).reshape(n // slab, -1)
-    trusted = (err.max(axis=1) <= tolerance) & (exact.reshape(n // slab, -1).max(axis=1) < V_HIGH)
+    trusted = err.max(axis=1) <= tolerance
     values = y
     values[checked] = exact
     redo = np.flatnonzero(np.repeat(~trusted, slab))
     if redo.size:
-        values[redo] = reference(points[redo])
+        values[redo] = np.clip(crp.device_qd(points[redo]), v_min, v_max)
     served = np.repeat(trusted, slab)
     served[checked] = False
     return values, served, trusted

@@ -1,7 +1,8 @@
 """The potential surrogate's network on the GPU (csrc/kernels_mlp.hpp through spo_hip_mlp_*): the gradient of one
 batch against the CPU oracle (oracle/mlp.py, which rounds the hidden layers' matrix operands to bf16 as the
 matrix cores take them, so only FP32 accumulation separates the two: 2e-3 of each block's largest entry), the
-prediction against the oracle's forward pass, and training on CRP energies. Every test prints its case first."""
+prediction against the oracle's forward pass, training on CRP energies, the trained surrogate's accuracy and the
+potential grid it builds with the CRP as its fallback. Every test prints its case first."""
 
 import os
 import sys
@@ -48,14 +49,14 @@ def test_gradient_matches_oracle():
 def test_training_on_crp_energies_converges():
     import spodvr_amd
     from spodvr_amd import surrogate
-    _note("training on CRP energies, SRP48 data set")
+    _note("training on CRP remainders, SRP48 data set")
     crp = spodvr_amd.CrpSurface(os.path.join(cases.GOLDEN_DIR, "crp", "cu111_srp48", "dataset"))
     net = spodvr_amd.Mlp(1024)
     try:
-        pts = surrogate.sample_points(crp.parameters(), 65536, seed=1)
-        v = crp.device_qd(pts)
-        x = surrogate.features(pts, crp.parameters()["lattice_constant"])
-        t = surrogate.target(v)
+        par = crp.parameters()
+        pts = surrogate.sample_points(par, 65536, seed=1)
+        x = surrogate.features(surrogate.clamp_points(pts, par), par["lattice_constant"])
+        t = surrogate.target(crp.device_qd(pts), surrogate.atom_energies(crp, pts))
         net.params = mlp.initial_parameters(5)
         losses = net.train(x, t, steps=4000, lr=2e-3)
         assert np.all(np.isfinite(losses))
@@ -67,46 +68,73 @@ def test_training_on_crp_energies_converges():
         crp.close()
 
 
-@pytest.mark.parametrize("tolerance", [0.01, 0.3])
-def test_surrogate_grid_with_crp_fallback(tolerance):
+@pytest.fixture(scope="module")
+def trained_surrogate():
+    """The surrogate trained once (surrogate.train: 2^19 CRP remainders, 18000 Adam steps of 4096)."""
+    import spodvr_amd
+    from spodvr_amd import surrogate
+    _note("training the surrogate, SRP48 data set")
+    crp = spodvr_amd.CrpSurface(os.path.join(cases.GOLDEN_DIR, "crp", "cu111_srp48", "dataset"))
+    net = spodvr_amd.Mlp(4096)
+    net.params = mlp.initial_parameters(5)
+    surrogate.train(net, crp)
+    yield crp, net
+    net.close()
+    crp.close()
+
+
+def test_surrogate_accuracy_on_held_out_geometries(trained_surrogate):
+    """The trained network (its remainder plus the two atom energies) against the CRP at 65536 held-out random
+    geometries, both clamped as the propagation clamps its grid. Measured on MI355X: RMS 0.021 eV below 0 eV, 73 %
+    of those points within 10 meV and 98 % within 50 meV. Learning V itself with the same six features reached an RMS
+    of 0.40 eV."""
+    from spodvr_amd import surrogate
+    crp, net = trained_surrogate
+    par = crp.parameters()
+    pts = surrogate.sample_points(par, 65536, seed=9)
+    ref = np.clip(crp.device_qd(pts), surrogate.V_MIN, surrogate.V_MAX)
+    y = net.predict(surrogate.features(surrogate.clamp_points(pts, par), par["lattice_constant"])).astype(np.float64)
+    err = np.abs(np.clip(y + surrogate.atom_energies(crp, pts), surrogate.V_MIN, surrogate.V_MAX) - ref)
+    low = ref < 0.0
+    rms = float(np.sqrt(np.mean(err[low] ** 2)))
+    within = float(np.mean(err[low] < 0.01))
+    _note(f"held out, V < 0 eV ({low.mean():.2f} of the points): RMS {rms:.4f} eV, within 10 meV {within:.3f}, "
+          f"within 50 meV {np.mean(err[low] < 0.05):.3f}")
+    assert rms < 0.05 and within > 0.5
+
+
+@pytest.mark.parametrize("tolerance,least_served", [(0.01, 0.05), (0.05, 0.3)])
+def test_surrogate_grid_with_crp_fallback(trained_surrogate, tolerance, least_served):
     """The surrogate on the potential grid of crp_cu111_srp48_small's shape (6 x 6 (x, y), 3 x 10 orientations,
-    86 Z x 48 r) against the full CRP grid: the slabs it does not trust hold the CRP's values exactly, and the
-    values it serves stay near the trust tolerance (a slab is trusted from every 16th point). At 10 meV it serves
-    nothing (validation RMS ~0.4 eV with six features); at 0.3 eV it serves part of the grid."""
+    86 Z x 48 r) against the full CRP grid, both clamped to the propagation's [VMin, VMax]: the slabs it does not trust
+    hold the CRP's values exactly, the values it serves stay near the trust tolerance (a slab is trusted from every
+    16th point), and it serves a measured share of the grid. Measured on MI355X: 12 % of the points at 10 meV, 62 %
+    at 50 meV (73 % at 0.1 eV). The CRP on the GPU builds the same grid in 0.03 s, against 1.7 s for the surrogate
+    path, whose features and atom energies are computed on the host."""
     import math
     import time
 
-    import spodvr_amd
     from spodvr_amd import surrogate
+    crp, net = trained_surrogate
     _note("surrogate grid with CRP fallback, SRP48 data set, tolerance", tolerance, "eV")
-    crp = spodvr_amd.CrpSurface(os.path.join(cases.GOLDEN_DIR, "crp", "cu111_srp48", "dataset"))
-    net = spodvr_amd.Mlp(4096)
-    try:
-        a = crp.parameters()["lattice_constant"]
-        net.params = mlp.initial_parameters(5)
-        surrogate.train(net, crp)
-        u = np.arange(6) * a / 6
-        thetas = np.array([0.436634949225522206, 1.00217680364312156, 1.57079632679489656])
-        phis = np.arange(10) * 2 * math.pi / 10
-        z = -1.0 + 0.15 * np.arange(86)
-        r = 0.4 + 0.2 * np.arange(48)
-        zz, rr, uu, vv, tt, pp = np.meshgrid(z, r, u, u, thetas, phis, indexing="ij")
-        pts = np.column_stack([uu.ravel(), vv.ravel(), zz.ravel(), rr.ravel(), tt.ravel(), pp.ravel()])
-        slab = 6 * 6 * 3 * 10
-        t0 = time.perf_counter()
-        full = crp.device_qd(pts)
-        t_crp = time.perf_counter() - t0
-        t0 = time.perf_counter()
-        values, served, trusted = surrogate.grid_values(pts, slab, a, net, crp.device_qd, tolerance=tolerance)
-        t_sur = time.perf_counter() - t0
-        err = np.abs(values - full)
-        _note(f"served {served.mean():.3f} of the points, {trusted.mean():.3f} of the slabs; max error served "
-              f"{err[served].max() if served.any() else 0:.4g} eV; CRP grid {t_crp:.3f} s, surrogate grid {t_sur:.3f} s")
-        assert np.all(err[~served] == 0.0)
-        if tolerance >= 0.3:
-            assert served.any()
-        if served.any():
-            assert np.quantile(err[served], 0.99) <= 2 * tolerance
-    finally:
-        net.close()
-        crp.close()
+    a = crp.parameters()["lattice_constant"]
+    u = np.arange(6) * a / 6
+    thetas = np.array([0.436634949225522206, 1.00217680364312156, 1.57079632679489656])
+    phis = np.arange(10) * 2 * math.pi / 10
+    z = -1.0 + 0.15 * np.arange(86)
+    r = 0.4 + 0.2 * np.arange(48)
+    zz, rr, uu, vv, tt, pp = np.meshgrid(z, r, u, u, thetas, phis, indexing="ij")
+    pts = np.column_stack([uu.ravel(), vv.ravel(), zz.ravel(), rr.ravel(), tt.ravel(), pp.ravel()])
+    slab = 6 * 6 * 3 * 10
+    t0 = time.perf_counter()
+    full = np.clip(crp.device_qd(pts), surrogate.V_MIN, surrogate.V_MAX)
+    t_crp = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    values, served, trusted = surrogate.grid_values(pts, slab, net, crp, tolerance=tolerance)
+    t_sur = time.perf_counter() - t0
+    err = np.abs(values - full)
+    _note(f"served {served.mean():.3f} of the points, {trusted.mean():.3f} of the slabs; max error served "
+          f"{err[served].max() if served.any() else 0:.4g} eV; CRP grid {t_crp:.3f} s, surrogate grid {t_sur:.3f} s")
+    assert np.all(err[~served] == 0.0)
+    assert served.mean() >= least_served
+    assert np.quantile(err[served], 0.99) <= 2 * tolerance

@@ -46,7 +46,7 @@ def test_surrogate_features_are_invariant():
 
     from spodvr_amd import surrogate
     a = 4.9
-    pts = surrogate.sample_points({"lattice_constant": a}, 200, 1)
+    pts = surrogate.sample_points({"lattice_constant": a}, 200, 1, z_range=(0.5, 8.0), r_range=(0.6, 3.0))
 
     def cart(u, v):
         return u + v * 0.5, v * math.sqrt(3) / 2
@@ -66,3 +66,40 @@ def test_surrogate_features_are_invariant():
     f0 = surrogate.features(pts, a)
     for im in images:
         np.testing.assert_allclose(surrogate.features(np.array(im), a), f0, rtol=0, atol=1e-6)
+
+
+def test_surrogate_geometry_is_the_qd_wrappers():
+    """The surrogate's atoms (surrogate.atom_positions, r and Z clamped as the QD wrapper clamps them) give the QD
+    wrapper's energy through the molecule-surface PES: molecule(A, B) in eV minus the free molecule's asymptote equals
+    qd(u, v, Z, r, theta, phi), at random geometries that include clamped ones (SRP48 data set, host CRP). A wrong
+    skew, a swapped component or a missing clamp moves the energies by far more than the 1e-9 eV allowed."""
+    import os
+
+    import spodvr_amd
+    from spodvr_amd import surrogate
+    from tests import cases
+    crp = spodvr_amd.CrpSurface(os.path.join(cases.GOLDEN_DIR, "crp", "cu111_srp48", "dataset"))
+    try:
+        par = crp.parameters()
+        pts = surrogate.sample_points(par, 2000, seed=4, z_range=(-1.0, 7.0), r_range=(0.4, 5.0))
+        a, b = surrogate.atom_positions(pts, par)
+        mol = crp.molecule(np.column_stack([a, b]), energy_unit=spodvr_amd.CRP_EV)[:, 0]
+        qd = crp.qd(pts)
+        shift = mol - qd  # the asymptote, the same constant at every geometry
+        assert np.max(np.abs(shift - np.median(shift))) < 1e-9
+        clamped = surrogate.clamp_points(pts, par)
+        assert np.array_equal(surrogate.atom_positions(clamped, par)[0], a)
+        assert np.allclose(crp.qd(clamped), qd, rtol=0, atol=1e-12)
+    finally:
+        crp.close()
+
+
+def test_surrogate_atom_weight():
+    """The fade of the atom terms across the switch to the gas phase: 1 up to zso_min, exactly 0 from zso_max on
+    (the atoms are not evaluated there), continuous and decreasing between."""
+    from spodvr_amd import surrogate
+    par = {"zso_min": 6.4, "zso_max": 7.6}
+    z = np.linspace(5.0, 9.0, 4001)
+    w = surrogate.atom_weight(z, par)
+    assert np.all(w[z <= 6.4] == 1.0) and np.all(w[z >= 7.6] == 0.0)
+    assert np.all(np.diff(w) <= 0.0) and np.max(np.abs(np.diff(w))) < 2e-3
