@@ -1,4 +1,5 @@
 // C ABI of the host driver (include/spo_host.h).
+#include <cmath>
 #include <cstdlib>
 #include <cstring>
 #include <exception>
@@ -62,8 +63,13 @@ extern "C" {
 
 const char* spo_last_error(void) { return g_error.c_str(); }
 
-int spo_run_create(const char* input_path, const char* overrides_json, int device, spo_backend_factory_fn factory,
-                   spo_run** out) {
+}  // extern "C"
+
+namespace {
+
+// spo_run_create, with the set-up's potential grid replaced by `potential` (n values) when it is not NULL.
+int create_run(const char* input_path, const char* overrides_json, int device, spo_backend_factory_fn factory,
+               const double* potential, size_t n_potential, spo_run** out) {
   return guarded([&] {
     if (!out) throw std::invalid_argument("spo_run_create: out is NULL");
     *out = nullptr;
@@ -87,6 +93,16 @@ int spo_run_create(const char* input_path, const char* overrides_json, int devic
       throw std::invalid_argument("SPO_HIP_POTENTIAL must be host or device");
     const bool device_potential = device == SPO_DEVICE_HIP && !(where && std::string(where) == "host");
     run->setup = spo::make_run_setup(spo::read_input(doc), device_potential);
+    if (potential) {
+      std::vector<double>& v = run->setup->potential.values;
+      if (n_potential != v.size())
+        throw std::invalid_argument("spo_run_create_with_potential: " + std::to_string(n_potential) +
+                                    " values for a potential grid of " + std::to_string(v.size()));
+      for (size_t k = 0; k < n_potential; ++k)
+        if (!std::isfinite(potential[k]))
+          throw std::invalid_argument("spo_run_create_with_potential: value " + std::to_string(k) + " is not finite");
+      v.assign(potential, potential + n_potential);
+    }
     spo::BackendFactory make;
     if (device == SPO_DEVICE_HIP) {
       make = spo::make_hip_backend;
@@ -108,6 +124,22 @@ int spo_run_create(const char* input_path, const char* overrides_json, int devic
     run->propagation = std::make_unique<spo::Propagation>(run->setup, counted);
     *out = run.release();
   });
+}
+
+}  // namespace
+
+extern "C" {
+
+int spo_run_create(const char* input_path, const char* overrides_json, int device, spo_backend_factory_fn factory,
+                   spo_run** out) {
+  return create_run(input_path, overrides_json, device, factory, nullptr, 0, out);
+}
+
+int spo_run_create_with_potential(const char* input_path, const char* overrides_json, int device,
+                                  spo_backend_factory_fn factory, const double* potential, size_t n_potential,
+                                  spo_run** out) {
+  if (!potential) return fail(SPO_E_OTHER, "spo_run_create_with_potential: potential is NULL");
+  return create_run(input_path, overrides_json, device, factory, potential, n_potential, out);
 }
 
 int spo_run_destroy(spo_run* run) {
@@ -309,6 +341,12 @@ int spo_run_array(const spo_run* run, const char* name, double* out, size_t capa
     else if (n == "gas.wave_functions") v = s.gas.wave_functions;
     else if (n == "potential.values") v = s.potential.values;
     else if (n == "potential.asymptotic") v = s.potential.asymptotic;
+    else if (n == "grids.x") v = s.grids.x.points;
+    else if (n == "grids.y") v = s.grids.y.points;
+    else if (n == "units.hartree_in_ev") v = {spo::Units::hartree_in_ev};
+    else if (n == "switch.z") {
+      if (s.config.input.use_switching) v = {s.config.input.z_min_switch, s.config.input.z_max_switch};
+    }
     else if (n == "angular.thetas") v = s.angular.thetas;
     else if (n == "angular.weights") v = s.angular.theta_weights;
     else if (n == "angular.phis") v = s.angular.phis;

@@ -62,6 +62,9 @@ def host_library() -> ctypes.CDLL:
         lib.spo_last_error.restype = ctypes.c_char_p
         lib.spo_run_create.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int, ctypes.c_void_p,
                                        ctypes.POINTER(ctypes.c_void_p)]
+        lib.spo_run_create_with_potential.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int, ctypes.c_void_p,
+                                                      ctypes.c_void_p, ctypes.c_size_t,
+                                                      ctypes.POINTER(ctypes.c_void_p)]
         lib.spo_run_destroy.argtypes = [ctypes.c_void_p]
         lib.spo_run_get_info.argtypes = [ctypes.c_void_p, ctypes.POINTER(RunInfo)]
         lib.spo_run_backend.argtypes = [ctypes.c_void_p]
@@ -363,9 +366,12 @@ class Run:
     """A propagation run: the set-up from a JSON input (plus overrides) and a backend.
 
     device="hip" uses the HIP backend; device="external" uses factory, a C function pointer
-    (void* (*)(const void*)) that makes a spo::PropagationBackend (e.g. the CPU oracle's, in tests)."""
+    (void* (*)(const void*)) that makes a spo::PropagationBackend (e.g. the CPU oracle's, in tests). potential, when
+    given, replaces the set-up's potential grid (hartree, in the order of array("potential.values"); see
+    spo_run_create_with_potential)."""
 
-    def __init__(self, input_path: str, overrides: Optional[dict] = None, device: str = "hip", factory=None):
+    def __init__(self, input_path: str, overrides: Optional[dict] = None, device: str = "hip", factory=None,
+                 potential: Optional[np.ndarray] = None):
         lib = host_library()
         handle = ctypes.c_void_p()
         over = json.dumps(overrides).encode() if overrides else None
@@ -377,7 +383,12 @@ class Run:
             code, fptr = SPO_DEVICE_EXTERNAL, ctypes.cast(factory, ctypes.c_void_p)
         else:
             raise ValueError(f"unknown device {device!r}")
-        _check(lib.spo_run_create(input_path.encode(), over, code, fptr, ctypes.byref(handle)))
+        if potential is None:
+            _check(lib.spo_run_create(input_path.encode(), over, code, fptr, ctypes.byref(handle)))
+        else:
+            grid = np.ascontiguousarray(potential, dtype=np.float64)
+            _check(lib.spo_run_create_with_potential(input_path.encode(), over, code, fptr, _ptr(grid), grid.size,
+                                                     ctypes.byref(handle)))
         self._h = handle
         info = RunInfo()
         _check(lib.spo_run_get_info(self._h, ctypes.byref(info)))

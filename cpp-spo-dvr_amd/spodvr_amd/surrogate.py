@@ -149,3 +149,45 @@ def grid_values(points: np.ndarray, slab: int, net, crp, stride: int = 16, toler
     served = np.repeat(trusted, slab)
     served[checked] = False
     return values, served, trusted
+
+
+def grid_points(run) -> tuple[np.ndarray, np.ndarray]:
+    """The geometries (u v Z r theta phi) of a run's potential grid in the order of its array("potential.values")
+    ((theta, phi) fastest, then the representative sites, then izr = iz NR + ir), and the mask of the entries at or
+    below ZminSwitch (all of them without UseSwitching): the rows the surface itself fills, before the switch to the
+    gas phase blends them."""
+    i = run.info
+    x, y, z, r = (run.array(f"grids.{k}") for k in "xyzr")
+    thetas, phis = run.array("angular.thetas"), run.array("angular.phis")
+    reps = run.array("symmetry.representatives").astype(int)
+    ix, iy = reps // i.ny, reps % i.ny
+    iz, ir = np.divmod(np.arange(i.nz_potential * i.nr), i.nr)
+    n_tp, n_rep = len(thetas) * len(phis), len(reps)
+    izr = np.repeat(np.arange(len(iz)), n_rep * n_tp)
+    irep = np.tile(np.repeat(np.arange(n_rep), n_tp), len(iz))
+    itp = np.tile(np.arange(n_tp), len(iz) * n_rep)
+    it, ip = np.divmod(itp, len(phis))
+    pts = np.column_stack([x[ix[irep]], y[iy[irep]], z[iz[izr]], r[ir[izr]], thetas[it], phis[ip]])
+    switch = run.array("switch.z")
+    below = np.ones(len(pts), bool) if switch.size == 0 else pts[:, 2] <= switch[0]
+    return pts, below
+
+
+def propagation_grid(run, net, crp, tolerance: float = 0.01, stride: int = 16):
+    """A potential grid for spodvr_amd.Run(..., potential=...), built from the network on the rows below the switch
+    to the gas phase: the (Z, r) slabs the network is trusted on (grid_values, checked against the CRP at every
+    `stride`-th point within `tolerance` eV) take its values in hartree, and every other entry keeps the value of the
+    run's own CRP-built grid exactly (untrusted slabs, the checked points, the switched rows). Returns (grid,
+    served): served marks the entries the network gave."""
+    pts, below = grid_points(run)
+    grid = run.array("potential.values").copy()
+    i = run.info
+    slab = i.n_theta * i.n_phi * i.n_representatives
+    rows = np.flatnonzero(below)
+    if rows.size == 0:
+        return grid, np.zeros(len(grid), bool)
+    values, served_rows, _ = grid_values(pts[rows], slab, net, crp, stride=stride, tolerance=tolerance)
+    served = np.zeros(len(grid), bool)
+    served[rows] = served_rows
+    grid[served] = values[served_rows] / run.array("units.hartree_in_ev")[0]
+    return grid, served

@@ -49,6 +49,14 @@ const char* spo_last_error(void);
  * set-up and the backend. factory is used with SPO_DEVICE_EXTERNAL only. */
 int spo_run_create(const char* input_path, const char* overrides_json, int device, spo_backend_factory_fn factory,
                    spo_run** out);
+/* spo_run_create with the potential grid of the set-up replaced by `potential` before the backend is built: n values
+ * in hartree in the grid's own order (spo_run_array "potential.values": (itheta phi, representative, izr) with
+ * (theta, phi) fastest, izr = iz NR + ir over the Z points of the interaction region), after the switch to the gas
+ * phase and the VMin / VMax clamp. No counterpart in the reference: it propagates a grid built elsewhere (a potential
+ * surrogate). A size that differs from the grid's, or a value that is not finite, is an error. */
+int spo_run_create_with_potential(const char* input_path, const char* overrides_json, int device,
+                                  spo_backend_factory_fn factory, const double* potential, size_t n_potential,
+                                  spo_run** out);
 int spo_run_destroy(spo_run* run);
 int spo_run_get_info(const spo_run* run, spo_run_info* info);
 /* The backend object (a spo::PropagationBackend*), for stage-level checks. */
@@ -113,8 +121,9 @@ int spo_run_energy_resolved(spo_run* run, double* sums, double* specular_sums);
  * "angular.phis", "basis.j", "basis.mj", "state.j", "state.mj", "state.n", "state.m", "state.order",
  * "level.energies" (E(v, j) in eV, j + (jmax+1) v), "symmetry.representatives", "tables.z_optical",
  * "tables.r_optical", "initial.specular" (complex), "angular.legendre" (sqrt(w_t / nphi) P_J^m(theta_t), ordered
- * (mJ block, theta, J)), "grids.z", "grids.r", "derived.cell" (lattice_x, lattice_y, skew angle, total mass, dx, dy,
- * dt, Cartesian parallel momentum x, y). count receives the element count (doubles, or complex
+ * (mJ block, theta, J)), "grids.x", "grids.y" (skewed coordinates), "grids.z", "grids.r", "derived.cell" (lattice_x,
+ * lattice_y, skew angle, total mass, dx, dy, dt, Cartesian parallel momentum x, y), "units.hartree_in_ev" (the
+ * set-up's factor), "switch.z" (ZminSwitch, ZmaxSwitch; empty without UseSwitching). count receives the element count (doubles, or complex
  * values for complex arrays); out may be NULL to query the count. */
 int spo_run_array(const spo_run* run, const char* name, double* out, size_t capacity, size_t* count);
 

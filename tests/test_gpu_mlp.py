@@ -138,3 +138,35 @@ def test_surrogate_grid_with_crp_fallback(trained_surrogate, tolerance, least_se
     assert np.all(err[~served] == 0.0)
     assert served.mean() >= least_served
     assert np.quantile(err[served], 0.99) <= 2 * tolerance
+
+
+def test_surrogate_grid_propagated_against_the_crp_grid(trained_surrogate):
+    """The surrogate's potential grid propagated: crp_cu111_srp48_small's grid built by the network where it is
+    trusted at 10 meV (the CRP-built values elsewhere, surrogate.propagation_grid) and the CRP-built grid, both runs
+    in complex64 stepped together as one batch to the end; the reaction and channel probabilities of the two are
+    compared, the CRP run being the judge. Measured on MI355X: the network gives 10 % of the grid's entries;
+    max |dP_reaction| 8.5e-4 (P_reaction up to 0.27) and max |dP| 1.5e-3. The bound is about twice that: a surrogate
+    trusted at 10 meV moves the probabilities by ~1e-3, ten times the complex64 mode's own error."""
+    import spodvr_amd
+    from spodvr_amd import surrogate
+    crp, net = trained_surrogate
+    path = cases.golden_input("crp_cu111_srp48_small")
+    over = dict(cases.golden_overrides("crp_cu111_srp48_small"), Precision="mixed")
+    _note("surrogate-built grid propagated beside the CRP grid, crp_cu111_srp48_small, complex64 batch")
+    ref = spodvr_amd.Run(path, over, device="hip")
+    try:
+        grid, served = surrogate.propagation_grid(ref, net, crp, tolerance=0.01)
+        sur = spodvr_amd.Run(path, over, device="hip", potential=grid)
+        try:
+            spodvr_amd.run_batch([ref, sur], 10 ** 9)
+            a, b = ref.results(), sur.results()
+        finally:
+            sur.close()
+    finally:
+        ref.close()
+    dr = float(np.max(np.abs(a["p_reaction"] - b["p_reaction"])))
+    dp = float(np.max(np.abs(a["p"] - b["p"])))
+    _note(f"network-served share of the grid {served.mean():.3f}; max |dP_reaction| {dr:.3e}, max |dP| {dp:.3e}, "
+          f"P_reaction up to {np.max(a['p_reaction']):.3f}")
+    assert served.mean() > 0.05
+    assert 0.0 < dr <= 2e-3 and dp <= 3e-3  # dr > 0: the runs did propagate different grids
