@@ -114,7 +114,7 @@ def sample_points(parameters: dict, n: int, seed: int, z_range=None, r_range=Non
                             rng.uniform(0, 2 * math.pi, n)])
 
 
-def train(net, crp, n: int = 1 << 19, seed: int = 1, schedule=((2e-3, 6000), (1e-3, 6000), (3e-4, 6000))):
+def train(net, crp, n: int = 1 << 21, seed: int = 1, schedule=((2e-3, 12000), (1e-3, 12000), (3e-4, 12000))):
     """Trains `net` (spodvr_amd.Mlp) on the remainders of n CRP energies at random geometries (the molecule energy
     from the CRP on the GPU, the atom energies on the host); returns the batch losses of every step."""
     par = crp.parameters()

@@ -70,7 +70,7 @@ def test_training_on_crp_energies_converges():
 
 @pytest.fixture(scope="module")
 def trained_surrogate():
-    """The surrogate trained once (surrogate.train: 2^19 CRP remainders, 18000 Adam steps of 4096)."""
+    """The surrogate trained once (surrogate.train: 2^21 CRP remainders, 36000 Adam steps of 4096)."""
     import spodvr_amd
     from spodvr_amd import surrogate
     _note("training the surrogate, SRP48 data set")
@@ -85,7 +85,7 @@ def trained_surrogate():
 
 def test_surrogate_accuracy_on_held_out_geometries(trained_surrogate):
     """The trained network (its remainder plus the two atom energies) against the CRP at 65536 held-out random
-    geometries, both clamped as the propagation clamps its grid. Measured on MI355X: RMS 0.021 eV below 0 eV, 73 %
+    geometries, both clamped as the propagation clamps its grid. Measured on MI355X: RMS 0.021 eV below 0 eV, 78 %
     of those points within 10 meV and 98 % within 50 meV. Learning V itself with the same six features reached an RMS
     of 0.40 eV."""
     from spodvr_amd import surrogate
@@ -108,9 +108,9 @@ def test_surrogate_grid_with_crp_fallback(trained_surrogate, tolerance, least_se
     """The surrogate on the potential grid of crp_cu111_srp48_small's shape (6 x 6 (x, y), 3 x 10 orientations,
     86 Z x 48 r) against the full CRP grid, both clamped to the propagation's [VMin, VMax]: the slabs it does not trust
     hold the CRP's values exactly, the values it serves stay near the trust tolerance (a slab is trusted from every
-    16th point), and it serves a measured share of the grid. Measured on MI355X: 12 % of the points at 10 meV, 62 %
-    at 50 meV (73 % at 0.1 eV). The CRP on the GPU builds the same grid in 0.03 s, against 1.7 s for the surrogate
-    path, whose features and atom energies are computed on the host."""
+    16th point), and it serves a measured share of the grid. Measured on MI355X: 24 % of the points at 10 meV, 65 %
+    at 50 meV. The CRP on the GPU builds the same grid in 0.03 s, against 1.7 s for the surrogate path, whose
+    features and atom energies are computed on the host."""
     import math
     import time
 
@@ -144,9 +144,11 @@ def test_surrogate_grid_propagated_against_the_crp_grid(trained_surrogate):
     """The surrogate's potential grid propagated: crp_cu111_srp48_small's grid built by the network where it is
     trusted at 10 meV (the CRP-built values elsewhere, surrogate.propagation_grid) and the CRP-built grid, both runs
     in complex64 stepped together as one batch to the end; the reaction and channel probabilities of the two are
-    compared, the CRP run being the judge. Measured on MI355X: the network gives 10 % of the grid's entries;
-    max |dP_reaction| 8.5e-4 (P_reaction up to 0.27) and max |dP| 1.5e-3. The bound is about twice that: a surrogate
-    trusted at 10 meV moves the probabilities by ~1e-3, ten times the complex64 mode's own error."""
+    compared, the CRP run being the judge. Measured on MI355X: the network gives 21 % of the grid's entries;
+    max |dP_reaction| 1.9e-3 (P_reaction up to 0.27) and max |dP| 7.7e-3. The bounds are about twice that: a
+    surrogate trusted at 10 meV moves the probabilities by ~1e-3 to 1e-2, far more than the complex64 mode's own
+    error (< 1e-4). (Trained on 2^19 points for half the steps, it gave 10 % of the grid and moved P_reaction by
+    8.5e-4 and P by 1.5e-3: the share the network serves and the probability error grow together.)"""
     import spodvr_amd
     from spodvr_amd import surrogate
     crp, net = trained_surrogate
@@ -169,4 +171,4 @@ def test_surrogate_grid_propagated_against_the_crp_grid(trained_surrogate):
     _note(f"network-served share of the grid {served.mean():.3f}; max |dP_reaction| {dr:.3e}, max |dP| {dp:.3e}, "
           f"P_reaction up to {np.max(a['p_reaction']):.3f}")
     assert served.mean() > 0.05
-    assert 0.0 < dr <= 2e-3 and dp <= 3e-3  # dr > 0: the runs did propagate different grids
+    assert 0.0 < dr <= 4e-3 and dp <= 1.5e-2  # dr > 0: the runs did propagate different grids
