@@ -57,16 +57,6 @@ int guarded(F&& f) {
 
 void copy_complex(const std::vector<spo::cplx>& v, double* out) { std::memcpy(out, v.data(), v.size() * sizeof(spo::cplx)); }
 
-}  // namespace
-
-extern "C" {
-
-const char* spo_last_error(void) { return g_error.c_str(); }
-
-}  // extern "C"
-
-namespace {
-
 // spo_run_create, with the set-up's potential grid replaced by `potential` (n values) when it is not NULL.
 int create_run(const char* input_path, const char* overrides_json, int device, spo_backend_factory_fn factory,
                const double* potential, size_t n_potential, spo_run** out) {
@@ -129,6 +119,8 @@ int create_run(const char* input_path, const char* overrides_json, int device, s
 }  // namespace
 
 extern "C" {
+
+const char* spo_last_error(void) { return g_error.c_str(); }
 
 int spo_run_create(const char* input_path, const char* overrides_json, int device, spo_backend_factory_fn factory,
                    spo_run** out) {
