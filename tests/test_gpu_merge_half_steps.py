@@ -97,3 +97,19 @@ def test_merged_runs_are_bitwise_identical():
     b = _steps_from_seeded_wave("cu111_dailight_short_sym", "fp64", True, 4)
     for k in a:
         assert np.array_equal(a[k], b[k]), k
+
+
+def test_switch_values_other_than_0_and_1_are_refused():
+    import spodvr_amd
+    print("case hardwall_short: SPO_HIP_MERGE_HALF_STEPS=2 at run creation", flush=True)
+    path, overrides = _case_input("hardwall_short")
+    old = os.environ.get("SPO_HIP_MERGE_HALF_STEPS")
+    os.environ["SPO_HIP_MERGE_HALF_STEPS"] = "2"
+    try:
+        with pytest.raises(spodvr_amd.SpoError, match="SPO_HIP_MERGE_HALF_STEPS must be 0 or 1"):
+            spodvr_amd.Run(path, overrides or None, device="hip")
+    finally:
+        if old is None:
+            os.environ.pop("SPO_HIP_MERGE_HALF_STEPS", None)
+        else:
+            os.environ["SPO_HIP_MERGE_HALF_STEPS"] = old
