@@ -272,7 +272,7 @@ def main() -> int:
         return 0
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the propagation of one run is not sharded "
-              "across GPUs in this build", file=sys.stderr)
+              "across GPUs", file=sys.stderr)
         return 2
 
     from tests import cases

@@ -57,7 +57,9 @@ struct InputParameters {
   int j_max = 6, mj_max = -1;
   int projection_operator = 2;
   // Restart (src/input/config.hpp:67, :96-99, :179-182): NewCalc = false continues the run whose checkpoint is in
-  // RestartDir, from tStart (native files; the Fortran-layout ones are not read or written by this build).
+  // RestartDir, from tStart: native files by default; RestartFormat / CheckpointFormat "fortran" (or "both" for
+  // writing) select the Fortran program's fort.8 / fort.22 (else fort.9 / fort.21) on reading and fort.9 / fort.21 on
+  // writing.
   bool new_calculation = true;
   double t_start = 0.0;
   std::string restart_dir;
@@ -130,7 +132,7 @@ struct Config {
 SystemParameters system_parameters_for(const std::string& system_id, const std::string& crp_data_set = "");
 SystemParameters system_parameters_for(const struct InputParameters& input);
 
-// The canonical spelling of a SystemID this build knows, matched without case as the reference compares SystemIDs
+// The canonical spelling of a SystemID the set-up knows, matched without case as the reference compares SystemIDs
 // (src/input/json_input.cpp:419-420); other names are returned unchanged.
 std::string canonical_system_id(const std::string& id);
 

@@ -55,7 +55,7 @@ void Propagation::restore(const std::string& dir) {
   const RunSetup& s = *setup_;
   const Config& c = s.config;
   if (!make_energy_resolved_setup(c, s.gas).site_ixy.empty())
-    throw NotSupportedError("a restart with EnergyResolvedSites (the earlier run's sums) is not supported by this build");
+    throw NotSupportedError("a restart with EnergyResolvedSites (the earlier run's sums) is not supported");
   const int n_phi = s.angular.n_phi();
   const std::filesystem::path d(dir);
   auto add_record = [&](double t, const std::vector<cplx>& bk) {

@@ -23,7 +23,7 @@ enum {
   SPO_HIP_OK = 0,
   SPO_HIP_E_DEVICE = 1,        /* HIP runtime error (reference: CudaError -> exit 1, src/cuda/cuda_error.hpp:10-13) */
   SPO_HIP_E_ARGUMENT = 2,      /* inconsistent shapes or tables */
-  SPO_HIP_E_NOT_SUPPORTED = 3  /* no usable device, too little memory, or a shape this build cannot run
+  SPO_HIP_E_NOT_SUPPORTED = 3  /* no usable device, too little memory, or a shape the library cannot run
                                   (reference: NotSupportedError -> exit 3, src/app/backend_factory.cpp:27, :52) */
 };
 

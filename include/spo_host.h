@@ -17,7 +17,7 @@ enum {
   SPO_OK = 0,
   SPO_E_OTHER = 1,         /* device or other runtime error */
   SPO_E_INPUT = 2,         /* invalid input */
-  SPO_E_NOT_SUPPORTED = 3, /* input or device not supported by this build */
+  SPO_E_NOT_SUPPORTED = 3, /* input or device not supported by the library */
   SPO_E_CHECKPOINT = 4     /* a checkpoint or BK file missing, corrupt or written for another input */
 };
 
